@@ -1,0 +1,61 @@
+"""Pins the distance / candidate-list oracle with the reference's known answers."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.distances import candidate_boundary_ties, compute_candidate_list, compute_distance_matrix
+
+
+def test_three_point_euc2d_known_answer():
+  # reference tests/distances/test_unit_distance_matrix.py:29,97
+  pts = np.array([[0.0, 0.0], [3.1, 4.0], [1.5, 2.5]])
+  assert compute_distance_matrix(pts).tolist() == [[0, 5, 3], [5, 0, 2], [3, 2, 0]]
+
+
+def test_berlin52_known_distances(berlin52_coords):
+  # reference tests/distances/test_unit_distance_matrix.py:294-313
+  d = compute_distance_matrix(berlin52_coords)
+  assert d.shape == (52, 52) and d.dtype == np.int32
+  assert d[0, 1] == 666 and d[0, 2] == 281 and d[1, 2] == 649
+  assert np.array_equal(d, d.T) and not d.diagonal().any()
+
+
+def test_nint_closed_form_on_integer_coordinates():
+  rng = np.random.default_rng(42)
+  xy = rng.integers(0, 1_000_001, (300, 2)).astype(np.float64)
+  d = compute_distance_matrix(xy)
+  ix = xy.astype(np.int64)
+  s = ((ix[:, None, :] - ix[None, :, :]) ** 2).sum(axis=2)
+  # exact integer nint: the integer r with (r - 1/2)^2 <= s < (r + 1/2)^2, i.e. 4s in [(2r-1)^2, (2r+1)^2)
+  r = np.floor(np.sqrt(s.astype(np.float64))).astype(np.int64)
+  r = np.where((2 * r + 1) ** 2 <= 4 * s, r + 1, r)
+  np.fill_diagonal(r, 0)
+  assert np.array_equal(d.astype(np.int64), r)
+
+
+def test_chunked_path_matches_direct():
+  rng = np.random.default_rng(3)
+  xy = rng.integers(0, 1000, (2100, 2)).astype(np.float64)
+  d = compute_distance_matrix(xy)
+  sub = compute_distance_matrix(xy[:700])
+  assert np.array_equal(d[:700, :700], sub)
+
+
+def test_candidate_list_brute_force_without_boundary_ties():
+  rng = np.random.default_rng(0)
+  xy = rng.integers(0, 1_000_001, (400, 2)).astype(np.float64)
+  d = compute_distance_matrix(xy)
+  for k in (5, 20):
+    assert candidate_boundary_ties(d, k) == 0
+    got = compute_candidate_list(d, k)
+    dd = d.astype(np.int64)
+    np.fill_diagonal(dd, np.iinfo(np.int64).max)
+    want = np.argsort(dd, axis=1, kind="stable")[:, :k]
+    assert np.array_equal(got, want)
+
+
+def test_candidate_list_clamps_and_empty():
+  d = compute_distance_matrix(np.array([[0.0, 0.0], [1.0, 0.0], [5.0, 0.0]]))
+  assert compute_candidate_list(d, 20).tolist() == [[1, 2], [0, 2], [1, 0]]
+  assert compute_candidate_list(d[:1, :1], 3).shape == (1, 0)

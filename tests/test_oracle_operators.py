@@ -1,0 +1,114 @@
+"""Pins the GA-operator, RNG and initialisation oracles with hand-worked answers."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import init as oinit
+from oracle import operators as ops
+from oracle.distances import compute_distance_matrix
+
+
+def test_tour_cost_known_answers():
+  # reference tests/algorithms/strategies/evaluation/test_unit_cost_evaluation.py:21-53
+  d = np.array([[0, 2, 9, 10], [2, 0, 6, 4], [9, 6, 0, 3], [10, 4, 3, 0]], dtype=np.int32)
+  got = ops.tour_costs(np.array([[0, 1, 2, 3], [0, 2, 1, 3], [1, 3, 2, 0]], dtype=np.int32), d)
+  assert got.tolist() == [21.0, 29.0, 18.0]
+  assert ops.tour_costs(np.array([[0]], dtype=np.int32), d).tolist() == [0.0]
+  assert got.dtype == np.float64
+
+
+def test_pmx_hand_worked_single_and_chained_mapping():
+  p1 = np.array([0, 1, 2, 3, 4, 5, 6, 7], dtype=np.int32)
+  p2 = np.array([3, 7, 5, 1, 6, 0, 2, 4], dtype=np.int32)
+  assert ops.pmx_pair(p1, p2, 3, 5).tolist() == [1, 7, 0, 3, 4, 5, 2, 6]
+  q1 = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+  q2 = np.array([1, 2, 0, 4, 3], dtype=np.int32)
+  assert ops.pmx_pair(q1, q2, 1, 2).tolist() == [0, 1, 2, 4, 3]
+  # cuts given in either order
+  assert ops.pmx_pair(q1, q2, 2, 1).tolist() == [0, 1, 2, 4, 3]
+
+
+def test_pmx_batch_is_permutation_and_keeps_segment():
+  rng = np.random.default_rng(4)
+  n, p = 50, 9
+  pop = np.stack([rng.permutation(n) for _ in range(p)]).astype(np.int32)
+  lo, hi = ops.pmx_draw(np.random.default_rng(1), p, n)
+  kids = ops.pmx_apply(pop, lo, hi)
+  for r in range(p):
+    assert sorted(kids[r].tolist()) == list(range(n))
+  for k in range(p // 2):
+    assert np.array_equal(kids[2 * k, lo[k] : hi[k] + 1], pop[2 * k, lo[k] : hi[k] + 1])
+    assert np.array_equal(kids[2 * k + 1, lo[k] : hi[k] + 1], pop[2 * k + 1, lo[k] : hi[k] + 1])
+  assert np.array_equal(kids[-1], pop[-1])  # odd row copied
+
+
+def test_distinct_pairs_draw_order_and_range():
+  a = ops.sample_distinct_pairs(np.random.default_rng(0), 1000, 0, 7)
+  assert (a[:, 0] != a[:, 1]).all() and a.min() >= 0 and a.max() < 7
+  rng = np.random.default_rng(0)
+  first = rng.integers(0, 7, size=1000, dtype=np.int32)
+  sh = rng.integers(0, 6, size=1000, dtype=np.int32)
+  assert np.array_equal(a[:, 0], first) and np.array_equal(a[:, 1], sh + (sh >= first))
+  with pytest.raises(ValueError):
+    ops.sample_distinct_pairs(np.random.default_rng(0), 3, 0, 1)
+
+
+def test_inversion_reverses_inclusive_segment_where_prob_below_rate():
+  pop = np.tile(np.arange(6, dtype=np.int32), (3, 1))
+  out = ops.inversion_apply(pop, np.array([0.01, 0.9, 0.0]), np.array([4, 1, 0]), np.array([1, 3, 5]), 0.05)
+  assert out.tolist() == [[0, 4, 3, 2, 1, 5], [0, 1, 2, 3, 4, 5], [5, 4, 3, 2, 1, 0]]
+  assert pop.tolist()[0] == [0, 1, 2, 3, 4, 5]
+
+
+def test_tournament_first_minimum_wins():
+  fit = np.array([5.0, 1.0, 1.0, 0.5])
+  cand = np.array([[0, 1, 2], [2, 1, 0], [3, 3, 1]], dtype=np.int32)
+  assert ops.tournament_winners(cand, fit).tolist() == [1, 2, 3]
+
+
+def test_survivors_stable_ties_prefer_old_population():
+  old = np.array([[0, 1, 2], [1, 2, 0]], dtype=np.int32)
+  new = np.array([[2, 1, 0], [0, 2, 1]], dtype=np.int32)
+  pop, fit, idx = ops.survivors(old, np.array([3.0, 1.0]), new, np.array([1.0, 0.5]), 3)
+  assert idx.tolist() == [3, 1, 2] and fit.tolist() == [0.5, 1.0, 1.0]
+  assert pop.tolist() == [[0, 2, 1], [1, 2, 0], [2, 1, 0]]
+
+
+def test_splitmix64_published_vector():
+  # SplitMix64 from state 0: 0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F
+  s = 0
+  outs = []
+  for _ in range(3):
+    s, z = oinit.splitmix64_next_scalar(s)
+    outs.append(z)
+  assert outs == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+  st, z = oinit.splitmix64_step(np.array([0], dtype=np.uint64))
+  assert int(z[0]) == 0xE220A8397B1DCDAF
+
+
+def test_random_init_matches_scalar_fisher_yates():
+  seeds = oinit.draw_row_seeds(np.random.default_rng(42), 5)
+  pop = oinit.random_init(np.arange(30), 5, seeds)
+  for r in range(5):
+    row = list(range(30))
+    s = int(seeds[r])
+    for i in range(29, 0, -1):
+      s, j = oinit.splitmix64_bounded_scalar(s, i + 1)
+      row[i], row[j] = row[j], row[i]
+    assert pop[r].tolist() == row
+
+
+def test_nn_init_rows_are_permutations_and_follow_neighbours(berlin52_coords):
+  from oracle.distances import compute_candidate_list
+
+  d = compute_distance_matrix(berlin52_coords)
+  cand = compute_candidate_list(d, 5)
+  seeds = oinit.draw_row_seeds(np.random.default_rng(42), 8)
+  pop = oinit.nn_init(np.arange(52), 8, cand, seeds)
+  for r in range(8):
+    assert sorted(pop[r].tolist()) == list(range(52))
+  # most steps go to one of the 5 nearest neighbours
+  hits = sum(int(pop[r, s + 1] in cand[pop[r, s]]) for r in range(8) for s in range(51))
+  assert hits > 0.7 * 8 * 51
