@@ -1,0 +1,227 @@
+"""Benchmark of the memetic-TSP hot path on one MI355X.
+
+Headline (value): population-batched 2-opt deltas/s on n=1000 integer coordinates uniform in
+[0, 10^6]^2 (seed 42), int32 nint distances, P=1024 SplitMix64 random permutations, best
+improvement over the full O(n^2) neighbourhood, every tour iterated to its local optimum.
+One step = one launch refining all 1024 tours from the same start population.
+
+Also reported on the same line: the same workload with candidate_k=20 lists, and GA
+generations/s of the fused memetic generation (n=1000 clustered, P=1024, tournament 5, PMX,
+inversion 0.05, elite-16 2-opt k=20 max_iter 10) over 200 generations.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+BYTES_PER_DELTA = 8  # two int32 distance gathers per 2-opt delta
+N_CITIES = 1000
+POP = 1024
+
+
+def c2_inputs(torch, native):
+  from tsp_gpu_clean_amd.rng import draw_row_seeds
+
+  xy = np.random.default_rng(42).integers(0, 1_000_001, (N_CITIES, 2)).astype(np.float64)
+  D = torch.empty((N_CITIES, N_CITIES), dtype=torch.int32, device="cuda")
+  native.dist_euc2d(torch.from_numpy(xy).cuda(), D)
+  seeds = draw_row_seeds(np.random.default_rng(42), POP)
+  tours = torch.empty((POP, N_CITIES), dtype=torch.int32, device="cuda")
+  native.random_init(torch.from_numpy(seeds.view(np.int64)).cuda(),
+                     torch.arange(N_CITIES, dtype=torch.int32, device="cuda"), tours)
+  return xy, D, tours
+
+
+def c3_coords():
+  rng = np.random.default_rng(42)
+  centres = rng.uniform(0, 1e6, (20, 2))
+  lab = rng.integers(0, 20, N_CITIES)
+  xy = centres[lab] + rng.normal(0, 0.02 * 1e6, (N_CITIES, 2))
+  return np.rint(np.clip(xy, 0, 1e6))
+
+
+def time_two_opt(torch, native, D, start, cand, steps, warmup):
+  """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch)."""
+  work = torch.empty_like(start)
+  stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  for _ in range(warmup):
+    work.copy_(start)
+    native.two_opt(work, D, cand, 0, -1, stats)
+  torch.cuda.synchronize()
+  stats.zero_()
+  stream = torch.cuda.current_stream()
+  ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+  t0 = time.perf_counter()
+  for i in range(steps):
+    work.copy_(start)
+    ev[i][0].record(stream)
+    native.two_opt(work, D, cand, 0, -1, stats)
+    ev[i][1].record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  kernel_ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+  s = stats.cpu().tolist()
+  return wall / steps, s[0] / steps, s[1] / steps, kernel_ms, work
+
+
+def time_generations(torch, native, generations=200, warmup=5):
+  from tsp_gpu_clean_amd import Problem, ProblemContext
+  from tsp_gpu_clean_amd.engine import DeviceMemeticLoop, draw_generations, upload
+  from tsp_gpu_clean_amd.rng import draw_row_seeds
+
+  ctx = ProblemContext(Problem.from_coordinates("c3", c3_coords()), seed=42)
+  seeds = draw_row_seeds(ctx.rng, POP)
+  pop0 = torch.empty((POP, N_CITIES), dtype=torch.int32, device="cuda")
+  native.random_init(torch.from_numpy(seeds.view(np.int64)).cuda(),
+                     torch.arange(N_CITIES, dtype=torch.int32, device="cuda"), pop0)
+  draws = upload(draw_generations(ctx.rng, generations + warmup, POP, N_CITIES, 5))
+  loop = DeviceMemeticLoop(ctx, POP, 0.05, tournament_k=5, elite_count=16, ls_max_iterations=10, ls_candidate_k=20)
+  loop.load(pop0)
+  for g in range(warmup):
+    loop.step(draws, g)
+  torch.cuda.synchronize()
+  best = torch.empty(generations, dtype=torch.int32, device="cuda")
+  t0 = time.perf_counter()
+  for g in range(generations):
+    loop.step(draws, warmup + g, best[g : g + 1])
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  best_cost = float(loop.fitness.min().item())
+  return generations / wall, wall, best_cost
+
+
+def cpu_baseline(seconds_hint: float = 20.0) -> dict:
+  """Reference-faithful NumPy 2-opt (single core) on a bounded sample of the C2 workload."""
+  code = f"""
+import os, sys, time, json
+os.sched_setaffinity(0, {{sorted(os.sched_getaffinity(0))[0]}})
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import two_opt as o2, init as oinit
+from oracle.distances import compute_distance_matrix
+xy = np.random.default_rng(42).integers(0, 1_000_001, ({N_CITIES}, 2)).astype(np.float64)
+d = compute_distance_matrix(xy)
+tours = oinit.random_init(np.arange({N_CITIES}), {POP}, oinit.draw_row_seeds(np.random.default_rng(42), {POP}))
+c = o2.new_counter()
+t0 = time.perf_counter(); rows = 0
+while rows < {POP} and (time.perf_counter() - t0) < {seconds_hint}:
+  o2.improve_batch(tours[rows:rows + 1], d, None, "best", 10**9, c); rows += 1
+wall = time.perf_counter() - t0
+print(json.dumps({{"deltas": c.deltas, "wall": wall, "rows": rows, "passes": c.passes}}))
+"""
+  env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+  out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600, check=True)
+  r = json.loads(out.stdout.strip().splitlines()[-1])
+  cpu = "unknown"
+  try:
+    with open("/proc/cpuinfo", encoding="utf-8") as fh:
+      cpu = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "unknown")
+  except OSError:
+    pass
+  return {
+    "value": r["deltas"] / r["wall"],
+    "unit": "deltas/s",
+    "cores": 1,
+    "kind": "port",
+    "sample": (f"{r['rows']} tour(s) of the same workload (n=1000, full neighbourhood, best, to local optimum; "
+               f"{r['passes']} passes, {r['deltas']} deltas in {r['wall']:.1f} s) with the NumPy restatement of "
+               f"the reference CPU scan, 1 thread pinned to 1 core of {cpu}; host has "
+               f"{len(os.sched_getaffinity(0))} usable cores"),
+  }
+
+
+def load_traffic():
+  path = os.path.join(ROOT, "profiles", "two_opt_c2_traffic.json")
+  if not os.path.exists(path):
+    return None
+  with open(path, encoding="utf-8") as fh:
+    return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def main() -> int:
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--no-cpu-baseline", action="store_true")
+  ap.add_argument("--no-extras", action="store_true")
+  ap.add_argument("--generations", type=int, default=200)
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(f"bench.py measures one MI355X only: --gpus {args.gpus} is not supported (every GPU box here has one GPU; "
+          "tours shard by population with no exchange, see DESIGN.md).", file=sys.stderr)
+    return 2
+
+  import torch
+
+  from tsp_gpu_clean_amd import _native
+
+  _native.require_device()
+  print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
+  _, D, start = c2_inputs(torch, _native)
+  sec, deltas, moves, kms, _ = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
+  print(f"bench: full neighbourhood {deltas / sec:.3e} deltas/s, {kms:.1f} ms/launch", file=sys.stderr, flush=True)
+  value = deltas / sec
+  achieved = deltas * BYTES_PER_DELTA / (kms * 1e-3) / 1e9
+  line = {
+    "metric": "2-opt deltas/s & gens/s (n=1000,pop=1024); time-to-2% gap, learned vs plain kNN",
+    "value": value,
+    "unit": "deltas/s",
+    "n_gpus": 1,
+    "steps": args.steps,
+    "warmup": args.warmup,
+    "ms_per_step": sec * 1e3,
+    "higher_is_better": True,
+    "scaling": "weak",
+    "vs_baseline": None,
+    "dtype": "int32",
+    "data": "synthetic",
+    "config": {
+      "workload": "batched 2-opt best-improvement, full O(n^2) neighbourhood, to local optimum",
+      "n": N_CITIES, "population": POP, "coords": "uniform integer [0,1e6]^2 seed 42",
+      "init": "SplitMix64 random permutations (seeds from default_rng(42))",
+      "deltas_per_step": deltas, "moves_per_step": moves,
+    },
+    "roofline": {
+      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+      "traffic": load_traffic(),
+      "kernel": "tspmi::two_opt_kernel", "kernel_ms_per_launch": kms,
+      "bytes_per_unit": BYTES_PER_DELTA, "units_per_launch": deltas,
+    },
+  }
+  if not args.no_extras:
+    print("bench: candidate_k=20", file=sys.stderr, flush=True)
+    cand = torch.empty((N_CITIES, 20), dtype=torch.int32, device="cuda")
+    _native.knn(D, 20, cand)
+    s20, d20, _, k20ms, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1)
+    line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
+                             "deltas_per_step": d20}
+    print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)
+    gps, gwall, gbest = time_generations(torch, _native, args.generations)
+    line["gens_per_s"] = gps
+    line["ga"] = {"generations": args.generations, "wall_s": gwall, "best_cost": gbest,
+                  "workload": "n=1000 clustered (20 Gaussian clusters, sigma 2% of 1e6), P=1024, tournament 5, PMX, "
+                              "inversion 0.05, elite-16 2-opt best k=20 max_iter 10"}
+  if not args.no_cpu_baseline:
+    print("bench: CPU baseline (NumPy restatement, 1 core)", file=sys.stderr, flush=True)
+    line["cpu_baseline"] = cpu_baseline()
+  print(json.dumps(line), flush=True)
+  return 0
+
+
+if __name__ == "__main__":
+  sys.exit(main())

@@ -1,0 +1,164 @@
+/*
+ * tspmi — C ABI of the MI355X-native memetic-TSP hot path (libtspmi.so, gfx950).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: TSPMI_OK (0) or a negative code; the text of
+ *     the last failure on the calling thread is returned by tspmi_last_error().
+ *   - All array arguments are caller-owned DEVICE pointers (HIP/ROCm global memory),
+ *     row-major, contiguous.  Sizes are plain ints.  `stream` is a hipStream_t passed as
+ *     void* (NULL = the default stream); every call only enqueues work on that stream and
+ *     never synchronises the host, except tspmi_device_count().
+ *   - The library never allocates device memory behind the caller's back except the
+ *     per-call scratch that an entry point documents.
+ *   - Tours are permutations of 0..n-1 stored as int32 (P, n); distance matrices are int32
+ *     (n, ld) with 0 <= D < 2^29 so every 2-opt delta fits an int32.
+ *
+ * Each entry point cites the reference (lucasgaldinos/tsp_gpu_clean) interface it replaces.
+ */
+#ifndef TSPMI_H
+#define TSPMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSPMI_OK 0
+#define TSPMI_EINVAL (-1) /* bad argument (shape, mode, null pointer, size limit) */
+#define TSPMI_EHIP (-2)   /* HIP runtime error (launch, device) */
+#define TSPMI_ENODEV (-3) /* no HIP device */
+
+#define TSPMI_MODE_BEST 0
+#define TSPMI_MODE_FIRST 1
+
+/* Library identification and device probe. */
+int tspmi_version(void);
+const char* tspmi_last_error(void);
+int tspmi_device_count(int* count);
+
+/*
+ * EUC_2D TSPLIB nint distances: D[u,v] = (int)(sqrt(dx^2 + dy^2) + 0.5) in FP64 with a
+ * correctly rounded sqrt, diagonal 0.  xy is (n, 2) float64.  Writes max(D) to *max_out
+ * (device int32, may be NULL).
+ * Replaces: src/distances/matrix.py:214-273 (compute_distance_matrix, EUC_2D branch :84-86)
+ *           and src/distances/kernels/pairwise_device.cuh:53-60 (gpu_nint / gpu_euc_2d).
+ */
+int tspmi_dist_euc2d(const double* xy, int n, int32_t* D, int ld, int32_t* max_out, void* stream);
+
+/*
+ * k-nearest-neighbour candidate table: out[u, r] = r-th nearest v != u in (D[u,v], v)
+ * order, k_eff = min(k, n-1) columns.  `ties_out` (device int32, may be NULL) receives the
+ * number of rows whose k-th and (k+1)-th distances tie (where NumPy's argpartition choice
+ * is implementation-defined).
+ * Replaces: src/distances/matrix.py:140-211 (compute_candidate_list).
+ */
+int tspmi_knn(const int32_t* D, int n, int ld, int k, int32_t* out, int32_t* ties_out, void* stream);
+
+/*
+ * Closed-tour cost: out[r] = (double) sum_x D[t_x, t_{x+1}] (+ D[t_{n-1}, t_0] if closed).
+ * Replaces: src/algorithms/strategies/evaluation/tour_cost.py:17-65 and
+ *           evaluation/kernels/tour_cost.cu:21 (tour_cost_kernel).
+ */
+int tspmi_tour_cost(const int32_t* tours, int P, int n, const int32_t* D, int ld, int closed,
+                    double* out, void* stream);
+
+/*
+ * Population-batched 2-opt on cyclic TSP tours, refined in place.
+ *   mode: TSPMI_MODE_BEST (steepest, ties -> lowest (i,j) / (i,r) scan index) or
+ *         TSPMI_MODE_FIRST (first improving pair in scan order).
+ *   cand: (n, k) candidate table or NULL/k=0 for the full neighbourhood.
+ *   max_iter: passes per tour; < 0 runs each tour to its local optimum.
+ *   stats: device int64[3] accumulated with {deltas evaluated, moves applied, passes}
+ *          (may be NULL).
+ * Replaces: src/algorithms/strategies/local_search/two_opt.py:831-899 (_improve_grid) with
+ *           kernels/two_opt_batch.cu:51 (two_opt_batch_kernel), one launch per call instead of
+ *           one launch per pass, and the CPU acceptance rules of two_opt.py:386-489.
+ */
+int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* cand, int k,
+                  int mode, int max_iter, int64_t* stats, void* stream);
+
+/*
+ * Tournament selection: out[r] = pop[w_r] with w_r = cand[r, argmin_c fitness[cand[r, c]]]
+ * (first minimum).  cand is (num_parents, k) int32 drawn on the host.
+ * Replaces: src/algorithms/strategies/selection/tournament_selection.py:78-128 and
+ *           selection/kernels/tournament_selection.cu:11.
+ */
+int tspmi_tournament(const int32_t* pop, int P, int n, const double* fitness, const int32_t* cand, int k,
+                     int num_parents, int32_t* out, int32_t* winners_out, void* stream);
+
+/*
+ * Partially mapped crossover of pairs (2q, 2q+1) with shared cuts [lo[q], hi[q]]
+ * (lo <= hi); an odd last row is copied.
+ * Replaces: src/algorithms/strategies/crossover/pmx_crossover.py:49-171 and
+ *           crossover/kernels/pmx_crossover.cu:19.
+ */
+int tspmi_pmx(const int32_t* parents, int P, int n, const int32_t* cut_lo, const int32_t* cut_hi,
+              int32_t* out, void* stream);
+
+/*
+ * Inversion mutation in place: rows with probs[r] < rate reverse [min(p1,p2), max(p1,p2)].
+ * Replaces: src/algorithms/strategies/mutation/inversion_mutation.py:65-214 and
+ *           mutation/kernels/inversion_mutation.cu:11.
+ */
+int tspmi_inversion(int32_t* routes, int P, int n, const double* probs, const int32_t* pos1,
+                    const int32_t* pos2, double rate, void* stream);
+
+/*
+ * Fused generation: tournament -> PMX -> inversion -> evaluate, one workgroup per child pair.
+ * Inputs are the host-drawn buffers of one generation: tour_cand (P, k), cut_lo/cut_hi (P/2),
+ * probs/pos1/pos2 (P).  Writes offspring (P, n) and their costs off_fit (P).
+ * Replaces: the SelectionStep -> CrossoverStep -> MutationStep -> EvaluationStep sequence of
+ *           src/algorithms/metaheuristics/genetic_algorithm.py:120-213.
+ */
+int tspmi_generation(const int32_t* pop, const double* fitness, int P, int n, const int32_t* D, int ld,
+                     const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                     const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
+                     int32_t* offspring, double* off_fit, void* stream);
+
+/*
+ * Stable rank selection: for the M keys, rank[i] = #{j : key[j] < key[i] or
+ * (key[j] == key[i] and j < i)}; writes order[rank[i]] = i for rank[i] < keep.
+ * With keys = concat(old_fit, new_fit) this is the (mu + lambda) survivor order of
+ * genetic_algorithm.py:618-706 (lexsort :689); with keys = fitness and keep = 16 it is the
+ * elite set of pipeline_steps.py:153-163.
+ */
+int tspmi_rank_select(const double* keys, int M, int keep, int32_t* order, void* stream);
+
+/*
+ * Row gather of two sources: out[d] = (src[d] < P0 ? A[src[d]] : B[src[d] - P0]) for d < rows,
+ * and out_fit[d] likewise from fitA/fitB (out_fit may be NULL).
+ * Replaces: the combined_pop[best_indices] gather of genetic_algorithm.py:628,706.
+ */
+int tspmi_gather_rows(const int32_t* A, const double* fitA, int P0, const int32_t* B, const double* fitB,
+                      int n, const int32_t* src, int rows, int32_t* out, double* out_fit, void* stream);
+
+/*
+ * Row scatter: dst[idx[r]] = src[r] (and dst_fit[idx[r]] = src_fit[r] when non-NULL).
+ * Replaces: the elite write-back of pipeline_steps.py:183-184.
+ */
+int tspmi_scatter_rows(const int32_t* src, const double* src_fit, const int32_t* idx, int rows, int n,
+                       int32_t* dst, double* dst_fit, void* stream);
+
+/*
+ * SplitMix64 Fisher-Yates permutations of `customers` (length n), one stream per row seed.
+ * Replaces: src/algorithms/strategies/initialization/random_init.py:24-118 and
+ *           initialization/kernels/random_init.cu:9.
+ */
+int tspmi_random_init(const uint64_t* seeds, int P, const int32_t* customers, int n, int32_t* out,
+                      void* stream);
+
+/*
+ * GRASP nearest-neighbour tours over a (n, k) candidate table with SplitMix64 draws.
+ * `work` is caller-owned device scratch of 2 * P * n int32.
+ * Replaces: src/algorithms/strategies/initialization/nearest_neighbor_init.py:31-238 and
+ *           initialization/kernels/nearest_neighbor_init.cu:10.
+ */
+int tspmi_nn_init(const uint64_t* seeds, int P, const int32_t* customers, int n, const int32_t* cand,
+                  int k, int32_t* work, int32_t* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSPMI_H */

@@ -1,0 +1,350 @@
+"""GPU parity of every HIP entry point against the oracle, called through the C ABI.
+
+Bar: bit-exact tours, int32 distances, candidate tables and float64 costs (integer sums).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import c_two_opt
+from oracle import init as oinit
+from oracle import operators as ops
+from oracle import two_opt as o2
+from oracle.distances import candidate_boundary_ties, compute_candidate_list, compute_distance_matrix
+from tests.conftest import lattice_matrix, nearest, rounded_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+  print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+  import torch
+
+  from tsp_gpu_clean_amd import _native
+
+  _native.require_device()
+  return torch
+
+
+@pytest.fixture(scope="module")
+def nat():
+  from tsp_gpu_clean_amd import _native
+
+  return _native
+
+
+def cu(a):
+  import torch
+
+  return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def uniform_coords(n, seed=42, hi=1_000_000):
+  return np.random.default_rng(seed).integers(0, hi + 1, (n, 2)).astype(np.float64)
+
+
+def clustered_coords(n, seed=42, box=1_000_000, clusters=20):
+  rng = np.random.default_rng(seed)
+  centres = rng.uniform(0, box, (clusters, 2))
+  lab = rng.integers(0, clusters, n)
+  xy = centres[lab] + rng.normal(0, 0.02 * box, (n, 2))
+  return np.rint(np.clip(xy, 0, box))
+
+
+def device_dist(nat, torch, xy):
+  n = xy.shape[0]
+  D = torch.empty((n, n), dtype=torch.int32, device="cuda")
+  mx = torch.zeros(1, dtype=torch.int32, device="cuda")
+  nat.dist_euc2d(cu(xy), D, mx)
+  return D, int(mx.item())
+
+
+# ---- distances and candidate tables -------------------------------------------------------------
+
+
+@pytest.mark.parametrize("kind", ["uniform1000", "clustered1000", "berlin52", "three_point", "float2100"])
+def test_dist_euc2d_matches_oracle(torch_mod, nat, berlin52_coords, kind):
+  say(f"dist_euc2d {kind}")
+  xy = {
+    "uniform1000": lambda: uniform_coords(1000),
+    "clustered1000": lambda: clustered_coords(1000),
+    "berlin52": lambda: berlin52_coords,
+    "three_point": lambda: np.array([[0.0, 0.0], [3.1, 4.0], [1.5, 2.5]]),
+    "float2100": lambda: np.random.default_rng(3).random((2100, 2)) * 1000.0,
+  }[kind]()
+  D, mx = device_dist(nat, torch_mod, xy)
+  want = compute_distance_matrix(xy)
+  np.testing.assert_array_equal(D.cpu().numpy(), want)
+  assert mx == int(want.max())
+
+
+@pytest.mark.parametrize("n,k", [(1000, 20), (1000, 32), (52, 5), (7, 20), (2, 3)])
+def test_knn_matches_oracle(torch_mod, nat, n, k):
+  say(f"knn n={n} k={k}")
+  xy = uniform_coords(n, seed=n + k)
+  D, _ = device_dist(nat, torch_mod, xy)
+  k_eff = min(k, n - 1)
+  out = torch_mod.empty((n, k_eff), dtype=torch_mod.int32, device="cuda")
+  ties = torch_mod.zeros(1, dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, k_eff, out, ties)
+  d = compute_distance_matrix(xy)
+  want = compute_candidate_list(d, k)
+  got = out.cpu().numpy()
+  assert int(ties.item()) == candidate_boundary_ties(d, k)
+  # every row follows (distance, node index) order ...
+  np.testing.assert_array_equal(got, nearest(d, k))
+  # ... which is the reference's table on every row without a k-th boundary tie
+  dd = d.astype(np.int64)
+  np.fill_diagonal(dd, np.iinfo(np.int64).max)
+  srt = np.sort(dd, axis=1)
+  clean = srt[:, k_eff - 1] != srt[:, k_eff] if k_eff < n - 1 else np.ones(n, bool)
+  np.testing.assert_array_equal(got[clean], want[clean])
+
+
+def test_knn_tie_rows_follow_distance_then_index(torch_mod, nat):
+  say("knn lattice ties")
+  d = lattice_matrix(49)
+  D = cu(d)
+  out = torch_mod.empty((49, 6), dtype=torch_mod.int32, device="cuda")
+  ties = torch_mod.zeros(1, dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, 6, out, ties)
+  np.testing.assert_array_equal(out.cpu().numpy(), nearest(d, 6))
+  assert int(ties.item()) == candidate_boundary_ties(d, 6)
+
+
+# ---- tour cost ---------------------------------------------------------------------------------------
+
+
+def test_tour_cost_known_answers_and_random(torch_mod, nat):
+  say("tour_cost")
+  d = np.array([[0, 2, 9, 10], [2, 0, 6, 4], [9, 6, 0, 3], [10, 4, 3, 0]], dtype=np.int32)
+  t = np.array([[0, 1, 2, 3], [0, 2, 1, 3], [1, 3, 2, 0]], dtype=np.int32)
+  out = torch_mod.empty(3, dtype=torch_mod.float64, device="cuda")
+  nat.tour_cost(cu(t), cu(d), out)
+  assert out.cpu().tolist() == [21.0, 29.0, 18.0]
+  xy = uniform_coords(1000)
+  dd = compute_distance_matrix(xy)
+  rng = np.random.default_rng(1)
+  tours = np.stack([rng.permutation(1000) for _ in range(37)]).astype(np.int32)
+  out = torch_mod.empty(37, dtype=torch_mod.float64, device="cuda")
+  nat.tour_cost(cu(tours), cu(dd), out)
+  np.testing.assert_array_equal(out.cpu().numpy(), ops.tour_costs(tours, dd))
+  nat.tour_cost(cu(tours), cu(dd), out, closed=False)
+  np.testing.assert_array_equal(out.cpu().numpy(), ops.tour_costs(tours, dd, return_to_depot=False))
+
+
+# ---- 2-opt -------------------------------------------------------------------------------------------
+
+
+def _run_two_opt(nat, torch, tours, d, cand, mode, iters):
+  t = cu(tours.astype(np.int32))
+  stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  nat.two_opt(t, cu(d), None if cand is None else cu(cand), {"best": 0, "first": 1}[mode], iters, stats)
+  return t.cpu().numpy(), stats.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("k", [None, 3, 20])
+@pytest.mark.parametrize("n", [4, 5, 6, 8, 10, 17, 40])
+def test_two_opt_small_matches_numpy_oracle(torch_mod, nat, mode, k, n):
+  say(f"two_opt small mode={mode} k={k} n={n}")
+  d = rounded_matrix(n, seed=n)
+  cand = None if k is None else nearest(d, k)
+  rng = np.random.default_rng(n * 100 + (k or 0))
+  tours = np.stack([rng.permutation(n) for _ in range(6)]).astype(np.int32)
+  for iters in (1, 3, 10):
+    got, _ = _run_two_opt(nat, torch_mod, tours, d, cand, mode, iters)
+    np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, mode, iters))
+
+
+@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("k", [None, 3])
+@pytest.mark.parametrize("n", [9, 16, 25, 64])
+def test_two_opt_lattice_ties_match_numpy_oracle(torch_mod, nat, mode, k, n):
+  say(f"two_opt lattice mode={mode} k={k} n={n}")
+  d = lattice_matrix(n)
+  cand = None if k is None else nearest(d, k)
+  tours = np.stack([np.random.default_rng(n * 31 + t).permutation(n) for t in range(4)]).astype(np.int32)
+  got, _ = _run_two_opt(nat, torch_mod, tours, d, cand, mode, 10)
+  np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, mode, 10))
+  got, _ = _run_two_opt(nat, torch_mod, tours, d, cand, mode, -1)
+  want, _ = c_two_opt.improve_batch(tours, d, cand, mode, -1)
+  np.testing.assert_array_equal(got, want)
+
+
+def test_two_opt_duplicate_candidates(torch_mod, nat):
+  say("two_opt duplicate candidates")
+  n = 30
+  d = rounded_matrix(n, seed=5)
+  cand = nearest(d, 6)
+  cand[:, 3] = cand[:, 1]  # repeated neighbours: pairs appear twice, first occurrence wins ties
+  tours = np.stack([np.random.default_rng(t).permutation(n) for t in range(8)]).astype(np.int32)
+  for mode in ("best", "first"):
+    got, _ = _run_two_opt(nat, torch_mod, tours, d, cand, mode, 20)
+    np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, mode, 20))
+
+
+@pytest.mark.parametrize("k", [None, 20])
+@pytest.mark.parametrize("mode", ["best", "first"])
+def test_two_opt_n1000_to_local_optimum_matches_c_oracle(torch_mod, nat, k, mode):
+  say(f"two_opt n=1000 to optimum k={k} mode={mode}")
+  xy = uniform_coords(1000)
+  d = compute_distance_matrix(xy)
+  cand = None if k is None else compute_candidate_list(d, k)
+  rows = 6 if k is None else 24
+  tours = oinit.random_init(np.arange(1000), rows, oinit.draw_row_seeds(np.random.default_rng(42), rows))
+  got, stats = _run_two_opt(nat, torch_mod, tours, d, cand, mode, -1)
+  want, wstats = c_two_opt.improve_batch(tours, d, cand, mode, -1)
+  np.testing.assert_array_equal(got, want)
+  assert stats[1] == wstats[1]  # same number of moves
+  np.testing.assert_array_equal(ops.tour_costs(got, d), ops.tour_costs(want, d))
+
+
+def test_two_opt_full_population_is_locally_optimal(torch_mod, nat):
+  """P=1024, n=1000, k=20 to local optimum: permutations, no improving candidate pair left."""
+  say("two_opt P=1024 n=1000 k=20 to optimum")
+  xy = uniform_coords(1000)
+  d = compute_distance_matrix(xy)
+  cand = compute_candidate_list(d, 20)
+  P = 1024
+  tours = oinit.random_init(np.arange(1000), P, oinit.draw_row_seeds(np.random.default_rng(42), P))
+  got, stats = _run_two_opt(nat, torch_mod, tours, d, cand, "best", -1)
+  assert (np.sort(got, axis=1) == np.arange(1000)).all()
+  assert (ops.tour_costs(got, d) < ops.tour_costs(tours, d)).all()
+  again, _ = c_two_opt.improve_batch(got, d, cand, "best", 1)
+  np.testing.assert_array_equal(again, got)
+  sample = [0, 511, 1023]
+  want, _ = c_two_opt.improve_batch(tours[sample], d, cand, "best", -1)
+  np.testing.assert_array_equal(got[sample], want)
+
+
+def test_two_opt_noop_cases(torch_mod, nat):
+  say("two_opt no-op cases")
+  d = rounded_matrix(3, seed=1)
+  t = np.array([[2, 0, 1]], dtype=np.int32)
+  got, stats = _run_two_opt(nat, torch_mod, t, d, None, "best", 10)
+  np.testing.assert_array_equal(got, t)
+  d = rounded_matrix(12, seed=1)
+  t = np.random.default_rng(0).permutation(12).astype(np.int32)[None]
+  got, _ = _run_two_opt(nat, torch_mod, t, d, None, "best", 0)
+  np.testing.assert_array_equal(got, t)
+
+
+# ---- GA operators ----------------------------------------------------------------------------------
+
+
+def _population(n, P, seed):
+  rng = np.random.default_rng(seed)
+  return np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+
+
+@pytest.mark.parametrize("P,n", [(64, 52), (1024, 1000), (7, 30)])
+def test_tournament_pmx_inversion_match_oracle(torch_mod, nat, P, n):
+  say(f"ga ops P={P} n={n}")
+  pop = _population(n, P, seed=P + n)
+  d = compute_distance_matrix(uniform_coords(n, seed=n))
+  fit = ops.tour_costs(pop, d)
+  fit[::5] = fit[1]  # ties: the first minimum must win
+  g = np.random.default_rng(11)
+  cand = ops.tournament_draw(g, P, P, 5)
+  sel = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  win = torch_mod.empty(P, dtype=torch_mod.int32, device="cuda")
+  nat.tournament(cu(pop), cu(fit), cu(cand), sel, win)
+  np.testing.assert_array_equal(win.cpu().numpy(), ops.tournament_winners(cand, fit))
+  np.testing.assert_array_equal(sel.cpu().numpy(), pop[ops.tournament_winners(cand, fit)])
+  lo, hi = ops.pmx_draw(g, P, n)
+  kids = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  nat.pmx(sel, cu(lo), cu(hi), kids)
+  want = ops.pmx_apply(pop[ops.tournament_winners(cand, fit)], lo, hi)
+  np.testing.assert_array_equal(kids.cpu().numpy(), want)
+  probs, p1, p2 = ops.inversion_draw(g, P, n)
+  nat.inversion(kids, cu(probs), cu(p1), cu(p2), 0.3)
+  np.testing.assert_array_equal(kids.cpu().numpy(), ops.inversion_apply(want, probs, p1, p2, 0.3))
+
+
+@pytest.mark.parametrize("P,n", [(128, 52), (1024, 1000), (9, 40), (1, 10)])
+def test_fused_generation_matches_oracle(torch_mod, nat, P, n):
+  say(f"fused generation P={P} n={n}")
+  pop = _population(n, P, seed=3 * P + n)
+  d = compute_distance_matrix(uniform_coords(n, seed=n + 1))
+  fit = ops.tour_costs(pop, d)
+  g = np.random.default_rng(5)
+  cand = ops.tournament_draw(g, P, P, 5)
+  lo, hi = ops.pmx_draw(g, P, n)
+  probs, p1, p2 = ops.inversion_draw(g, P, n)
+  if P < 2:
+    lo, hi = np.zeros(1, np.int32), np.zeros(1, np.int32)
+  off = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  off_fit = torch_mod.empty(P, dtype=torch_mod.float64, device="cuda")
+  nat.generation(cu(pop), cu(fit), cu(d), cu(cand), cu(lo), cu(hi), cu(probs), cu(p1), cu(p2), 0.5, off, off_fit)
+  sel = pop[ops.tournament_winners(cand, fit)]
+  kids = ops.inversion_apply(ops.pmx_apply(sel, lo, hi) if P >= 2 else sel, probs, p1, p2, 0.5)
+  np.testing.assert_array_equal(off.cpu().numpy(), kids)
+  np.testing.assert_array_equal(off_fit.cpu().numpy(), ops.tour_costs(kids, d))
+
+
+def test_rank_select_is_stable_survival(torch_mod, nat):
+  say("rank_select survival")
+  rng = np.random.default_rng(2)
+  old = rng.integers(0, 50, 1024).astype(np.float64)  # many ties
+  new = rng.integers(0, 50, 1024).astype(np.float64)
+  keys = np.concatenate([old, new])
+  order = torch_mod.empty(1024, dtype=torch_mod.int32, device="cuda")
+  nat.rank_select(cu(keys), 1024, order)
+  zeros = np.zeros_like(keys)
+  np.testing.assert_array_equal(order.cpu().numpy(), np.lexsort(np.vstack([keys, zeros, zeros]))[:1024])
+  order16 = torch_mod.empty(16, dtype=torch_mod.int32, device="cuda")
+  nat.rank_select(cu(old), 16, order16)
+  np.testing.assert_array_equal(order16.cpu().numpy(), np.argsort(old, kind="stable")[:16])
+
+
+def test_gather_and_scatter_rows(torch_mod, nat):
+  say("gather/scatter rows")
+  A = _population(20, 6, 1)
+  B = _population(20, 5, 2)
+  fa = np.arange(6, dtype=np.float64)
+  fb = 10 + np.arange(5, dtype=np.float64)
+  src = np.array([7, 0, 10, 3], dtype=np.int32)
+  out = torch_mod.empty((4, 20), dtype=torch_mod.int32, device="cuda")
+  of = torch_mod.empty(4, dtype=torch_mod.float64, device="cuda")
+  nat.gather_rows(cu(A), cu(fa), cu(B), cu(fb), cu(src), out, of)
+  both = np.vstack([A, B])
+  np.testing.assert_array_equal(out.cpu().numpy(), both[src])
+  np.testing.assert_array_equal(of.cpu().numpy(), np.concatenate([fa, fb])[src])
+  dst = cu(A.copy())
+  dfit = cu(fa.copy())
+  nat.scatter_rows(cu(B[:2]), cu(fb[:2]), cu(np.array([4, 1], np.int32)), dst, dfit)
+  want = A.copy()
+  want[[4, 1]] = B[:2]
+  np.testing.assert_array_equal(dst.cpu().numpy(), want)
+
+
+# ---- initialisers ----------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("P,n", [(1024, 1000), (3, 1), (5, 2)])
+def test_random_init_matches_oracle(torch_mod, nat, P, n):
+  say(f"random_init P={P} n={n}")
+  seeds = oinit.draw_row_seeds(np.random.default_rng(42), P)
+  out = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  nat.random_init(cu(seeds.view(np.int64)), cu(np.arange(n, dtype=np.int32)), out)
+  np.testing.assert_array_equal(out.cpu().numpy(), oinit.random_init(np.arange(n), P, seeds))
+
+
+@pytest.mark.parametrize("k", [5, 20])
+def test_nn_init_matches_oracle(torch_mod, nat, berlin52_coords, k):
+  say(f"nn_init k={k}")
+  d = compute_distance_matrix(berlin52_coords)
+  cand = compute_candidate_list(d, k)  # the kernel consumes the given table
+  P, n = 128, 52
+  seeds = oinit.draw_row_seeds(np.random.default_rng(42), P)
+  out = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  work = torch_mod.empty(2 * P * n, dtype=torch_mod.int32, device="cuda")
+  nat.nn_init(cu(seeds.view(np.int64)), cu(np.arange(n, dtype=np.int32)), cu(cand), work, out)
+  np.testing.assert_array_equal(out.cpu().numpy(), oinit.nn_init(np.arange(n), P, cand, seeds))

@@ -1,0 +1,287 @@
+"""ctypes binding of ``libtspmi.so`` — the only way the package reaches the device.
+
+Every wrapper takes torch CUDA (ROCm/HIP) tensors, checks shapes and dtypes on the host,
+passes raw device pointers plus the current HIP stream, and maps the library status to an
+exception: ``ValueError`` for a rejected argument, ``RuntimeError`` for a HIP failure and
+``GPUUnavailableError`` when the library or the device is missing.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Any
+
+from .errors import GPUUnavailableError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtspmi.so")
+
+_lib: Any = None
+_lock = threading.Lock()
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+# name -> argtypes (all return int status)
+_SIGNATURES: dict[str, list[Any]] = {
+  "tspmi_version": [],
+  "tspmi_device_count": [_P],
+  "tspmi_dist_euc2d": [_P, _I, _P, _I, _P, _P],
+  "tspmi_knn": [_P, _I, _I, _I, _P, _P, _P],
+  "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
+  "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
+  "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
+  "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
+  "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
+  "tspmi_generation": [_P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P],
+  "tspmi_rank_select": [_P, _I, _I, _P, _P],
+  "tspmi_gather_rows": [_P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P],
+  "tspmi_scatter_rows": [_P, _P, _P, _I, _I, _P, _P, _P],
+  "tspmi_random_init": [_P, _I, _P, _I, _P, _P],
+  "tspmi_nn_init": [_P, _I, _P, _I, _P, _I, _P, _P, _P],
+}
+
+EXPORTED = tuple(_SIGNATURES) + ("tspmi_last_error",)
+
+TSPMI_EINVAL = -1
+TSPMI_EHIP = -2
+TSPMI_ENODEV = -3
+
+
+def load_library(path: str | None = None) -> Any:
+  """Load and type the shared library (no device work)."""
+  global _lib
+  with _lock:
+    if _lib is not None and path is None:
+      return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+      raise GPUUnavailableError(
+        f"libtspmi.so not found at {p}; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+      )
+    try:
+      lib = ctypes.CDLL(p)
+    except OSError as exc:  # missing HIP runtime etc.
+      raise GPUUnavailableError(f"cannot load {p}: {exc}") from exc
+    for name, argtypes in _SIGNATURES.items():
+      fn = getattr(lib, name)
+      fn.argtypes = argtypes
+      fn.restype = ctypes.c_int
+    lib.tspmi_last_error.argtypes = []
+    lib.tspmi_last_error.restype = ctypes.c_char_p
+    if path is None:
+      _lib = lib
+    return lib
+
+
+def _check(rc: int, fn: str) -> None:
+  if rc == 0:
+    return
+  msg = (_lib.tspmi_last_error() or b"").decode(errors="replace")
+  if rc == TSPMI_EINVAL:
+    raise ValueError(f"{fn}: {msg}")
+  if rc == TSPMI_ENODEV:
+    raise GPUUnavailableError(f"{fn}: {msg}")
+  raise RuntimeError(f"{fn} failed ({rc}): {msg}")
+
+
+def require_device():
+  """Return torch with a usable HIP device, else raise GPUUnavailableError (never falls back)."""
+  try:
+    import torch
+  except Exception as exc:  # pragma: no cover
+    raise GPUUnavailableError(f"torch unavailable: {exc}") from exc
+  if not torch.cuda.is_available():
+    raise GPUUnavailableError("no HIP device visible to torch; the GPU strategies never fall back to CPU")
+  load_library()
+  return torch
+
+
+def _stream(torch_mod) -> int:
+  return int(torch_mod.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> int | None:
+  if t is None:
+    return None
+  return int(t.data_ptr())
+
+
+def _need(t, dtype, name: str, ndim: int | None = None) -> None:
+  import torch
+
+  if not isinstance(t, torch.Tensor) or not t.is_cuda:
+    raise ValueError(f"{name} must be a device tensor")
+  if t.dtype != dtype:
+    raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+  if not t.is_contiguous():
+    raise ValueError(f"{name} must be contiguous")
+  if ndim is not None and t.dim() != ndim:
+    raise ValueError(f"{name} must be {ndim}-D, got shape {tuple(t.shape)}")
+
+
+# ---------------------------------------------------------------------------------------------
+# Thin typed wrappers (device tensors in, enqueue on the current stream, no host sync)
+# ---------------------------------------------------------------------------------------------
+
+
+def dist_euc2d(xy, D, max_out=None) -> None:
+  torch = require_device()
+  _need(xy, torch.float64, "xy", 2)
+  _need(D, torch.int32, "D", 2)
+  n = xy.shape[0]
+  if xy.shape[1] != 2 or D.shape[0] < n or D.shape[1] < n:
+    raise ValueError(f"dist_euc2d: xy {tuple(xy.shape)} vs D {tuple(D.shape)}")
+  if max_out is not None:
+    _need(max_out, torch.int32, "max_out")
+  _check(_lib.tspmi_dist_euc2d(_ptr(xy), n, _ptr(D), D.shape[1], _ptr(max_out), _stream(torch)), "tspmi_dist_euc2d")
+
+
+def knn(D, k: int, out, ties_out=None) -> None:
+  torch = require_device()
+  _need(D, torch.int32, "D", 2)
+  _need(out, torch.int32, "out", 2)
+  n = D.shape[0]
+  if out.shape != (n, k):
+    raise ValueError(f"knn: out shape {tuple(out.shape)} != ({n}, {k})")
+  if ties_out is not None:
+    _need(ties_out, torch.int32, "ties_out")
+  _check(_lib.tspmi_knn(_ptr(D), n, D.shape[1], k, _ptr(out), _ptr(ties_out), _stream(torch)), "tspmi_knn")
+
+
+def tour_cost(tours, D, out, closed: bool = True) -> None:
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(out, torch.float64, "out", 1)
+  P, n = tours.shape
+  if out.shape[0] != P:
+    raise ValueError("tour_cost: out length != P")
+  _check(_lib.tspmi_tour_cost(_ptr(tours), P, n, _ptr(D), D.shape[1], int(bool(closed)), _ptr(out), _stream(torch)),
+         "tspmi_tour_cost")
+
+
+def two_opt(tours, D, cand=None, mode: int = 0, max_iter: int = 10, stats=None) -> None:
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  k = 0
+  if cand is not None:
+    _need(cand, torch.int32, "cand", 2)
+    k = cand.shape[1]
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+    if stats.numel() < 3:
+      raise ValueError("two_opt: stats needs 3 int64 slots")
+  P, n = tours.shape
+  _check(_lib.tspmi_two_opt(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(cand), k, int(mode), int(max_iter),
+                            _ptr(stats), _stream(torch)), "tspmi_two_opt")
+
+
+def tournament(pop, fitness, cand, out, winners_out=None) -> None:
+  torch = require_device()
+  _need(pop, torch.int32, "pop", 2)
+  _need(fitness, torch.float64, "fitness", 1)
+  _need(cand, torch.int32, "cand", 2)
+  _need(out, torch.int32, "out", 2)
+  P, n = pop.shape
+  rows, k = cand.shape
+  if out.shape != (rows, n) or fitness.shape[0] != P:
+    raise ValueError("tournament: shape mismatch")
+  _check(_lib.tspmi_tournament(_ptr(pop), P, n, _ptr(fitness), _ptr(cand), k, rows, _ptr(out), _ptr(winners_out),
+                               _stream(torch)), "tspmi_tournament")
+
+
+def pmx(parents, cut_lo, cut_hi, out) -> None:
+  torch = require_device()
+  _need(parents, torch.int32, "parents", 2)
+  _need(out, torch.int32, "out", 2)
+  P, n = parents.shape
+  if P >= 2:
+    _need(cut_lo, torch.int32, "cut_lo", 1)
+    _need(cut_hi, torch.int32, "cut_hi", 1)
+    if cut_lo.shape[0] < P // 2 or cut_hi.shape[0] < P // 2:
+      raise ValueError("pmx: need P // 2 cut pairs")
+  _check(_lib.tspmi_pmx(_ptr(parents), P, n, _ptr(cut_lo), _ptr(cut_hi), _ptr(out), _stream(torch)), "tspmi_pmx")
+
+
+def inversion(routes, probs, pos1, pos2, rate: float) -> None:
+  torch = require_device()
+  _need(routes, torch.int32, "routes", 2)
+  _need(probs, torch.float64, "probs", 1)
+  _need(pos1, torch.int32, "pos1", 1)
+  _need(pos2, torch.int32, "pos2", 1)
+  P, n = routes.shape
+  if min(probs.shape[0], pos1.shape[0], pos2.shape[0]) < P:
+    raise ValueError("inversion: need P probabilities and positions")
+  _check(_lib.tspmi_inversion(_ptr(routes), P, n, _ptr(probs), _ptr(pos1), _ptr(pos2), float(rate), _stream(torch)),
+         "tspmi_inversion")
+
+
+def generation(pop, fitness, D, tour_cand, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring, off_fit) -> None:
+  torch = require_device()
+  P, n = pop.shape
+  for t, dt, nm in ((pop, torch.int32, "pop"), (fitness, torch.float64, "fitness"), (D, torch.int32, "D"),
+                    (tour_cand, torch.int32, "tour_cand"), (cut_lo, torch.int32, "cut_lo"),
+                    (cut_hi, torch.int32, "cut_hi"), (probs, torch.float64, "probs"), (pos1, torch.int32, "pos1"),
+                    (pos2, torch.int32, "pos2"), (offspring, torch.int32, "offspring"),
+                    (off_fit, torch.float64, "off_fit")):
+    _need(t, dt, nm)
+  if tour_cand.shape[0] != P or offspring.shape != (P, n) or off_fit.shape[0] != P:
+    raise ValueError("generation: shape mismatch")
+  _check(_lib.tspmi_generation(_ptr(pop), _ptr(fitness), P, n, _ptr(D), D.shape[1], _ptr(tour_cand),
+                               tour_cand.shape[1], _ptr(cut_lo), _ptr(cut_hi), _ptr(probs), _ptr(pos1), _ptr(pos2),
+                               float(rate), _ptr(offspring), _ptr(off_fit), _stream(torch)), "tspmi_generation")
+
+
+def rank_select(keys, keep: int, order) -> None:
+  torch = require_device()
+  _need(keys, torch.float64, "keys", 1)
+  _need(order, torch.int32, "order", 1)
+  if order.shape[0] < keep:
+    raise ValueError("rank_select: order shorter than keep")
+  _check(_lib.tspmi_rank_select(_ptr(keys), keys.shape[0], int(keep), _ptr(order), _stream(torch)), "tspmi_rank_select")
+
+
+def gather_rows(A, fitA, B, fitB, src, out, out_fit=None) -> None:
+  torch = require_device()
+  _need(A, torch.int32, "A", 2)
+  _need(src, torch.int32, "src", 1)
+  _need(out, torch.int32, "out", 2)
+  n = A.shape[1]
+  _check(_lib.tspmi_gather_rows(_ptr(A), _ptr(fitA), A.shape[0], _ptr(B), _ptr(fitB), n, _ptr(src), src.shape[0],
+                                _ptr(out), _ptr(out_fit), _stream(torch)), "tspmi_gather_rows")
+
+
+def scatter_rows(src, src_fit, idx, dst, dst_fit=None) -> None:
+  torch = require_device()
+  _need(src, torch.int32, "src", 2)
+  _need(idx, torch.int32, "idx", 1)
+  _need(dst, torch.int32, "dst", 2)
+  _check(_lib.tspmi_scatter_rows(_ptr(src), _ptr(src_fit), _ptr(idx), idx.shape[0], src.shape[1], _ptr(dst),
+                                 _ptr(dst_fit), _stream(torch)), "tspmi_scatter_rows")
+
+
+def random_init(seeds, customers, out) -> None:
+  torch = require_device()
+  _need(seeds, torch.int64, "seeds", 1)
+  _need(customers, torch.int32, "customers", 1)
+  _need(out, torch.int32, "out", 2)
+  _check(_lib.tspmi_random_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(out),
+                                _stream(torch)), "tspmi_random_init")
+
+
+def nn_init(seeds, customers, cand, work, out) -> None:
+  torch = require_device()
+  _need(seeds, torch.int64, "seeds", 1)
+  _need(customers, torch.int32, "customers", 1)
+  _need(out, torch.int32, "out", 2)
+  _need(work, torch.int32, "work")
+  k = 0 if cand is None else cand.shape[1]
+  if work.numel() < 2 * out.shape[0] * out.shape[1]:
+    raise ValueError("nn_init: work needs 2 * P * n int32")
+  _check(_lib.tspmi_nn_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(cand), k, _ptr(work),
+                            _ptr(out), _stream(torch)), "tspmi_nn_init")

@@ -1,0 +1,138 @@
+"""ProblemContext: one run's generator plus the device-resident distance and candidate tables.
+
+Mirrors the reference's ``ProblemContext`` (src/protocols/problem_context.py:75-616): a PCG64
+``rng`` that every strategy draws from, lazily built and cached distance matrix and k-NN tables,
+and an ``h2d_bytes`` counter of physical uploads.  Here the tables are built on the device by
+``tspmi_dist_euc2d`` / ``tspmi_knn``; host copies are downloads of those device tables.
+"""
+
+from __future__ import annotations
+
+from typing import Any
+
+import numpy as np
+
+from . import _native
+from .errors import VRAMInsufficientError
+from .problem import Problem
+from .rng import make_rng
+
+#: Largest |D| for which every int32 2-opt delta is exact (a + b - c - d with 0 <= D < 2^29).
+MAX_DISTANCE = (1 << 29) - 1
+DEFAULT_MAX_MATRIX_DIMENSION = 10_000
+
+
+class ProblemContext:
+  def __init__(
+    self,
+    problem: Problem,
+    xp: Any = None,
+    seed: Any = None,
+    max_matrix_dimension: int = DEFAULT_MAX_MATRIX_DIMENSION,
+    candidate_cache: dict | None = None,
+  ) -> None:
+    self.xp = xp
+    self.problem = problem
+    self.dimension = int(problem.dimension)
+    self.capacity = problem.capacity
+    self.max_matrix_dimension = max_matrix_dimension
+    self.rng: np.random.Generator = make_rng(seed)
+    self.seed = seed
+    self.h2d_bytes = 0
+    self._gpu_distances = None
+    self._cpu_distances: np.ndarray | None = None
+    self._gpu_coordinates = None
+    self._gpu_candidates: dict[int, Any] = {}
+    self._cpu_candidates: dict[int, np.ndarray] = candidate_cache if candidate_cache is not None else {}
+    self._candidate_ties: dict[int, int] = {}
+    self.max_distance: int | None = None
+
+  # -- topology (problem_context.py:219-237) ---------------------------------------------------
+  @property
+  def return_to_depot(self) -> bool:
+    return bool(self.problem.constraints.return_to_depot)
+
+  @property
+  def is_closed(self) -> bool:
+    return False  # TSP/ATSP permutations are open (no depot-bounded routes)
+
+  # -- device tables ----------------------------------------------------------------------------
+  def _check_vram(self, nbytes: int, what: str) -> None:
+    torch = _native.require_device()
+    free, _ = torch.cuda.mem_get_info()
+    if nbytes > free:
+      raise VRAMInsufficientError(f"{what} needs {nbytes} B but only {free} B of device memory are free")
+
+  def get_gpu_coordinates(self):
+    torch = _native.require_device()
+    if self._gpu_coordinates is None and self.problem.coordinates is not None:
+      xy = np.ascontiguousarray(self.problem.coordinates, dtype=np.float64)
+      self._gpu_coordinates = torch.from_numpy(xy).to("cuda")
+      self.h2d_bytes += int(xy.nbytes)
+    return self._gpu_coordinates
+
+  def get_gpu_distances(self):
+    """Dense (n, n) int32 matrix on the device, built once (problem_context.py:337-397)."""
+    torch = _native.require_device()
+    if self._gpu_distances is None:
+      n = self.dimension
+      self._check_vram(4 * n * n, f"dense distance matrix (n={n})")
+      if self.problem.distances is not None:
+        host = np.ascontiguousarray(self.problem.distances, dtype=np.int32)
+        d = torch.from_numpy(host).to("cuda")
+        self.h2d_bytes += int(host.nbytes)
+        mx = int(host.max()) if host.size else 0
+        mn = int(host.min()) if host.size else 0
+        if mn < 0:
+          raise ValueError("explicit distances must be non-negative")
+      else:
+        xy = self.get_gpu_coordinates()
+        d = torch.empty((n, n), dtype=torch.int32, device="cuda")
+        mx_t = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _native.dist_euc2d(xy, d, mx_t)
+        mx = int(mx_t.item())
+      if mx > MAX_DISTANCE:
+        raise ValueError(f"max distance {mx} exceeds {MAX_DISTANCE}: int32 2-opt deltas would not be exact")
+      self.max_distance = mx
+      self._gpu_distances = d
+    return self._gpu_distances
+
+  def get_cpu_distances(self) -> np.ndarray:
+    """Host copy of the device matrix (problem_context.py:266-301)."""
+    if self._cpu_distances is None:
+      self._cpu_distances = self.get_gpu_distances().cpu().numpy()
+    return self._cpu_distances
+
+  def get_gpu_candidate_list(self, k: int = 20):
+    """(n, min(k, n-1)) int32 k-NN table in (distance, index) order, built on device (problem_context.py:456-480)."""
+    torch = _native.require_device()
+    table = self._gpu_candidates.get(k)
+    if table is None:
+      n = self.dimension
+      k_eff = min(k, n - 1) if n > 1 else 0
+      cached = self._cpu_candidates.get(k)
+      if cached is not None:
+        table = torch.from_numpy(np.ascontiguousarray(cached, dtype=np.int32)).to("cuda")
+        self.h2d_bytes += int(cached.nbytes)
+      else:
+        d = self.get_gpu_distances()
+        table = torch.empty((n, k_eff), dtype=torch.int32, device="cuda")
+        ties = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if k_eff > 0:
+          _native.knn(d, k_eff, table, ties)
+        self._candidate_ties[k] = ties
+      self._gpu_candidates[k] = table
+    return table
+
+  def get_cpu_candidate_list(self, k: int = 20) -> np.ndarray:
+    cached = self._cpu_candidates.get(k)
+    if cached is None:
+      cached = self.get_gpu_candidate_list(k).cpu().numpy()
+      self._cpu_candidates[k] = cached
+    return cached
+
+  def candidate_boundary_ties(self, k: int = 20) -> int:
+    """Rows whose k-th and (k+1)-th distances tie (where the reference's argpartition is arbitrary)."""
+    self.get_gpu_candidate_list(k)
+    t = self._candidate_ties.get(k)
+    return 0 if t is None else int(t.item())

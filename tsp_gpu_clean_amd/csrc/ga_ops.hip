@@ -1,0 +1,360 @@
+// GA operators on device: tournament, PMX, inversion, the fused generation, stable rank
+// selection and row gather/scatter (gfx950).
+//
+// Random numbers are never drawn here: the host draws them from the run's generator in the
+// reference order and passes the buffers in, so every kernel is a pure function of its inputs.
+#include "common.hpp"
+
+namespace tspmi {
+
+constexpr int kRowBlock = 256;
+
+// First minimum of fitness over one tournament row (tournament_selection.py:125-127).
+__device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ cand_row, int k,
+                                                 const double* __restrict__ fitness, int P) {
+  int w = clamp_id(cand_row[0], P);
+  double best = fitness[w];
+  for (int c = 1; c < k; ++c) {
+    const int x = clamp_id(cand_row[c], P);
+    const double f = fitness[x];
+    if (f < best) {
+      best = f;
+      w = x;
+    }
+  }
+  return w;
+}
+
+// One wave per output row.
+__global__ __launch_bounds__(kRowBlock) void tournament_kernel(const int32_t* __restrict__ pop, int P, int n,
+                                                               const double* __restrict__ fitness,
+                                                               const int32_t* __restrict__ cand, int k, int rows,
+                                                               int32_t* __restrict__ out,
+                                                               int32_t* __restrict__ winners_out) {
+  const int r = blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= rows) return;
+  const int w = tournament_winner(cand + (size_t)r * k, k, fitness, P);
+  if (winners_out != nullptr && lane == 0) winners_out[r] = w;
+  const int32_t* src = pop + (size_t)w * n;
+  int32_t* dst = out + (size_t)r * n;
+  for (int x = lane; x < n; x += kWave) dst[x] = src[x];
+}
+
+// PMX child of (p1, p2) with cuts [c1, c2] into `child` (pmx_crossover.py:49-105, open
+// permutation).  inv1/inv2 map node -> position in p1/p2.  Caller synchronises afterwards.
+__device__ void pmx_child(const int32_t* p1, const int32_t* p2, const int32_t* inv1, const int32_t* inv2, int n,
+                          int c1, int c2, int32_t* child) {
+  for (int x = threadIdx.x; x < n; x += blockDim.x) child[x] = (x >= c1 && x <= c2) ? p1[x] : -1;
+  __syncthreads();
+  const int seg_len = c2 - c1 + 1;
+  for (int idx = c1 + threadIdx.x; idx <= c2; idx += blockDim.x) {
+    const int val = p2[idx];
+    const int pv = inv1[val];
+    if (pv >= c1 && pv <= c2) continue;  // val already in the copied segment
+    int pos = idx, steps = 0;
+    while (pos >= c1 && pos <= c2 && steps <= seg_len) {
+      pos = inv2[p1[pos]];
+      ++steps;
+    }
+    if ((unsigned)pos < (unsigned)n && (pos < c1 || pos > c2)) child[pos] = val;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < n; x += blockDim.x)
+    if (child[x] == -1) child[x] = p2[x];
+}
+
+__device__ __forceinline__ void load_parent(const int32_t* __restrict__ src, int n, int32_t* p, int32_t* inv) {
+  for (int x = threadIdx.x; x < n; x += blockDim.x) {
+    const int v = clamp_id(src[x], n);
+    p[x] = v;
+    inv[v] = x;
+  }
+}
+
+__device__ __forceinline__ void reverse_range(int32_t* t, int lo, int hi) {
+  const int half = (hi - lo + 1) >> 1;
+  for (int q = threadIdx.x; q < half; q += blockDim.x) {
+    const int32_t tmp = t[lo + q];
+    t[lo + q] = t[hi - q];
+    t[hi - q] = tmp;
+  }
+}
+
+// One workgroup per pair (2q, 2q+1); LDS: p1, p2, inv1, inv2 (4n ints).
+__global__ __launch_bounds__(kRowBlock) void pmx_kernel(const int32_t* __restrict__ parents, int P, int n,
+                                                        const int32_t* __restrict__ cut_lo,
+                                                        const int32_t* __restrict__ cut_hi,
+                                                        int32_t* __restrict__ out) {
+  extern __shared__ int32_t lds[];
+  const int q = blockIdx.x;
+  const int r0 = 2 * q;
+  if (r0 + 1 >= P) {  // odd last row is copied (pmx_crossover.py:168-169)
+    for (int x = threadIdx.x; x < n; x += blockDim.x) out[(size_t)r0 * n + x] = parents[(size_t)r0 * n + x];
+    return;
+  }
+  int32_t* p1 = lds;
+  int32_t* p2 = lds + n;
+  int32_t* inv1 = lds + 2 * n;
+  int32_t* inv2 = lds + 3 * n;
+  load_parent(parents + (size_t)r0 * n, n, p1, inv1);
+  load_parent(parents + (size_t)(r0 + 1) * n, n, p2, inv2);
+  int c1 = cut_lo[q], c2 = cut_hi[q];
+  if (c1 > c2) {
+    const int t = c1;
+    c1 = c2;
+    c2 = t;
+  }
+  c1 = c1 < 0 ? 0 : c1;
+  c2 = c2 >= n ? n - 1 : c2;
+  __syncthreads();
+  pmx_child(p1, p2, inv1, inv2, n, c1, c2, out + (size_t)r0 * n);
+  pmx_child(p2, p1, inv2, inv1, n, c1, c2, out + (size_t)(r0 + 1) * n);
+}
+
+// One wave per row.
+__global__ __launch_bounds__(kRowBlock) void inversion_kernel(int32_t* __restrict__ routes, int P, int n,
+                                                              const double* __restrict__ probs,
+                                                              const int32_t* __restrict__ pos1,
+                                                              const int32_t* __restrict__ pos2, double rate) {
+  const int r = blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= P || !(probs[r] < rate)) return;
+  int lo = pos1[r], hi = pos2[r];
+  if (lo > hi) {
+    const int t = lo;
+    lo = hi;
+    hi = t;
+  }
+  lo = lo < 0 ? 0 : lo;
+  hi = hi >= n ? n - 1 : hi;
+  int32_t* t = routes + (size_t)r * n;
+  const int half = (hi - lo + 1) >> 1;
+  for (int q = lane; q < half; q += kWave) {
+    const int32_t a = t[lo + q];
+    const int32_t b = t[hi - q];
+    t[lo + q] = b;
+    t[hi - q] = a;
+  }
+}
+
+__device__ __forceinline__ double block_cost(const int32_t* t, int n, const int32_t* __restrict__ D, int ld,
+                                             int64_t* red) {
+  int64_t s = 0;
+  for (int x = threadIdx.x; x < n; x += blockDim.x) {
+    const int nx = (x + 1 == n) ? 0 : x + 1;
+    if (n > 1) s += D[(size_t)t[x] * ld + t[nx]];
+  }
+  return (double)block_sum_i64(s, red);
+}
+
+// Fused tournament -> PMX -> inversion -> evaluate; one workgroup per child pair.
+// LDS: p1, p2, inv1, inv2, childA, childB (6n ints).
+__global__ __launch_bounds__(kRowBlock) void generation_kernel(
+    const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P, int n, const int32_t* __restrict__ D,
+    int ld, const int32_t* __restrict__ tour_cand, int k, const int32_t* __restrict__ cut_lo,
+    const int32_t* __restrict__ cut_hi, const double* __restrict__ probs, const int32_t* __restrict__ pos1,
+    const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ offspring, double* __restrict__ off_fit) {
+  extern __shared__ int32_t lds[];
+  __shared__ int64_t red[kRowBlock / kWave];
+  __shared__ int s_w[2];
+  const int q = blockIdx.x;
+  const int r0 = 2 * q;
+  const bool single = (r0 + 1 >= P);
+  int32_t* p1 = lds;
+  int32_t* p2 = lds + n;
+  int32_t* inv1 = lds + 2 * n;
+  int32_t* inv2 = lds + 3 * n;
+  int32_t* cA = lds + 4 * n;
+  int32_t* cB = lds + 5 * n;
+
+  if (threadIdx.x < 2 && (threadIdx.x == 0 || !single))
+    s_w[threadIdx.x] = tournament_winner(tour_cand + (size_t)(r0 + threadIdx.x) * k, k, fitness, P);
+  __syncthreads();
+  load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
+  if (!single) load_parent(pop + (size_t)s_w[1] * n, n, p2, inv2);
+  __syncthreads();
+
+  if (single) {
+    for (int x = threadIdx.x; x < n; x += blockDim.x) cA[x] = p1[x];
+    __syncthreads();
+  } else {
+    int c1 = cut_lo[q], c2 = cut_hi[q];
+    if (c1 > c2) {
+      const int t = c1;
+      c1 = c2;
+      c2 = t;
+    }
+    c1 = c1 < 0 ? 0 : c1;
+    c2 = c2 >= n ? n - 1 : c2;
+    pmx_child(p1, p2, inv1, inv2, n, c1, c2, cA);
+    pmx_child(p2, p1, inv2, inv1, n, c1, c2, cB);
+    __syncthreads();
+  }
+
+  const int nrows = single ? 1 : 2;
+  for (int c = 0; c < nrows; ++c) {
+    const int r = r0 + c;
+    int32_t* ch = c == 0 ? cA : cB;
+    if (probs[r] < rate) {
+      int lo = pos1[r], hi = pos2[r];
+      if (lo > hi) {
+        const int t = lo;
+        lo = hi;
+        hi = t;
+      }
+      lo = lo < 0 ? 0 : lo;
+      hi = hi >= n ? n - 1 : hi;
+      reverse_range(ch, lo, hi);
+      __syncthreads();
+    }
+    const double cost = block_cost(ch, n, D, ld, red);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) offspring[(size_t)r * n + x] = ch[x];
+    if (threadIdx.x == 0) off_fit[r] = cost;
+  }
+}
+
+// Stable rank of M float64 keys; order[rank[i]] = i for rank[i] < keep.
+// Keys are staged through LDS in tiles so each thread compares its key against all M keys.
+constexpr int kRankBlock = 256;
+constexpr int kRankTile = 2048;
+
+__global__ __launch_bounds__(kRankBlock) void rank_select_kernel(const double* __restrict__ keys, int M, int keep,
+                                                                 int32_t* __restrict__ order) {
+  __shared__ double tile[kRankTile];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double ki = i < M ? keys[i] : 0.0;
+  int rank = 0;
+  for (int base = 0; base < M; base += kRankTile) {
+    const int len = (M - base) < kRankTile ? (M - base) : kRankTile;
+    __syncthreads();
+    for (int x = threadIdx.x; x < len; x += blockDim.x) tile[x] = keys[base + x];
+    __syncthreads();
+    if (i < M) {
+      for (int x = 0; x < len; ++x) {
+        const double kj = tile[x];
+        const int j = base + x;
+        rank += (kj < ki) || (kj == ki && j < i);
+      }
+    }
+  }
+  if (i < M && rank < keep) order[rank] = i;
+}
+
+// One wave per destination row.
+__global__ __launch_bounds__(kRowBlock) void gather_rows_kernel(const int32_t* __restrict__ A,
+                                                                const double* __restrict__ fitA, int P0,
+                                                                const int32_t* __restrict__ B,
+                                                                const double* __restrict__ fitB, int n,
+                                                                const int32_t* __restrict__ src, int rows,
+                                                                int32_t* __restrict__ out, double* __restrict__ out_fit) {
+  const int d = blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (d >= rows) return;
+  const int s = src[d];
+  const int32_t* row = s < P0 ? A + (size_t)s * n : B + (size_t)(s - P0) * n;
+  for (int x = lane; x < n; x += kWave) out[(size_t)d * n + x] = row[x];
+  if (out_fit != nullptr && lane == 0) out_fit[d] = s < P0 ? fitA[s] : fitB[s - P0];
+}
+
+__global__ __launch_bounds__(kRowBlock) void scatter_rows_kernel(const int32_t* __restrict__ src,
+                                                                 const double* __restrict__ src_fit,
+                                                                 const int32_t* __restrict__ idx, int rows, int n,
+                                                                 int32_t* __restrict__ dst, double* __restrict__ dst_fit) {
+  const int r = blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= rows) return;
+  const int d = idx[r];
+  for (int x = lane; x < n; x += kWave) dst[(size_t)d * n + x] = src[(size_t)r * n + x];
+  if (dst_fit != nullptr && src_fit != nullptr && lane == 0) dst_fit[d] = src_fit[r];
+}
+
+inline int row_blocks(int rows) { return (rows + kRowBlock / kWave - 1) / (kRowBlock / kWave); }
+
+}  // namespace tspmi
+
+using namespace tspmi;
+
+extern "C" int tspmi_tournament(const int32_t* pop, int P, int n, const double* fitness, const int32_t* cand, int k,
+                                int num_parents, int32_t* out, int32_t* winners_out, void* stream) {
+  TSPMI_CHECK_ARG(P > 0 && n >= 0 && k >= 1 && num_parents >= 0, "tspmi_tournament: bad shape P=%d n=%d k=%d", P, n, k);
+  if (num_parents == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(pop && fitness && cand && out, "tspmi_tournament: null pointer");
+  hipLaunchKernelGGL(tournament_kernel, dim3(row_blocks(num_parents)), dim3(kRowBlock), 0, (hipStream_t)stream, pop, P,
+                     n, fitness, cand, k, num_parents, out, winners_out);
+  TSPMI_CHECK_LAUNCH("tspmi_tournament launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_pmx(const int32_t* parents, int P, int n, const int32_t* cut_lo, const int32_t* cut_hi,
+                         int32_t* out, void* stream) {
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_pmx: negative shape");
+  if (P == 0 || n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(parents && out && (P < 2 || (cut_lo && cut_hi)), "tspmi_pmx: null pointer");
+  const size_t lds = (size_t)4 * n * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds <= (size_t)kMaxLds, "tspmi_pmx: n=%d exceeds the LDS limit", n);
+  hipLaunchKernelGGL(pmx_kernel, dim3((P + 1) / 2), dim3(kRowBlock), lds, (hipStream_t)stream, parents, P, n, cut_lo,
+                     cut_hi, out);
+  TSPMI_CHECK_LAUNCH("tspmi_pmx launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_inversion(int32_t* routes, int P, int n, const double* probs, const int32_t* pos1,
+                               const int32_t* pos2, double rate, void* stream) {
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_inversion: negative shape");
+  if (P == 0 || n < 2) return TSPMI_OK;
+  TSPMI_CHECK_ARG(routes && probs && pos1 && pos2, "tspmi_inversion: null pointer");
+  hipLaunchKernelGGL(inversion_kernel, dim3(row_blocks(P)), dim3(kRowBlock), 0, (hipStream_t)stream, routes, P, n,
+                     probs, pos1, pos2, rate);
+  TSPMI_CHECK_LAUNCH("tspmi_inversion launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_generation(const int32_t* pop, const double* fitness, int P, int n, const int32_t* D, int ld,
+                                const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                                const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
+                                int32_t* offspring, double* off_fit, void* stream) {
+  TSPMI_CHECK_ARG(P > 0 && n >= 2 && k >= 1 && ld >= n, "tspmi_generation: bad shape P=%d n=%d k=%d ld=%d", P, n, k, ld);
+  TSPMI_CHECK_ARG(pop && fitness && D && tour_cand && probs && pos1 && pos2 && offspring && off_fit &&
+                      (P < 2 || (cut_lo && cut_hi)),
+                  "tspmi_generation: null pointer");
+  const size_t lds = (size_t)6 * n * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + 64 <= (size_t)kMaxLds, "tspmi_generation: n=%d exceeds the fused LDS limit", n);
+  hipLaunchKernelGGL(generation_kernel, dim3((P + 1) / 2), dim3(kRowBlock), lds, (hipStream_t)stream, pop, fitness, P,
+                     n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring, off_fit);
+  TSPMI_CHECK_LAUNCH("tspmi_generation launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_rank_select(const double* keys, int M, int keep, int32_t* order, void* stream) {
+  TSPMI_CHECK_ARG(M >= 0 && keep >= 0 && keep <= M, "tspmi_rank_select: bad sizes M=%d keep=%d", M, keep);
+  if (M == 0 || keep == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(keys && order, "tspmi_rank_select: null pointer");
+  hipLaunchKernelGGL(rank_select_kernel, dim3((M + kRankBlock - 1) / kRankBlock), dim3(kRankBlock), 0,
+                     (hipStream_t)stream, keys, M, keep, order);
+  TSPMI_CHECK_LAUNCH("tspmi_rank_select launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_gather_rows(const int32_t* A, const double* fitA, int P0, const int32_t* B, const double* fitB,
+                                 int n, const int32_t* src, int rows, int32_t* out, double* out_fit, void* stream) {
+  TSPMI_CHECK_ARG(P0 >= 0 && n >= 0 && rows >= 0, "tspmi_gather_rows: negative shape");
+  if (rows == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(src && out && (A || P0 == 0), "tspmi_gather_rows: null pointer");
+  TSPMI_CHECK_ARG(out_fit == nullptr || (fitA || P0 == 0), "tspmi_gather_rows: out_fit needs source fitness");
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(row_blocks(rows)), dim3(kRowBlock), 0, (hipStream_t)stream, A, fitA, P0,
+                     B, fitB, n, src, rows, out, out_fit);
+  TSPMI_CHECK_LAUNCH("tspmi_gather_rows launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_scatter_rows(const int32_t* src, const double* src_fit, const int32_t* idx, int rows, int n,
+                                  int32_t* dst, double* dst_fit, void* stream) {
+  TSPMI_CHECK_ARG(rows >= 0 && n >= 0, "tspmi_scatter_rows: negative shape");
+  if (rows == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(src && idx && dst, "tspmi_scatter_rows: null pointer");
+  hipLaunchKernelGGL(scatter_rows_kernel, dim3(row_blocks(rows)), dim3(kRowBlock), 0, (hipStream_t)stream, src, src_fit,
+                     idx, rows, n, dst, dst_fit);
+  TSPMI_CHECK_LAUNCH("tspmi_scatter_rows launch");
+  return TSPMI_OK;
+}

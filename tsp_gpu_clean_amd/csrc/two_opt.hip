@@ -1,0 +1,231 @@
+// Population-batched 2-opt for cyclic TSP tours (gfx950).
+//
+// One workgroup refines one tour for up to `max_iter` passes (or to its local optimum) in a
+// single launch.  The tour, its edge costs edge[x] = D[t_x, t_{x+1}] and (candidate mode) the
+// inverse map pos[node] live in LDS; distance rows are gathered from the L2-resident matrix.
+//
+// Semantics follow the reference CPU scan (src/algorithms/strategies/local_search/two_opt.py):
+//   full neighbourhood : pairs (i, j), 0 <= i <= n-3, i+2 <= j <= n-1, (0, n-1) excluded,
+//                        scan order (i asc, j asc)                        (two_opt.py:158-201)
+//   candidate lists    : pairs (i, r), j = pos[N[t_i, r]], kept iff j > i+1 and j < n and
+//                        not (i == 0 and j == n-1), scan order (i asc, r asc), duplicates kept
+//                                                                          (two_opt.py:347-367)
+//   delta              : D[a,c] + D[b,d] - D[a,b] - D[c,d]                 (two_opt.py:237-245)
+//   best               : first minimum in scan order, applied iff < 0      (two_opt.py:386-440)
+//   first              : first pair in scan order with delta < 0           (two_opt.py:442-489)
+//   move               : reverse positions i+1 .. j                        (two_opt.py:438)
+// Exact tie-breaking comes from packing (delta, scan index) into one 64-bit key whose unsigned
+// minimum is the reference's choice.
+#include "common.hpp"
+
+namespace tspmi {
+
+constexpr int kTwoOptBlock = 256;
+constexpr int kTwoOptWaves = kTwoOptBlock / kWave;
+
+struct TwoOptShared {
+  uint64_t red[kTwoOptWaves];
+  int64_t red64[kTwoOptWaves];
+};
+
+__device__ __forceinline__ int wrap_next(int x, int n) { return (x + 1 == n) ? 0 : x + 1; }
+
+// Reverse s_tour[i+1 .. j] and refresh edge costs (and positions) of the touched range.
+__device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int32_t* s_pos, int i, int j, int n,
+                                           const int32_t* __restrict__ D, int ld) {
+  const int half = (j - i) >> 1;
+  for (int t = threadIdx.x; t < half; t += blockDim.x) {
+    const int lo = i + 1 + t, hi = j - t;
+    const int32_t tmp = s_tour[lo];
+    s_tour[lo] = s_tour[hi];
+    s_tour[hi] = tmp;
+  }
+  __syncthreads();
+  for (int x = i + threadIdx.x; x <= j; x += blockDim.x) {
+    const int a = s_tour[x];
+    s_edge[x] = D[(size_t)a * ld + s_tour[wrap_next(x, n)]];
+    if (s_pos != nullptr && x > i) s_pos[a] = x;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restrict__ tours, int n,
+                                                               const int32_t* __restrict__ D, int ld,
+                                                               const int32_t* __restrict__ cand, int k, int mode,
+                                                               int max_iter, int64_t* __restrict__ stats) {
+  extern __shared__ int32_t lds[];
+  __shared__ TwoOptShared sh;
+  int32_t* s_tour = lds;
+  int32_t* s_edge = lds + n;
+  int32_t* s_pos = (cand != nullptr) ? lds + 2 * n : nullptr;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+
+  for (int x = tid; x < n; x += blockDim.x) {
+    int v = tour[x];
+    v = v < 0 ? 0 : (v >= ld ? ld - 1 : v);  // ids outside [0, ld) are an invalid input, never a fault
+    s_tour[x] = v;
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += blockDim.x) {
+    const int a = s_tour[x];
+    s_edge[x] = D[(size_t)a * ld + s_tour[wrap_next(x, n)]];
+    if (s_pos != nullptr) s_pos[a] = x;
+  }
+  __syncthreads();
+
+  int64_t my_deltas = 0;
+  int moves = 0, passes = 0;
+  const int rows = n - 2;  // i in [0, n-3]
+
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = kNoMove;
+    if (cand == nullptr && mode == TSPMI_MODE_BEST) {
+      for (int i = wid; i < rows; i += kTwoOptWaves) {
+        const int a = s_tour[i], b = s_tour[i + 1];
+        const int eab = s_edge[i];
+        const int32_t* __restrict__ ra = D + (size_t)a * ld;
+        const int32_t* __restrict__ rb = D + (size_t)b * ld;
+        const int lim = (i == 0) ? n - 1 : n;
+        const uint32_t base = (uint32_t)i * (uint32_t)n;
+        for (int j = i + 2 + lane; j < lim; j += kWave) {
+          const int c = s_tour[j];
+          const int d = s_tour[wrap_next(j, n)];
+          const int delta = ra[c] + rb[d] - eab - s_edge[j];
+          if (delta < 0) {
+            const uint64_t key = pack_key(delta, base + (uint32_t)j);
+            best = key < best ? key : best;
+          }
+        }
+      }
+      if (tid == 0) my_deltas += (int64_t)n * (n - 3) / 2;
+      best = block_min_u64(best, sh.red);
+    } else if (cand == nullptr) {  // first improvement, full neighbourhood
+      for (int base_i = 0; base_i < rows; base_i += kTwoOptWaves) {
+        const int i = base_i + wid;
+        if (i < rows) {
+          const int a = s_tour[i], b = s_tour[i + 1];
+          const int eab = s_edge[i];
+          const int32_t* __restrict__ ra = D + (size_t)a * ld;
+          const int32_t* __restrict__ rb = D + (size_t)b * ld;
+          const int lim = (i == 0) ? n - 1 : n;
+          for (int j0 = i + 2; j0 < lim; j0 += kWave) {
+            const int j = j0 + lane;
+            bool imp = false;
+            if (j < lim) {
+              const int c = s_tour[j];
+              const int d = s_tour[wrap_next(j, n)];
+              imp = (ra[c] + rb[d] - eab - s_edge[j]) < 0;
+              ++my_deltas;
+            }
+            if (imp) best = best < (uint64_t)((uint32_t)i * (uint32_t)n + (uint32_t)j) ? best
+                                                                         : (uint64_t)((uint32_t)i * (uint32_t)n + (uint32_t)j);
+            if (__any(imp)) break;  // later j in this row cannot be first
+          }
+        }
+        best = block_min_u64(best, sh.red);
+        if (best != kNoMove) break;
+      }
+      if (best != kNoMove) best = pack_key(-1, (uint32_t)best);
+    } else {  // candidate lists
+      const int total = rows * k;
+      if (mode == TSPMI_MODE_BEST) {
+        for (int f = tid; f < total; f += blockDim.x) {
+          const int i = f / k;
+          const int a = s_tour[i];
+          const int nb = cand[(size_t)a * k + (f - i * k)];
+          if (nb < 0 || nb >= n) continue;
+          const int j = s_pos[nb];
+          const int lim = (i == 0) ? n - 1 : n;
+          if (j <= i + 1 || j >= lim) continue;
+          const int b = s_tour[i + 1];
+          const int d = s_tour[wrap_next(j, n)];
+          const int delta = D[(size_t)a * ld + nb] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
+          ++my_deltas;
+          if (delta < 0) {
+            const uint64_t key = pack_key(delta, (uint32_t)f);
+            best = key < best ? key : best;
+          }
+        }
+        best = block_min_u64(best, sh.red);
+      } else {
+        for (int f0 = 0; f0 < total; f0 += blockDim.x) {
+          const int f = f0 + tid;
+          if (f < total) {
+            const int i = f / k;
+            const int a = s_tour[i];
+            const int nb = cand[(size_t)a * k + (f - i * k)];
+            if (nb >= 0 && nb < n) {
+              const int j = s_pos[nb];
+              const int lim = (i == 0) ? n - 1 : n;
+              if (j > i + 1 && j < lim) {
+                const int b = s_tour[i + 1];
+                const int d = s_tour[wrap_next(j, n)];
+                const int delta = D[(size_t)a * ld + nb] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
+                ++my_deltas;
+                if (delta < 0) best = (uint64_t)f;
+              }
+            }
+          }
+          best = block_min_u64(best, sh.red);
+          if (best != kNoMove) break;
+        }
+        if (best != kNoMove) best = pack_key(-1, (uint32_t)best);
+      }
+    }
+
+    if (best == kNoMove) break;  // uniform: every thread holds the block minimum
+    const uint32_t idx = key_index(best);
+    int i, j;
+    if (cand == nullptr) {
+      i = (int)(idx / (uint32_t)n);
+      j = (int)(idx - (uint32_t)i * (uint32_t)n);
+    } else {
+      i = (int)(idx / (uint32_t)k);
+      j = s_pos[cand[(size_t)s_tour[i] * k + (int)(idx - (uint32_t)i * (uint32_t)k)]];
+    }
+    __syncthreads();  // all threads decoded (i, j) before the tour changes
+    apply_move(s_tour, s_edge, s_pos, i, j, n, D, ld);
+    ++moves;
+  }
+
+  for (int x = tid; x < n; x += blockDim.x) tour[x] = s_tour[x];
+  if (stats != nullptr) {
+    const int64_t total = block_sum_i64(my_deltas, sh.red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)total);
+      atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+      atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+    }
+  }
+}
+
+}  // namespace tspmi
+
+extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* cand, int k,
+                             int mode, int max_iter, int64_t* stats, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt: negative shape P=%d n=%d", P, n);
+  TSPMI_CHECK_ARG(mode == TSPMI_MODE_BEST || mode == TSPMI_MODE_FIRST, "tspmi_two_opt: bad mode %d", mode);
+  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;  // reference: n < 4 or no budget is a copy
+  TSPMI_CHECK_ARG(tours != nullptr && D != nullptr, "tspmi_two_opt: null tours or distances");
+  TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt: ld=%d < n=%d", ld, n);
+  TSPMI_CHECK_ARG((long)n * (long)n < (1L << 32), "tspmi_two_opt: n=%d too large for 32-bit scan indices", n);
+  const bool use_cand = cand != nullptr && k > 0;
+  if (use_cand) {
+    TSPMI_CHECK_ARG(ld == n, "tspmi_two_opt: candidate mode needs ld == n (got ld=%d n=%d)", ld, n);
+    TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt: (n-2)*k too large");
+  }
+  const size_t lds = (size_t)(use_cand ? 3 : 2) * n * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + sizeof(TwoOptShared) <= (size_t)kMaxLds,
+                  "tspmi_two_opt: n=%d needs %zu B of LDS (max %d)", n, lds, kMaxLds);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(two_opt_kernel, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld,
+                     use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, stats);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
+  return TSPMI_OK;
+}

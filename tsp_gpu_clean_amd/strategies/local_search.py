@@ -1,0 +1,108 @@
+"""Local search strategies: the HIP 2-opt drop-in for ``LocalSearchGPUTourTwoOpt``
+(reference src/algorithms/strategies/local_search/two_opt.py:631-899) and ``LocalSearchNoOp``."""
+
+from __future__ import annotations
+
+from typing import Any
+
+from .. import _native
+from ..route_helpers import ensure_routes_batch, is_gpu_array, restore_routes_shape, to_device_int32
+from ..timing import time_strategy
+from .base import StrategyBase
+
+TSPMI_MODE_BEST = 0
+TSPMI_MODE_FIRST = 1
+
+#: Accepted modes.  The reference GPU kernel families "batch" and "naive" both apply the single
+#: steepest move per pass; here they share the steepest kernel, whose ties follow the reference
+#: CPU scan order (lowest (i, j)).  "best"/"first" name the CPU acceptance rules directly.
+_MODES: dict[str, int] = {"batch": TSPMI_MODE_BEST, "naive": TSPMI_MODE_BEST, "best": TSPMI_MODE_BEST,
+                          "first": TSPMI_MODE_FIRST}
+
+
+class LocalSearchGPUTourTwoOpt(StrategyBase):
+  """Population-batched 2-opt on the MI355X; one launch refines every tour for all its passes."""
+
+  def __init__(
+    self,
+    max_iterations: int = 10,
+    mode: str = "batch",
+    threads_per_block: int = 256,
+    max_matrix_dimension: int | None = None,
+    max_shared_tour_cities: int = 13_000,
+    candidate_k: int | None = None,
+    seed: Any = None,
+  ) -> None:
+    super().__init__(seed)
+    if mode not in _MODES:
+      raise ValueError(f"Invalid 2-opt GPU mode {mode!r}; expected one of {sorted(_MODES)}")
+    if max_iterations < 0:
+      raise ValueError(f"max_iterations must be >= 0, got {max_iterations}")
+    self.mode = mode
+    self.max_iterations = max_iterations
+    self.threads_per_block = threads_per_block
+    self.max_matrix_dimension = max_matrix_dimension
+    self.max_shared_tour_cities = max_shared_tour_cities
+    self.candidate_k = candidate_k
+    self.last_stats: tuple[int, int, int] | None = None
+    self._stats = None
+
+  def _warmup_impl(self, context) -> None:
+    self.improve_batch(self._warmup_batch(context, rows=1), context)
+
+  def _uses_candidates(self, context, n: int) -> bool:
+    """Candidate lists only for cyclic TSP tours spanning the whole instance (two_opt.py:711-736)."""
+    if self.candidate_k is None or self.candidate_k <= 0:
+      return False
+    if bool(getattr(context, "is_closed", False)) or not bool(getattr(context, "return_to_depot", True)):
+      return False
+    return n == int(context.dimension)
+
+  def improve(self, route, context):
+    return self.improve_batch(route, context)
+
+  @time_strategy
+  def improve_batch(self, routes, context):
+    """Refine (n,) or (N, n) tours; device in -> device out without a host sync."""
+    torch = _native.require_device()
+    batch, was_1d = ensure_routes_batch(routes)
+    num, n = int(batch.shape[0]), int(batch.shape[1])
+    on_gpu = is_gpu_array(batch)
+    if n < 4 or self.max_iterations <= 0 or num == 0:
+      out = batch.clone() if on_gpu else batch.copy()
+      return restore_routes_shape(out, was_1d)
+    if not bool(getattr(context, "return_to_depot", True)):
+      raise NotImplementedError("open-path (return_to_depot=False) 2-opt is not implemented on the device")
+    tours, _, up = to_device_int32(batch, copy=True)
+    self.h2d_bytes += up
+    dist = context.get_gpu_distances()
+    cand = context.get_gpu_candidate_list(int(self.candidate_k)) if self._uses_candidates(context, n) else None
+    if self._stats is None:
+      self._stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+    else:
+      self._stats.zero_()
+    _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
+    self.kernel_launches += 1
+    if on_gpu:
+      return restore_routes_shape(tours, was_1d)
+    host = tours.cpu().numpy()
+    self.d2h_bytes += int(host.nbytes)
+    return restore_routes_shape(host, was_1d)
+
+  def read_stats(self) -> tuple[int, int, int]:
+    """(deltas evaluated, moves applied, passes) of the last call (synchronises)."""
+    if self._stats is None:
+      return (0, 0, 0)
+    s = self._stats.cpu().tolist()
+    self.last_stats = (int(s[0]), int(s[1]), int(s[2]))
+    return self.last_stats
+
+
+class LocalSearchNoOp(StrategyBase):
+  """Identity local search (reference local_search/__init__.py LocalSearchNoOp)."""
+
+  def improve(self, route, context):
+    return route.clone() if is_gpu_array(route) else route.copy()
+
+  def improve_batch(self, routes, context):
+    return routes.clone() if is_gpu_array(routes) else routes.copy()
