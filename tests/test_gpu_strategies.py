@@ -1,0 +1,188 @@
+"""GPU strategies behind the reference's strategy API, and the GA end to end, against the oracle.
+
+Each strategy draws from ``context.rng`` exactly as the reference strategy does, so the oracle
+run with a generator of the same seed must produce identical routes.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import c_two_opt
+from oracle import ga as oga
+from oracle import init as oinit
+from oracle import operators as ops
+from oracle import two_opt as o2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+  print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def t():
+  import tsp_gpu_clean_amd as pkg
+  from tsp_gpu_clean_amd import _native
+
+  _native.require_device()
+  return pkg
+
+
+def c1_problem(t, n=52, seed=42, hi=1000):
+  xy = np.random.default_rng(seed).integers(0, hi + 1, (n, 2)).astype(np.float64)
+  return t.Problem.from_coordinates(f"u{n}", xy)
+
+
+def clustered_problem(t, n, seed=42):
+  rng = np.random.default_rng(seed)
+  centres = rng.uniform(0, 1e6, (20, 2))
+  lab = rng.integers(0, 20, n)
+  xy = np.rint(np.clip(centres[lab] + rng.normal(0, 0.02 * 1e6, (n, 2)), 0, 1e6))
+  return t.Problem.from_coordinates(f"c{n}", xy)
+
+
+def test_residency_shape_and_no_aliasing(t):
+  import torch
+
+  say("strategy residency")
+  ctx = t.ProblemContext(c1_problem(t), seed=1)
+  ls = t.LocalSearchGPUTourTwoOpt(max_iterations=5)
+  host = np.stack([np.random.default_rng(i).permutation(52) for i in range(4)]).astype(np.int32)
+  keep = host.copy()
+  out = ls.improve_batch(host, ctx)
+  assert isinstance(out, np.ndarray) and out.shape == host.shape
+  np.testing.assert_array_equal(host, keep)
+  dev = torch.from_numpy(host).cuda()
+  out_d = ls.improve_batch(dev, ctx)
+  assert isinstance(out_d, torch.Tensor) and out_d.is_cuda and out_d.data_ptr() != dev.data_ptr()
+  np.testing.assert_array_equal(dev.cpu().numpy(), keep)
+  np.testing.assert_array_equal(out_d.cpu().numpy(), out)
+  one = ls.improve(host[0], ctx)
+  assert one.shape == (52,)
+  np.testing.assert_array_equal(one, out[0])
+  ev = t.EvaluationGPUCost()
+  assert ev.evaluate(host[0], ctx) == ops.tour_costs(host[:1], ctx.get_cpu_distances())[0]
+  a = ev.assess_batch(dev, ctx, return_gpu=True)
+  assert a.objectives.is_cuda and bool(a.is_feasible.all())
+
+
+@pytest.mark.parametrize("k", [None, 20])
+@pytest.mark.parametrize("mode", ["batch", "first"])
+def test_local_search_strategy_matches_oracle(t, k, mode):
+  say(f"LocalSearchGPUTourTwoOpt k={k} mode={mode}")
+  ctx = t.ProblemContext(c1_problem(t, 300, seed=7, hi=1_000_000), seed=3)
+  routes = np.stack([np.random.default_rng(i).permutation(300) for i in range(16)]).astype(np.int32)
+  ls = t.LocalSearchGPUTourTwoOpt(max_iterations=10, mode=mode, candidate_k=k)
+  got = ls.improve_batch(routes, ctx)
+  cand = None if k is None else ctx.get_cpu_candidate_list(k)
+  want = o2.improve_batch(routes, ctx.get_cpu_distances(), cand, "best" if mode == "batch" else "first", 10)
+  np.testing.assert_array_equal(got, want)
+  deltas, moves, passes = ls.read_stats()
+  assert moves > 0 and passes >= moves
+
+
+def test_operator_strategies_match_oracle_with_same_generator(t):
+  say("selection/crossover/mutation strategies")
+  prob = c1_problem(t, 120, seed=9, hi=1_000_000)
+  ctx = t.ProblemContext(prob, seed=123)
+  rng = np.random.default_rng(123)
+  pop = np.stack([np.random.default_rng(100 + i).permutation(120) for i in range(64)]).astype(np.int32)
+  d = ctx.get_cpu_distances()
+  fit = ops.tour_costs(pop, d)
+  sel = t.SelectionGPUTournament(k=5).select_batch(pop, fit, 64, ctx)
+  want_sel = ops.tournament_select(rng, pop, fit, 5, 64)
+  np.testing.assert_array_equal(sel, want_sel)
+  kids = t.CrossoverGPUPermPartiallyMapped().crossover_batch(sel, ctx)
+  want_kids = ops.pmx_batch(rng, want_sel)
+  np.testing.assert_array_equal(kids, want_kids)
+  mut = t.MutationGPUPermInversion().mutate_batch(kids, 0.3, ctx)
+  np.testing.assert_array_equal(mut, ops.inversion_batch(rng, want_kids, 0.3))
+  assert ctx.rng.bit_generator.state == rng.bit_generator.state
+
+
+def test_initializers_match_oracle(t):
+  say("initializers")
+  prob = c1_problem(t)
+  ctx = t.ProblemContext(prob, seed=42)
+  rng = np.random.default_rng(42)
+  pop = t.InitializationGPUPermRandom(return_gpu=False).initialize(list(range(52)), 128, ctx)
+  np.testing.assert_array_equal(pop, oinit.random_init(np.arange(52), 128, oinit.draw_row_seeds(rng, 128)))
+  pop = t.InitializationGPUTourNearestNeighbor(k=5, return_gpu=False).initialize(list(range(52)), 128, ctx)
+  cand = ctx.get_cpu_candidate_list(5)
+  np.testing.assert_array_equal(pop, oinit.nn_init(np.arange(52), 128, cand, oinit.draw_row_seeds(rng, 128)))
+
+
+def test_warmup_leaves_rng_and_counters_untouched(t):
+  say("warmup contract")
+  ctx = t.ProblemContext(c1_problem(t), seed=5)
+  before = ctx.rng.bit_generator.state
+  for s in (t.LocalSearchGPUTourTwoOpt(candidate_k=20), t.EvaluationGPUCost(), t.SelectionGPUTournament(),
+            t.CrossoverGPUPermPartiallyMapped(), t.MutationGPUPermInversion(), t.InitializationGPUPermRandom(),
+            t.InitializationGPUTourNearestNeighbor(k=5)):
+    s.warmup(ctx)
+    assert s.kernel_launches == 0 and s.call_count == 0 and s.h2d_bytes == 0
+  assert ctx.rng.bit_generator.state == before
+
+
+def _c1_ga(t, ctx, generations=50, patience=None):
+  ls = t.LocalSearchGPUTourTwoOpt(mode="best", max_iterations=10, candidate_k=20)
+  ev = t.EvaluationGPUCost()
+  ga = t.GeneticAlgorithm(
+    population_size=128, mutation_rate=0.05, seed=42,
+    initialization=t.InitializationGPUTourNearestNeighbor(k=5, seed=42),
+    selection=t.SelectionGPUTournament(k=5, seed=42), crossover=t.CrossoverGPUPermPartiallyMapped(seed=42),
+    mutation=t.MutationGPUPermInversion(seed=42), local_search=ls, evaluation=ev, local_search_stage="none",
+    pipeline=[t.PostInitLocalSearchStep(ls, ev), t.EliteOnlyLocalSearchStep(ls, ev, elite_count=16)])
+  return ga, ga.evolve(ctx, list(range(ctx.dimension)), generations, patience=patience)
+
+
+def test_config1_ga_end_to_end_matches_oracle(t):
+  """Config 1: 52 uniform integer cities in [0,1000]^2 (seed 42), README memetic pipeline, 50 generations."""
+  say("GA config 1 end to end")
+  prob = c1_problem(t)
+  ctx = t.ProblemContext(prob, seed=42)
+  ga, (best, stats) = _c1_ga(t, ctx)
+  d = ctx.get_cpu_distances()
+  rng = np.random.default_rng(42)
+  pop0 = oinit.nn_init(np.arange(52), 128, ctx.get_cpu_candidate_list(5), oinit.draw_row_seeds(rng, 128))
+  res = oga.evolve(rng, d, pop0, max_generations=50, mutation_rate=0.05, tournament_k=5,
+                   local_search=oga.LocalSearchSpec("best", 10, ctx.get_cpu_candidate_list(20)), post_init=True,
+                   elite_count=16)
+  assert stats["convergence_history"] == res.stats["convergence_history"]
+  assert stats["generations_completed"] == res.stats["generations_completed"]
+  assert stats["stop_reason"] == res.stats["stop_reason"]
+  np.testing.assert_array_equal(best, res.best_tour)
+  assert stats["best_fitness"] == ops.tour_costs(best[None], d)[0]
+  assert len(stats["convergence_history"]) == stats["generations_completed"] + 1
+  assert all(a >= b for a, b in zip(stats["convergence_history"], stats["convergence_history"][1:]))
+
+
+def test_device_loop_matches_step_pipeline_and_oracle(t):
+  """The no-sync device loop equals the oracle GA population after every generation."""
+  import torch
+
+  from tsp_gpu_clean_amd.engine import DeviceMemeticLoop, draw_generations, upload
+
+  say("device memetic loop vs oracle")
+  n, P, G = 200, 128, 12
+  prob = clustered_problem(t, n)
+  ctx = t.ProblemContext(prob, seed=42)
+  rng = np.random.default_rng(42)
+  from tsp_gpu_clean_amd.rng import draw_row_seeds
+
+  draw_row_seeds(ctx.rng, P)  # the device run consumes the same init draw as the oracle below
+  pop0 = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(rng, P))
+  draws = upload(draw_generations(ctx.rng, G, P, n, 5))
+  loop = DeviceMemeticLoop(ctx, P, 0.05, tournament_k=5, elite_count=16, ls_max_iterations=10, ls_candidate_k=20)
+  loop.load(torch.from_numpy(pop0).cuda())
+  res = oga.evolve(rng, ctx.get_cpu_distances(), pop0, max_generations=G, mutation_rate=0.05, tournament_k=5,
+                   local_search=oga.LocalSearchSpec("best", 10, ctx.get_cpu_candidate_list(20)), elite_count=16,
+                   patience=10**6, record=True)
+  for g in range(G):
+    loop.step(draws, g)
+    np.testing.assert_array_equal(loop.population.cpu().numpy(), res.populations[g], err_msg=f"generation {g + 1}")
+    np.testing.assert_array_equal(loop.fitness.cpu().numpy(), res.fitnesses[g])
+  assert ctx.rng.bit_generator.state == rng.bit_generator.state

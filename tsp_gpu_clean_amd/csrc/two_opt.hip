@@ -49,6 +49,50 @@ __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int
   __syncthreads();
 }
 
+// Steepest scan of the full neighbourhood with a diagonal lane mapping.
+//
+// Lane (wave w, lane L) owns the j offsets j = i + 2 + 64*(w + 4q) + L of every row i.  Moving from
+// row i to row i+1 shifts each owned j by one, so the lane's second gather of row i,
+// D[t_{i+1}, t_{j+1}], is exactly its first gather of row i+1, D[t_{i+1}, t_{j'}] with j' = j+1: it is
+// carried in a register and every delta costs one distance gather instead of two.  Each lane
+// visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per lane
+// and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
+template <int QMAX>
+__device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
+                                                        const int32_t* __restrict__ D, int ld, int wid, int lane) {
+  const int rows = n - 2;
+  int carry[QMAX];
+  const int32_t* __restrict__ r0 = D + (size_t)s_tour[0] * ld;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int j = 2 + (wid + kTwoOptWaves * q) * kWave + lane;
+    carry[q] = (j < n) ? r0[s_tour[j]] : 0;
+  }
+  int bd = 0;
+  uint32_t bi = 0xffffffffu;
+  for (int i = 0; i < rows; ++i) {
+    if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
+    const int32_t* __restrict__ rb = D + (size_t)s_tour[i + 1] * ld;
+    const int eab = s_edge[i];
+    const int lim = (i == 0) ? n - 1 : n;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = i + 2 + (wid + kTwoOptWaves * q) * kWave + lane;
+      if (j < lim) {
+        const int g = rb[s_tour[(j + 1 == n) ? 0 : j + 1]];
+        const int delta = carry[q] + g - eab - s_edge[j];
+        carry[q] = g;
+        if (delta < bd) {
+          bd = delta;
+          bi = (uint32_t)i * (uint32_t)n + (uint32_t)j;
+        }
+      }
+    }
+  }
+  return bd < 0 ? pack_key(bd, bi) : kNoMove;
+}
+
+template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restrict__ tours, int n,
                                                                const int32_t* __restrict__ D, int ld,
                                                                const int32_t* __restrict__ cand, int k, int mode,
@@ -84,7 +128,11 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
     uint64_t best = kNoMove;
-    if (cand == nullptr && mode == TSPMI_MODE_BEST) {
+    if (cand == nullptr && mode == TSPMI_MODE_BEST && QMAX > 0) {
+      best = scan_full_best_diag<(QMAX > 0 ? QMAX : 1)>(s_tour, s_edge, n, D, ld, wid, lane);
+      if (tid == 0) my_deltas += (int64_t)n * (n - 3) / 2;
+      best = block_min_u64(best, sh.red);
+    } else if (cand == nullptr && mode == TSPMI_MODE_BEST) {
       for (int i = wid; i < rows; i += kTwoOptWaves) {
         const int a = s_tour[i], b = s_tour[i + 1];
         const int eab = s_edge[i];
@@ -224,8 +272,15 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   TSPMI_CHECK_ARG(lds + sizeof(TwoOptShared) <= (size_t)kMaxLds,
                   "tspmi_two_opt: n=%d needs %zu B of LDS (max %d)", n, lds, kMaxLds);
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(two_opt_kernel, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld,
-                     use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, stats);
+  // Diagonal register carry covers rows of up to kTwoOptWaves * 8 * 64 offsets (n <= 2050).
+  constexpr int kQ = 8;
+  if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * kQ * kWave) {
+    hipLaunchKernelGGL(two_opt_kernel<kQ>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, nullptr, 0, mode,
+                       max_iter, stats);
+  } else {
+    hipLaunchKernelGGL(two_opt_kernel<0>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld,
+                       use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, stats);
+  }
   TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
   return TSPMI_OK;
 }
