@@ -54,39 +54,55 @@ __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int
 // Lane (wave w, lane L) owns the j offsets j = i + 2 + 64*(w + 4q) + L of every row i.  Moving from
 // row i to row i+1 shifts each owned j by one, so the lane's second gather of row i,
 // D[t_{i+1}, t_{j+1}], is exactly its first gather of row i+1, D[t_{i+1}, t_{j'}] with j' = j+1: it is
-// carried in a register and every delta costs one distance gather instead of two.  Each lane
-// visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per lane
-// and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
+// carried in a register and every delta costs one distance gather instead of two.  The gathers of
+// row i+1 are issued before row i is evaluated (one-row software pipeline), the distance-row base
+// is wave-uniform (scalar base + 32-bit lane offset) and s_tour[n] mirrors s_tour[0] so the
+// successor of position n-1 needs no wrap test.  Each lane visits its pairs in increasing scan
+// order, so a strict `<` keeps the first minimum per lane and the packed-key block minimum then
+// restores the global lowest-(i, j) tie-break.
 template <int QMAX>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane) {
   const int rows = n - 2;
-  int carry[QMAX];
-  const int32_t* __restrict__ r0 = D + (size_t)s_tour[0] * ld;
+  const int jq0 = 2 + wid * kWave + lane;  // lane's j at row 0 for q = 0; q adds 256
+  int carry[QMAX], gcur[QMAX], gnext[QMAX];
+  {
+    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
+    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
 #pragma unroll
-  for (int q = 0; q < QMAX; ++q) {
-    const int j = 2 + (wid + kTwoOptWaves * q) * kWave + lane;
-    carry[q] = (j < n) ? r0[s_tour[j]] : 0;
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = jq0 + q * kTwoOptBlock;
+      carry[q] = (j < n) ? r0[s_tour[j]] : 0;
+      gcur[q] = (j < n) ? r1[s_tour[j + 1]] : 0;
+    }
   }
   int bd = 0;
   uint32_t bi = 0xffffffffu;
   for (int i = 0; i < rows; ++i) {
     if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
-    const int32_t* __restrict__ rb = D + (size_t)s_tour[i + 1] * ld;
+    if (i + 1 < rows) {
+      const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
+#pragma unroll
+      for (int q = 0; q < QMAX; ++q) {
+        const int j1 = i + 1 + jq0 + q * kTwoOptBlock;  // the lane's j at row i+1
+        gnext[q] = (j1 < n) ? rn[s_tour[j1 + 1]] : 0;
+      }
+    }
     const int eab = s_edge[i];
     const int lim = (i == 0) ? n - 1 : n;
+    const uint32_t base = (uint32_t)i * (uint32_t)n;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
-      const int j = i + 2 + (wid + kTwoOptWaves * q) * kWave + lane;
+      const int j = i + jq0 + q * kTwoOptBlock;
       if (j < lim) {
-        const int g = rb[s_tour[(j + 1 == n) ? 0 : j + 1]];
-        const int delta = carry[q] + g - eab - s_edge[j];
-        carry[q] = g;
+        const int delta = carry[q] + gcur[q] - eab - s_edge[j];
         if (delta < bd) {
           bd = delta;
-          bi = (uint32_t)i * (uint32_t)n + (uint32_t)j;
+          bi = base + (uint32_t)j;
         }
       }
+      carry[q] = gcur[q];
+      gcur[q] = gnext[q];
     }
   }
   return bd < 0 ? pack_key(bd, bi) : kNoMove;
@@ -99,9 +115,9 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
                                                                int max_iter, int64_t* __restrict__ stats) {
   extern __shared__ int32_t lds[];
   __shared__ TwoOptShared sh;
-  int32_t* s_tour = lds;
-  int32_t* s_edge = lds + n;
-  int32_t* s_pos = (cand != nullptr) ? lds + 2 * n : nullptr;
+  int32_t* s_tour = lds;  // n + 1 entries: s_tour[n] mirrors s_tour[0] (position 0 never moves)
+  int32_t* s_edge = lds + n + 1;
+  int32_t* s_pos = (cand != nullptr) ? lds + 2 * n + 1 : nullptr;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -112,6 +128,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
     int v = tour[x];
     v = v < 0 ? 0 : (v >= ld ? ld - 1 : v);  // ids outside [0, ld) are an invalid input, never a fault
     s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
   }
   __syncthreads();
   for (int x = tid; x < n; x += blockDim.x) {
@@ -274,11 +291,14 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   hipStream_t s = (hipStream_t)stream;
   // Diagonal register carry covers rows of up to kTwoOptWaves * 8 * 64 offsets (n <= 2050).
   constexpr int kQ = 8;
-  if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * kQ * kWave) {
-    hipLaunchKernelGGL(two_opt_kernel<kQ>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, nullptr, 0, mode,
-                       max_iter, stats);
+  if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * 4 * kWave) {
+    hipLaunchKernelGGL(two_opt_kernel<4>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
+                       nullptr, 0, mode, max_iter, stats);
+  } else if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * kQ * kWave) {
+    hipLaunchKernelGGL(two_opt_kernel<kQ>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
+                       nullptr, 0, mode, max_iter, stats);
   } else {
-    hipLaunchKernelGGL(two_opt_kernel<0>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld,
+    hipLaunchKernelGGL(two_opt_kernel<0>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
                        use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, stats);
   }
   TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
