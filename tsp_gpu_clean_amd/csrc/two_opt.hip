@@ -55,11 +55,18 @@ __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int
 // row i to row i+1 shifts each owned j by one, so the lane's second gather of row i,
 // D[t_{i+1}, t_{j+1}], is exactly its first gather of row i+1, D[t_{i+1}, t_{j'}] with j' = j+1: it is
 // carried in a register and every delta costs one distance gather instead of two.  The gathers of
-// row i+1 are issued before row i is evaluated (one-row software pipeline), the distance-row base
-// is wave-uniform (scalar base + 32-bit lane offset) and s_tour[n] mirrors s_tour[0] so the
-// successor of position n-1 needs no wrap test.  Each lane visits its pairs in increasing scan
-// order, so a strict `<` keeps the first minimum per lane and the packed-key block minimum then
-// restores the global lowest-(i, j) tie-break.
+// row i+1 are issued before row i is evaluated (one-row software pipeline).  The loop body is
+// branch-free so every load is unconditional and the pipeline really overlaps: s_tour and s_edge
+// carry `pad` zero entries past their ends (node 0 is a valid gather target), lanes past the row
+// end compute a harmless delta that the select discards.  The distance-row base is wave-uniform and
+// the gather uses a 32-bit unsigned byte offset.  s_tour[n] mirrors s_tour[0], so the successor of
+// position n-1 needs no wrap test.  Each lane visits its pairs in increasing scan order, so a
+// strict `<` keeps the first minimum per lane and the packed-key block minimum then restores the
+// global lowest-(i, j) tie-break.
+__device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node) {
+  return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(row) + ((uint32_t)node << 2));
+}
+
 template <int QMAX>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane) {
@@ -72,35 +79,32 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int j = jq0 + q * kTwoOptBlock;
-      carry[q] = (j < n) ? r0[s_tour[j]] : 0;
-      gcur[q] = (j < n) ? r1[s_tour[j + 1]] : 0;
+      carry[q] = row_at(r0, s_tour[j]);
+      gcur[q] = row_at(r1, s_tour[j + 1]);
     }
   }
   int bd = 0;
   uint32_t bi = 0xffffffffu;
+  const int last = rows - 1;
   for (int i = 0; i < rows; ++i) {
     if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
-    if (i + 1 < rows) {
-      const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
+    // row i+1's gathers (for the last row this re-reads row i+1's own row harmlessly)
+    const int32_t* __restrict__ rn =
+        D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i < last ? i + 2 : i + 1]) * ld;
+    const int* tq = s_tour + i + jq0 + 2;
 #pragma unroll
-      for (int q = 0; q < QMAX; ++q) {
-        const int j1 = i + 1 + jq0 + q * kTwoOptBlock;  // the lane's j at row i+1
-        gnext[q] = (j1 < n) ? rn[s_tour[j1 + 1]] : 0;
-      }
-    }
+    for (int q = 0; q < QMAX; ++q) gnext[q] = row_at(rn, tq[q * kTwoOptBlock]);
     const int eab = s_edge[i];
     const int lim = (i == 0) ? n - 1 : n;
     const uint32_t base = (uint32_t)i * (uint32_t)n;
+    const int* eq = s_edge + i + jq0;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int j = i + jq0 + q * kTwoOptBlock;
-      if (j < lim) {
-        const int delta = carry[q] + gcur[q] - eab - s_edge[j];
-        if (delta < bd) {
-          bd = delta;
-          bi = base + (uint32_t)j;
-        }
-      }
+      const int delta = carry[q] + gcur[q] - eab - eq[q * kTwoOptBlock];
+      const bool take = (j < lim) & (delta < bd);
+      bd = take ? delta : bd;
+      bi = take ? base + (uint32_t)j : bi;
       carry[q] = gcur[q];
       gcur[q] = gnext[q];
     }
@@ -112,12 +116,14 @@ template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restrict__ tours, int n,
                                                                const int32_t* __restrict__ D, int ld,
                                                                const int32_t* __restrict__ cand, int k, int mode,
-                                                               int max_iter, int64_t* __restrict__ stats) {
+                                                               int max_iter, int pad, int64_t* __restrict__ stats) {
   extern __shared__ int32_t lds[];
   __shared__ TwoOptShared sh;
-  int32_t* s_tour = lds;  // n + 1 entries: s_tour[n] mirrors s_tour[0] (position 0 never moves)
-  int32_t* s_edge = lds + n + 1;
-  int32_t* s_pos = (cand != nullptr) ? lds + 2 * n + 1 : nullptr;
+  // s_tour: n + 1 entries (s_tour[n] mirrors s_tour[0]; position 0 never moves) then `pad` zeros;
+  // s_edge: n entries then `pad` zeros; s_pos: n entries (candidate mode).
+  int32_t* s_tour = lds;
+  int32_t* s_edge = lds + n + 1 + pad;
+  int32_t* s_pos = (cand != nullptr) ? s_edge + n + pad : nullptr;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -129,6 +135,10 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
     v = v < 0 ? 0 : (v >= ld ? ld - 1 : v);  // ids outside [0, ld) are an invalid input, never a fault
     s_tour[x] = v;
     if (x == 0) s_tour[n] = v;
+  }
+  for (int x = tid; x < pad; x += blockDim.x) {
+    s_tour[n + 1 + x] = 0;
+    s_edge[n + x] = 0;
   }
   __syncthreads();
   for (int x = tid; x < n; x += blockDim.x) {
@@ -285,21 +295,25 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
     TSPMI_CHECK_ARG(ld == n, "tspmi_two_opt: candidate mode needs ld == n (got ld=%d n=%d)", ld, n);
     TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt: (n-2)*k too large");
   }
-  const size_t lds = (size_t)(use_cand ? 3 : 2) * n * sizeof(int32_t);
+  hipStream_t s = (hipStream_t)stream;
+  // The diagonal scan reads up to QMAX * 256 + 64 entries past the tour/edge arrays (zero padding).
+  constexpr int kQ = 8;
+  const int q = (!use_cand && mode == TSPMI_MODE_BEST) ? (n - 2 <= kTwoOptWaves * 4 * kWave ? 4
+                                                          : (n - 2 <= kTwoOptWaves * kQ * kWave ? kQ : 0))
+                                                       : 0;
+  const int pad = q > 0 ? q * kTwoOptBlock + kWave : 0;
+  const size_t lds = ((size_t)(use_cand ? 3 : 2) * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + sizeof(TwoOptShared) <= (size_t)kMaxLds,
                   "tspmi_two_opt: n=%d needs %zu B of LDS (max %d)", n, lds, kMaxLds);
-  hipStream_t s = (hipStream_t)stream;
-  // Diagonal register carry covers rows of up to kTwoOptWaves * 8 * 64 offsets (n <= 2050).
-  constexpr int kQ = 8;
-  if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * 4 * kWave) {
-    hipLaunchKernelGGL(two_opt_kernel<4>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
-                       nullptr, 0, mode, max_iter, stats);
-  } else if (!use_cand && mode == TSPMI_MODE_BEST && n - 2 <= kTwoOptWaves * kQ * kWave) {
-    hipLaunchKernelGGL(two_opt_kernel<kQ>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
-                       nullptr, 0, mode, max_iter, stats);
+  if (q == 4) {
+    hipLaunchKernelGGL(two_opt_kernel<4>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, nullptr, 0, mode,
+                       max_iter, pad, stats);
+  } else if (q == kQ) {
+    hipLaunchKernelGGL(two_opt_kernel<kQ>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, nullptr, 0, mode,
+                       max_iter, pad, stats);
   } else {
-    hipLaunchKernelGGL(two_opt_kernel<0>, dim3(P), dim3(kTwoOptBlock), lds + sizeof(int32_t), s, tours, n, D, ld,
-                       use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, stats);
+    hipLaunchKernelGGL(two_opt_kernel<0>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld,
+                       use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, 0, stats);
   }
   TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
   return TSPMI_OK;
