@@ -72,7 +72,7 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane) {
   const int rows = n - 2;
   const int jq0 = 2 + wid * kWave + lane;  // lane's j at row 0 for q = 0; q adds 256
-  int carry[QMAX], gcur[QMAX], gnext[QMAX];
+  int carry[QMAX], gcur[QMAX], gnext[QMAX], bd[QMAX], bi[QMAX];
   {
     const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
     const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
@@ -81,35 +81,48 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
       const int j = jq0 + q * kTwoOptBlock;
       carry[q] = row_at(r0, s_tour[j]);
       gcur[q] = row_at(r1, s_tour[j + 1]);
+      bd[q] = 0;
+      bi[q] = 0;
     }
   }
-  int bd = 0;
-  uint32_t bi = 0xffffffffu;
-  const int last = rows - 1;
-  for (int i = 0; i < rows; ++i) {
-    if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
-    // row i+1's gathers (for the last row this re-reads row i+1's own row harmlessly)
-    const int32_t* __restrict__ rn =
-        D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i < last ? i + 2 : i + 1]) * ld;
+  // Row i, lane slot q evaluates pair (i, j = i + jq0 + 256 q).  Slots past the row end read the
+  // sentinel edge padding, which makes their delta positive, so only row 0's excluded pair
+  // (0, n-1) needs an explicit test.  Per slot the first minimum (strict <) and its row are kept.
+  auto row_step = [&](int i, bool first_row) {
+    const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
     const int* tq = s_tour + i + jq0 + 2;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) gnext[q] = row_at(rn, tq[q * kTwoOptBlock]);
-    const int eab = s_edge[i];
-    const int lim = (i == 0) ? n - 1 : n;
-    const uint32_t base = (uint32_t)i * (uint32_t)n;
+    const int neg_eab = -s_edge[i];
     const int* eq = s_edge + i + jq0;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
-      const int j = i + jq0 + q * kTwoOptBlock;
-      const int delta = carry[q] + gcur[q] - eab - eq[q * kTwoOptBlock];
-      const bool take = (j < lim) & (delta < bd);
-      bd = take ? delta : bd;
-      bi = take ? base + (uint32_t)j : bi;
+      int delta = carry[q] + gcur[q] + neg_eab - eq[q * kTwoOptBlock];
+      if (first_row) delta = (jq0 + q * kTwoOptBlock < n - 1) ? delta : 1;
+      const bool take = delta < bd[q];
+      bd[q] = take ? delta : bd[q];
+      bi[q] = take ? i : bi[q];
       carry[q] = gcur[q];
       gcur[q] = gnext[q];
     }
+  };
+  row_step(0, true);
+  int i = 1;
+#pragma unroll 2
+  for (; i < rows; ++i) {
+    if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
+    row_step(i, false);
   }
-  return bd < 0 ? pack_key(bd, bi) : kNoMove;
+  uint64_t best = kNoMove;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (bd[q] < 0) {
+      const uint32_t r = (uint32_t)bi[q];
+      const uint64_t key = pack_key(bd[q], r * (uint32_t)n + r + (uint32_t)(jq0 + q * kTwoOptBlock));
+      best = key < best ? key : best;
+    }
+  }
+  return best;
 }
 
 template <int QMAX>
@@ -136,9 +149,11 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
     s_tour[x] = v;
     if (x == 0) s_tour[n] = v;
   }
+  // Padding: node 0 past the tour (a valid gather target) and a sentinel edge cost past the edges
+  // that keeps every out-of-row delta positive (0 <= D < 2^29, so carry + g - eab + 2^30 > 0).
   for (int x = tid; x < pad; x += blockDim.x) {
     s_tour[n + 1 + x] = 0;
-    s_edge[n + x] = 0;
+    s_edge[n + x] = -(1 << 30);
   }
   __syncthreads();
   for (int x = tid; x < n; x += blockDim.x) {
