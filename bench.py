@@ -160,10 +160,12 @@ def main() -> int:
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--no-extras", action="store_true")
   ap.add_argument("--generations", type=int, default=200)
+  ap.add_argument("--copy-probe-mib", type=int, default=0,
+                  help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one MI355X only: --gpus {args.gpus} is not supported (every GPU box here has one GPU; "
-          "tours shard by population with no exchange, see DESIGN.md).", file=sys.stderr)
+          "tours shard by population with no exchange).", file=sys.stderr)
     return 2
 
   import torch
@@ -171,6 +173,14 @@ def main() -> int:
   from tsp_gpu_clean_amd import _native
 
   _native.require_device()
+  if args.copy_probe_mib > 0:
+    src = torch.empty(args.copy_probe_mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+      dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20, "copies": 3}))
+    return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
   _, D, start = c2_inputs(torch, _native)
   sec, deltas, moves, kms, _ = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
