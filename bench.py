@@ -58,9 +58,15 @@ def time_two_opt(torch, native, D, start, cand, steps, warmup):
   """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch)."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  if cand is not None:
+    table = torch.empty((cand.shape[0], native.candidate_lanes(cand.shape[1]), 2), dtype=torch.int32, device="cuda")
+    native.candidate_table(cand, D, table)
+    run = lambda: native.two_opt_packed(work, D, table, cand.shape[1], -1, stats)  # noqa: E731
+  else:
+    run = lambda: native.two_opt(work, D, None, 0, -1, stats)  # noqa: E731
   for _ in range(warmup):
     work.copy_(start)
-    native.two_opt(work, D, cand, 0, -1, stats)
+    run()
   torch.cuda.synchronize()
   stats.zero_()
   stream = torch.cuda.current_stream()
@@ -69,7 +75,7 @@ def time_two_opt(torch, native, D, start, cand, steps, warmup):
   for i in range(steps):
     work.copy_(start)
     ev[i][0].record(stream)
-    native.two_opt(work, D, cand, 0, -1, stats)
+    run()
     ev[i][1].record(stream)
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0

@@ -79,6 +79,24 @@ int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const 
                   int mode, int max_iter, int64_t* stats, void* stream);
 
 /*
+ * Packed candidate table for tspmi_two_opt_packed: out is (n, KP) pairs of int32
+ * {N[u, r], D[u, N[u, r]]} (pairs {-1, 0} for k <= r < KP) with KP = 8, 16 or 32 — the smallest of
+ * those >= k.  Build once per (instance, k); 1 <= k <= 32.
+ * Replaces: the per-pass candidate lookups of src/algorithms/strategies/local_search/
+ *           kernels/two_opt_batch.cu:122-157 (candidate scan), with the first distance of every
+ *           candidate pair precomputed.
+ */
+int tspmi_candidate_table(const int32_t* cand, int n, int k, const int32_t* D, int ld, int32_t* out, void* stream);
+
+/*
+ * Steepest (best-mode) candidate-list 2-opt on a packed table from tspmi_candidate_table: the
+ * same pairs, order, tie-break and moves as tspmi_two_opt(mode = BEST, cand, k), faster.
+ * Replaces: src/algorithms/strategies/local_search/two_opt.py:831-899 in candidate mode.
+ */
+int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                         int max_iter, int64_t* stats, void* stream);
+
+/*
  * Tournament selection: out[r] = pop[w_r] with w_r = cand[r, argmin_c fitness[cand[r, c]]]
  * (first minimum).  cand is (num_parents, k) int32 drawn on the host.
  * Replaces: src/algorithms/strategies/selection/tournament_selection.py:78-128 and

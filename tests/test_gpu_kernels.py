@@ -348,3 +348,57 @@ def test_nn_init_matches_oracle(torch_mod, nat, berlin52_coords, k):
   work = torch_mod.empty(2 * P * n, dtype=torch_mod.int32, device="cuda")
   nat.nn_init(cu(seeds.view(np.int64)), cu(np.arange(n, dtype=np.int32)), cu(cand), work, out)
   np.testing.assert_array_equal(out.cpu().numpy(), oinit.nn_init(np.arange(n), P, cand, seeds))
+
+
+# ---- packed candidate 2-opt ------------------------------------------------------------------------
+
+
+def _run_packed(nat, torch, tours, d, cand, iters):
+  D = cu(d)
+  c = cu(cand)
+  table = torch.empty((cand.shape[0], nat.candidate_lanes(cand.shape[1]), 2), dtype=torch.int32, device="cuda")
+  nat.candidate_table(c, D, table)
+  t = cu(tours.astype(np.int32))
+  stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  nat.two_opt_packed(t, D, table, cand.shape[1], iters, stats)
+  return t.cpu().numpy(), stats.cpu().numpy(), table.cpu().numpy()
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 12, 20, 32])
+@pytest.mark.parametrize("n", [5, 17, 40, 300])
+def test_packed_candidate_two_opt_matches_oracle(torch_mod, nat, k, n):
+  say(f"two_opt_packed n={n} k={k}")
+  d = rounded_matrix(n, seed=n + k)
+  cand = nearest(d, k)
+  tours = np.stack([np.random.default_rng(n * 7 + t).permutation(n) for t in range(6)]).astype(np.int32)
+  for iters in (1, 4, 25):
+    got, _, table = _run_packed(nat, torch_mod, tours, d, cand, iters)
+    np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, "best", iters))
+  kk = cand.shape[1]
+  np.testing.assert_array_equal(table[:, :kk, 0], cand)
+  np.testing.assert_array_equal(table[:, :kk, 1], np.take_along_axis(d, cand, axis=1))
+  assert (table[:, kk:, 0] == -1).all()
+
+
+def test_packed_candidate_ties_and_duplicates(torch_mod, nat):
+  say("two_opt_packed lattice + duplicates")
+  d = lattice_matrix(64)
+  cand = nearest(d, 6)
+  cand[:, 4] = cand[:, 2]
+  tours = np.stack([np.random.default_rng(t).permutation(64) for t in range(8)]).astype(np.int32)
+  got, _, _ = _run_packed(nat, torch_mod, tours, d, cand, 30)
+  np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, "best", 30))
+
+
+@pytest.mark.parametrize("P", [24, 512])  # 1024-thread (few tours) and 256-thread workgroups
+def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
+  say(f"two_opt_packed n=1000 P={P} to optimum")
+  d = compute_distance_matrix(uniform_coords(1000))
+  cand = compute_candidate_list(d, 20)
+  tours = oinit.random_init(np.arange(1000), P, oinit.draw_row_seeds(np.random.default_rng(3), P))
+  got, stats, _ = _run_packed(nat, torch_mod, tours, d, cand, -1)
+  rows = list(range(0, P, max(1, P // 12)))
+  want, wstats = c_two_opt.improve_batch(tours[rows], d, cand, "best", -1)
+  np.testing.assert_array_equal(got[rows], want)
+  same, _ = c_two_opt.improve_batch(got, d, cand, "best", 1)
+  np.testing.assert_array_equal(same, got)

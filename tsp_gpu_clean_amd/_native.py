@@ -33,6 +33,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_knn": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
+  "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
+  "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
   "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
@@ -179,6 +181,37 @@ def two_opt(tours, D, cand=None, mode: int = 0, max_iter: int = 10, stats=None) 
   P, n = tours.shape
   _check(_lib.tspmi_two_opt(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(cand), k, int(mode), int(max_iter),
                             _ptr(stats), _stream(torch)), "tspmi_two_opt")
+
+
+def candidate_lanes(k: int) -> int:
+  """Row width KP of the packed candidate table for k candidates."""
+  return 8 if k <= 8 else (16 if k <= 16 else 32)
+
+
+def candidate_table(cand, D, out) -> None:
+  torch = require_device()
+  _need(cand, torch.int32, "cand", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(out, torch.int32, "out", 3)
+  n, k = cand.shape
+  if out.shape != (n, candidate_lanes(k), 2):
+    raise ValueError(f"candidate_table: out shape {tuple(out.shape)} != ({n}, {candidate_lanes(k)}, 2)")
+  _check(_lib.tspmi_candidate_table(_ptr(cand), n, k, _ptr(D), D.shape[1], _ptr(out), _stream(torch)),
+         "tspmi_candidate_table")
+
+
+def two_opt_packed(tours, D, table, k: int, max_iter: int = 10, stats=None) -> None:
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(table, torch.int32, "table", 3)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  P, n = tours.shape
+  if table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+    raise ValueError("two_opt_packed: table does not match n / k")
+  _check(_lib.tspmi_two_opt_packed(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
+                                   _ptr(stats), _stream(torch)), "tspmi_two_opt_packed")
 
 
 def tournament(pop, fitness, cand, out, winners_out=None) -> None:

@@ -124,6 +124,20 @@ class ProblemContext:
       self._gpu_candidates[k] = table
     return table
 
+  def get_gpu_candidate_table(self, k: int = 20):
+    """Packed (n, KP, 2) int32 {neighbour, distance} table for the fast candidate 2-opt (k <= 32)."""
+    torch = _native.require_device()
+    key = ("packed", k)
+    table = self._gpu_candidates.get(key)
+    if table is None:
+      cand = self.get_gpu_candidate_list(k)
+      n, k_eff = cand.shape
+      table = torch.empty((n, _native.candidate_lanes(max(1, k_eff)), 2), dtype=torch.int32, device="cuda")
+      if k_eff > 0:
+        _native.candidate_table(cand, self.get_gpu_distances(), table)
+      self._gpu_candidates[key] = table
+    return table
+
   def get_cpu_candidate_list(self, k: int = 20) -> np.ndarray:
     cached = self._cpu_candidates.get(k)
     if cached is None:

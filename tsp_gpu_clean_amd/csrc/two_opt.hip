@@ -294,6 +294,123 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
   }
 }
 
+// Candidate-list steepest 2-opt on a packed candidate table.
+//
+// `ct` is (n, KP) int2 {neighbour, D[node, neighbour]} (neighbour -1 past the k real entries), so
+// the first distance of every candidate pair comes with the neighbour id in one 8-byte load.  A
+// wave covers 64 / KP consecutive rows per step (lane = row offset * KP + r), with no integer
+// division; UNROLL steps are issued together so their loads overlap.  Within a lane the pairs
+// are visited in increasing (i, r) scan order, so a strict `<` keeps the lane's first minimum and
+// the packed (delta, i*k + r) key restores the reference tie-break across lanes.
+template <int BLOCK, int KP, int UNROLL>
+__global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict__ tours, int n,
+                                                             const int32_t* __restrict__ D, int ld,
+                                                             const int2* __restrict__ ct, int k, int max_iter,
+                                                             int64_t* __restrict__ stats) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr int RPW = kWave / KP;  // rows per wave step
+  extern __shared__ int32_t lds[];
+  __shared__ uint64_t red[NW];
+  __shared__ int64_t red64[NW];
+  int32_t* s_tour = lds;           // n + 1 (s_tour[n] mirrors s_tour[0])
+  int32_t* s_edge = lds + n + 1;   // n
+  int32_t* s_pos = s_edge + n;     // n
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  const int sub = lane / KP;       // row offset inside the wave step
+  const int r = lane - sub * KP;   // candidate rank handled by this lane
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+  for (int x = tid; x < n; x += BLOCK) {
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += BLOCK) {
+    const int a = s_tour[x];
+    s_edge[x] = D[(size_t)a * ld + s_tour[x + 1]];
+    s_pos[a] = x;
+  }
+  __syncthreads();
+
+  const int rows = n - 2;
+  int64_t my_deltas = 0;
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    int bd = 0;
+    uint32_t bf = 0xffffffffu;
+    for (int base = wid * RPW * UNROLL; base < rows; base += NW * RPW * UNROLL) {
+      int2 c[UNROLL];
+      int row[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        row[u] = base + u * RPW + sub;
+        const int a = s_tour[row[u] < rows ? row[u] : 0];
+        c[u] = ct[(size_t)a * KP + r];
+      }
+      int g[UNROLL], jj[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int i = row[u];
+        const int nb = c[u].x;
+        int j = (nb >= 0 && nb < n) ? s_pos[nb] : -1;
+        const int lim = (i == 0) ? n - 1 : n;
+        j = (r < k && i < rows && j > i + 1 && j < lim) ? j : -1;
+        jj[u] = j;
+        const int b = s_tour[i < rows ? i + 1 : 0];
+        g[u] = D[(size_t)b * ld + s_tour[j >= 0 ? j + 1 : 0]];
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int j = jj[u];
+        if (j >= 0) {
+          const int i = row[u];
+          const int delta = c[u].y + g[u] - s_edge[i] - s_edge[j];
+          ++my_deltas;
+          if (delta < bd) {
+            bd = delta;
+            bf = (uint32_t)i * (uint32_t)k + (uint32_t)r;
+          }
+        }
+      }
+    }
+    uint64_t best = bd < 0 ? pack_key(bd, bf) : kNoMove;
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)k);
+    const int j = s_pos[ct[(size_t)s_tour[i] * KP + (int)(idx - (uint32_t)i * (uint32_t)k)].x];
+    __syncthreads();
+    apply_move(s_tour, s_edge, s_pos, i, j, n, D, ld);
+    ++moves;
+  }
+  for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
+  if (stats != nullptr) {
+    const int64_t total = block_sum_i64(my_deltas, red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)total);
+      atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+      atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+    }
+  }
+}
+
+// Packed candidate table: out[u, r] = {N[u, r], D[u, N[u, r]]} for r < k, {-1, 0} for k <= r < KP.
+__global__ void cand_table_kernel(const int32_t* __restrict__ cand, int n, int k, int kp,
+                                  const int32_t* __restrict__ D, int ld, int2* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * kp) return;
+  const int u = idx / kp, r = idx - u * kp;
+  int2 v = make_int2(-1, 0);
+  if (r < k) {
+    const int nb = cand[(size_t)u * k + r];
+    if (nb >= 0 && nb < n) v = make_int2(nb, D[(size_t)u * ld + nb]);
+  }
+  out[idx] = v;
+}
+
 }  // namespace tspmi
 
 extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* cand, int k,
@@ -331,5 +448,57 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
                        use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, 0, stats);
   }
   TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
+  return TSPMI_OK;
+}
+
+namespace {
+int cand_lanes(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : 32); }
+}  // namespace
+
+extern "C" int tspmi_candidate_table(const int32_t* cand, int n, int k, const int32_t* D, int ld, int32_t* out,
+                                     void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(n >= 0 && k >= 1 && k <= 32 && ld >= n, "tspmi_candidate_table: bad shape n=%d k=%d ld=%d", n, k,
+                  ld);
+  if (n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(cand && D && out, "tspmi_candidate_table: null pointer");
+  const int kp = cand_lanes(k);
+  const int total = n * kp;
+  hipLaunchKernelGGL(cand_table_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, cand, n, k, kp,
+                     D, ld, reinterpret_cast<int2*>(out));
+  TSPMI_CHECK_LAUNCH("tspmi_candidate_table launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table,
+                                    int k, int max_iter, int64_t* stats, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_packed: negative shape");
+  TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_packed: k=%d outside [1, 32]", k);
+  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(tours && D && table, "tspmi_two_opt_packed: null pointer");
+  TSPMI_CHECK_ARG(ld >= n && (long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed: bad ld/n/k");
+  TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0, "tspmi_two_opt_packed: table must be 8-byte aligned");
+  const size_t lds = (size_t)(3 * n + 1) * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_packed: n=%d exceeds the LDS limit", n);
+  hipStream_t s = (hipStream_t)stream;
+  const int2* ct = reinterpret_cast<const int2*>(table);
+  const int kp = cand_lanes(k);
+  // Fewer tours than compute units: give each tour a 1024-thread workgroup.
+  const bool wide = P < 256;
+#define TSPMI_LAUNCH_CAND(BL, KPV)                                                                              \
+  hipLaunchKernelGGL((two_opt_cand_kernel<BL, KPV, 4>), dim3(P), dim3(BL), lds, s, tours, n, D, ld, ct, k, max_iter, \
+                     stats)
+  if (wide) {
+    if (kp == 8) TSPMI_LAUNCH_CAND(1024, 8);
+    else if (kp == 16) TSPMI_LAUNCH_CAND(1024, 16);
+    else TSPMI_LAUNCH_CAND(1024, 32);
+  } else {
+    if (kp == 8) TSPMI_LAUNCH_CAND(256, 8);
+    else if (kp == 16) TSPMI_LAUNCH_CAND(256, 16);
+    else TSPMI_LAUNCH_CAND(256, 32);
+  }
+#undef TSPMI_LAUNCH_CAND
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed launch");
   return TSPMI_OK;
 }

@@ -85,6 +85,9 @@ class DeviceMemeticLoop:
     self.ls_iters = int(ls_max_iterations)
     self.D = context.get_gpu_distances()
     self.cand = context.get_gpu_candidate_list(ls_candidate_k) if ls_candidate_k else None
+    self.cand_k = 0 if self.cand is None else int(self.cand.shape[1])
+    self.table = (context.get_gpu_candidate_table(ls_candidate_k)
+                  if self.cand is not None and 1 <= self.cand_k <= 32 else None)
     P, n = self.P, self.n
     dev = "cuda"
     self.pop = [torch.empty((P, n), dtype=torch.int32, device=dev) for _ in range(2)]
@@ -127,7 +130,10 @@ class DeviceMemeticLoop:
     if self.elite:
       _native.rank_select(kb[:P], self.elite, self.eidx)
       _native.gather_rows(self.pop[b], None, None, None, self.eidx, self.elites, None)
-      _native.two_opt(self.elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
+      if self.table is not None:
+        _native.two_opt_packed(self.elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
+      else:
+        _native.two_opt(self.elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
       _native.tour_cost(self.elites, self.D, self.efit)
       _native.scatter_rows(self.elites, self.efit, self.eidx, self.pop[b], kb[:P])
     if best_out is not None:

@@ -81,7 +81,11 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       self._stats = torch.zeros(3, dtype=torch.int64, device="cuda")
     else:
       self._stats.zero_()
-    _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
+    if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
+      table = context.get_gpu_candidate_table(int(self.candidate_k))
+      _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
+    else:
+      _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
     self.kernel_launches += 1
     if on_gpu:
       return restore_routes_shape(tours, was_1d)
