@@ -101,9 +101,14 @@ def time_generations(torch, native, generations=200, warmup=5):
     loop.step(draws, g)
   torch.cuda.synchronize()
   best = torch.empty(generations, dtype=torch.int32, device="cuda")
+  start_pop = loop.population.clone()
+  start_fit = loop.fitness.clone()
+  generations -= generations % 2
+  graph = loop.capture(draws, generations, best, start=warmup)
+  loop.load(start_pop, start_fit)
+  torch.cuda.synchronize()
   t0 = time.perf_counter()
-  for g in range(generations):
-    loop.step(draws, warmup + g, best[g : g + 1])
+  graph.replay()
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   best_cost = float(loop.fitness.min().item())

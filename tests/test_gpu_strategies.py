@@ -186,3 +186,38 @@ def test_device_loop_matches_step_pipeline_and_oracle(t):
     np.testing.assert_array_equal(loop.population.cpu().numpy(), res.populations[g], err_msg=f"generation {g + 1}")
     np.testing.assert_array_equal(loop.fitness.cpu().numpy(), res.fitnesses[g])
   assert ctx.rng.bit_generator.state == rng.bit_generator.state
+
+
+def test_device_loop_graph_replays_match_eager(t):
+  """A captured run of generations replayed several times (eager work and syncs in between)
+  gives the eager result every time."""
+  import torch
+
+  from tsp_gpu_clean_amd.engine import DeviceMemeticLoop, draw_generations, upload
+  from tsp_gpu_clean_amd.rng import draw_row_seeds
+
+  say("device loop hipGraph replays")
+  n, P, G = 150, 64, 8
+  ctx = t.ProblemContext(clustered_problem(t, n, seed=3), seed=11)
+  seeds = draw_row_seeds(ctx.rng, P)
+  pop0 = torch.from_numpy(oinit.random_init(np.arange(n), P, seeds)).cuda()
+  draws = upload(draw_generations(ctx.rng, G, P, n, 5))
+  loop = DeviceMemeticLoop(ctx, P, 0.2, tournament_k=5, elite_count=8, ls_max_iterations=5, ls_candidate_k=10)
+  best = torch.empty(G, dtype=torch.int32, device="cuda")
+  loop.load(pop0)
+  loop.run(draws, G, best)
+  want_pop = loop.population.cpu().numpy()
+  want_fit = loop.fitness.cpu().numpy()
+  want_best = best.cpu().numpy()
+  loop.load(pop0)
+  graph = loop.capture(draws, G, best)
+  for rep in range(3):
+    loop.load(pop0)
+    best.zero_()
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(loop.population.cpu().numpy(), want_pop, err_msg=f"replay {rep}")
+    np.testing.assert_array_equal(loop.fitness.cpu().numpy(), want_fit)
+    np.testing.assert_array_equal(best.cpu().numpy(), want_best)
+    _ = (loop.population.float() * 2).sum().item()  # unrelated eager work between replays

@@ -11,8 +11,9 @@ step-by-step ``GeneticAlgorithm`` pipeline.  Per generation the device runs:
   1. tspmi_generation  : fused tournament -> PMX -> inversion -> cost into the offspring buffer
   2. tspmi_rank_select : stable (mu + lambda) order of concat(fit, off_fit)
   3. tspmi_gather_rows : survivors into the other population buffer (ping-pong)
-  4. elite 2-opt       : rank_select(k) -> gather -> tspmi_two_opt -> tspmi_tour_cost -> scatter
+  4. elite 2-opt       : tspmi_two_opt_packed + tspmi_tour_cost on the sorted survivors' prefix
   5. rank_select(1)    : index of the generation's best row, kept on device for the history
+Whole runs of generations can be captured into one HIP graph (``capture``).
 """
 
 from __future__ import annotations
@@ -94,10 +95,6 @@ class DeviceMemeticLoop:
     self.keys = [torch.empty(2 * P, dtype=torch.float64, device=dev) for _ in range(2)]
     self.off = torch.empty((P, n), dtype=torch.int32, device=dev)
     self.order = torch.empty(P, dtype=torch.int32, device=dev)
-    if self.elite:
-      self.eidx = torch.empty(self.elite, dtype=torch.int32, device=dev)
-      self.elites = torch.empty((self.elite, n), dtype=torch.int32, device=dev)
-      self.efit = torch.empty(self.elite, dtype=torch.float64, device=dev)
     self.stats = torch.zeros(3, dtype=torch.int64, device=dev)
     self.cur = 0
 
@@ -119,7 +116,13 @@ class DeviceMemeticLoop:
     return self.keys[self.cur][: self.P]
 
   def step(self, draws: GenerationDraws, g: int, best_out=None) -> None:
-    """One generation using draw slot ``g``; ``best_out`` (device int32 (1,)) receives the best row."""
+    """One generation using draw slot ``g``; ``best_out`` (device int32 (1,)) receives the best row.
+
+    Survivors are gathered in their stable rank order, so the elite set of the reference's
+    ``argsort(fitness)[:elite]`` (pipeline_steps.py:163) is simply rows 0..elite-1: the elite
+    2-opt refines that prefix in place and re-scores it into the same key slots.  Because local
+    search never raises a cost, the generation's first minimum then lies within that prefix.
+    """
     P = self.P
     a, b = self.cur, 1 - self.cur
     ka, kb = self.keys[a], self.keys[b]
@@ -128,19 +131,36 @@ class DeviceMemeticLoop:
     _native.rank_select(ka, P, self.order)
     _native.gather_rows(self.pop[a], ka[:P], self.off, ka[P:], self.order, self.pop[b], kb[:P])
     if self.elite:
-      _native.rank_select(kb[:P], self.elite, self.eidx)
-      _native.gather_rows(self.pop[b], None, None, None, self.eidx, self.elites, None)
+      E = self.elite
+      elites = self.pop[b][:E]
       if self.table is not None:
-        _native.two_opt_packed(self.elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
+        _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
       else:
-        _native.two_opt(self.elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
-      _native.tour_cost(self.elites, self.D, self.efit)
-      _native.scatter_rows(self.elites, self.efit, self.eidx, self.pop[b], kb[:P])
+        _native.two_opt(elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
+      _native.tour_cost(elites, self.D, kb[:E])
     if best_out is not None:
-      _native.rank_select(kb[:P], 1, best_out)
+      _native.rank_select(kb[: (self.elite or 1)], 1, best_out)
     self.cur = b
 
-  def run(self, draws: GenerationDraws, generations: int, best_idx=None) -> None:
-    """Run ``generations`` steps; ``best_idx`` (device int32 (G,)) collects each generation's best row."""
-    for g in range(generations):
-      self.step(draws, g, None if best_idx is None else best_idx[g : g + 1])
+  def run(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0) -> None:
+    """Run ``generations`` steps from draw slot ``start``; ``best_idx`` collects each best row."""
+    for t in range(generations):
+      self.step(draws, start + t, None if best_idx is None else best_idx[t : t + 1])
+
+  def capture(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0):
+    """Capture ``generations`` (even) steps into one HIP graph; replaying it advances the loop.
+
+    The graph reads the buffers of the current population slot and, the count being even, ends
+    in the same slot, so every replay continues from the state the previous one left.  Each
+    replay consumes draw slots start .. start + generations - 1 again: refill them (or capture a
+    whole run) to keep the random stream advancing.
+    """
+    torch = _native.require_device()
+    if generations % 2:
+      raise ValueError("capture needs an even number of generations")
+    graph = torch.cuda.CUDAGraph()
+    cur0 = self.cur
+    with torch.cuda.graph(graph):
+      self.run(draws, generations, best_idx, start)
+    assert self.cur == cur0
+    return graph
