@@ -115,6 +115,35 @@ def time_generations(torch, native, generations=200, warmup=5):
   return generations / wall, wall, best_cost
 
 
+def time_uneven_batch(torch, generations=100):
+  """Config 4: 256 instances, n uniform in [50, 2000], P=64 each, 100 graph-captured generations."""
+  from tsp_gpu_clean_amd.batch import BatchedMemeticGA, c4_instances
+
+  coords = c4_instances(256, seed=7)
+  t0 = time.perf_counter()
+  ga = BatchedMemeticGA(coords, population_size=64, mutation_rate=0.05, tournament_k=5, elite_count=16,
+                        ls_max_iterations=10, ls_candidate_k=20, seeds=list(range(256)))
+  ga.init_random()
+  torch.cuda.synchronize()
+  setup = time.perf_counter() - t0
+  d = ga.draw(generations + 2)
+  shapes = ga.autotune(d, g=generations)
+  ga.run(d, 2, start=generations)  # warm-up generations on the spare draw slots
+  graph = ga.capture(d, generations)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  graph.replay()
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  ns = np.array([c.shape[0] for c in coords])
+  return {"instances": 256, "n_range": [int(ns.min()), int(ns.max())], "population": 64, "generations": generations,
+          "wall_s": wall, "gens_per_s": generations / wall, "instance_gens_per_s": 256 * generations / wall,
+          "setup_s": setup, "distance_bytes": int(4 * (ns * ns).sum()), "shapes": shapes,
+          "workload": "256 instances, n uniform-integer [50,2000], uniform coords [0,1e6]^2, P=64 each, tournament 5, "
+                      "PMX, inversion 0.05, elite-16 2-opt best k=20 max_iter 10, one launch per stage for all "
+                      "instances, 100 generations captured in one HIP graph, autotuned launch shapes"}
+
+
 def cpu_baseline(seconds_hint: float = 20.0) -> dict:
   """Reference-faithful NumPy 2-opt (single core) on a bounded sample of the C2 workload."""
   code = f"""
@@ -237,6 +266,8 @@ def main() -> int:
     line["ga"] = {"generations": args.generations, "wall_s": gwall, "best_cost": gbest,
                   "workload": "n=1000 clustered (20 Gaussian clusters, sigma 2% of 1e6), P=1024, tournament 5, PMX, "
                               "inversion 0.05, elite-16 2-opt best k=20 max_iter 10"}
+    print("bench: uneven batch (256 instances)", file=sys.stderr, flush=True)
+    line["uneven_batch"] = time_uneven_batch(torch)
   if not args.no_cpu_baseline:
     print("bench: CPU baseline (NumPy restatement, 1 core)", file=sys.stderr, flush=True)
     line["cpu_baseline"] = cpu_baseline()

@@ -175,6 +175,70 @@ int tspmi_random_init(const uint64_t* seeds, int P, const int32_t* customers, in
 int tspmi_nn_init(const uint64_t* seeds, int P, const int32_t* customers, int n, const int32_t* cand,
                   int k, int32_t* work, int32_t* out, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Uneven multi-instance batch (many instances of different n, each with its own population).
+ *
+ * `desc` is a device int64 table with TSPMI_BATCH_FIELDS records per instance:
+ *   [0] n, [1] P, [2] xy offset (float64 elements), [3] D offset (int32; D is n x n),
+ *   [4] packed candidate table offset (int2 pairs; n x 32), [5] k-NN offset (int32; n x k),
+ *   [6] population offset (int32; P x n), [7] row offset (sum of P over earlier instances),
+ *   [8] pair offset (sum of P/2 over earlier instances).
+ * Key buffers hold 2P float64 per instance at 2 * row offset: parents' fitness then offspring's.
+ * Host-drawn random buffers of one generation are concatenated in instance order: tour_cand
+ * (rows x k), cut_lo/cut_hi (pairs), probs/pos1/pos2 (rows).
+ * Work lists are flat (item_inst, item_idx) int32 arrays; order them longest-first.  Every stage
+ * runs the same device code as the single-instance entry points, so each instance evolves exactly
+ * as it would alone (genetic_algorithm.py:789-965 per instance).
+ * ------------------------------------------------------------------------------------------- */
+#define TSPMI_BATCH_FIELDS 9
+
+/* Distance rows: items are (instance, row u).  Replaces matrix.py:214-273 per instance. */
+int tspmi_batch_dist_euc2d(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                           const double* xy_all, int32_t* D_all, void* stream);
+
+/* k-NN rows (k <= 32) and packed candidate rows: items are (instance, row u).  Replaces
+ * matrix.py:140-211 per instance. */
+int tspmi_batch_knn(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems, int k,
+                    const int32_t* D_all, int32_t* cand_all, int32_t* table_all, void* stream);
+
+/* SplitMix64 random permutations of 0..n-1: items are (instance, row); seeds indexed by row
+ * offset + row.  Replaces random_init.py:24-118 per instance. */
+int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                            const uint64_t* seeds, int32_t* pop, void* stream);
+
+/* Tour costs of listed (instance, row) items into key slot 2 * row offset + row.  Replaces
+ * tour_cost.py:17-65 per instance. */
+int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                     const int32_t* pop, const int32_t* D_all, double* keys, void* stream);
+
+/* Fused generation of every instance: items are (instance, child pair q); block in
+ * {128, 256, 512, 1024}; grid <= 0 means one workgroup per item, else a persistent grid.
+ * Replaces genetic_algorithm.py:120-213 per instance. */
+int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                           int max_n, const int32_t* popA, double* keysA, const int32_t* D_all,
+                           const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                           const double* probs, const int32_t* pos1, const int32_t* pos2, double rate, int32_t* off,
+                           int block, int grid, void* stream);
+
+/* Stable (mu + lambda) survivor order of every instance (2P <= 4096) into order (P per instance,
+ * at row offset).  Replaces genetic_algorithm.py:618-706 per instance. */
+int tspmi_batch_survival(const int64_t* desc, int B, const double* keysA, int32_t* order, void* stream);
+
+/* Survivor rows and keys into popB/keysB in rank order: items are (instance, destination row). */
+int tspmi_batch_gather(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                       const int32_t* popA, const double* keysA, const int32_t* off, const int32_t* order,
+                       int32_t* popB, double* keysB, void* stream);
+
+/* Best-mode candidate 2-opt of listed (instance, row) tours in place, block in {256, 512, 1024},
+ * grid <= 0 one workgroup per item, else persistent.  Replaces pipeline_steps.py:153-195 +
+ * two_opt.py:831-899 per instance. */
+int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                               int max_n, int32_t* pop, const int32_t* D_all, const int32_t* table_all, int k,
+                               int max_iter, int block, int grid, int64_t* stats, void* stream);
+
+/* First minimum among the first `count` keys of every instance.  Replaces genetic_algorithm.py:708-739. */
+int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

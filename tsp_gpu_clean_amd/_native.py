@@ -44,6 +44,15 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_scatter_rows": [_P, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_random_init": [_P, _I, _P, _I, _P, _P],
   "tspmi_nn_init": [_P, _I, _P, _I, _P, _I, _P, _P, _P],
+  "tspmi_batch_dist_euc2d": [_P, _P, _P, _I, _P, _P, _P],
+  "tspmi_batch_knn": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
+  "tspmi_batch_random_init": [_P, _P, _P, _I, _P, _P, _P],
+  "tspmi_batch_cost": [_P, _P, _P, _I, _P, _P, _P, _P],
+  "tspmi_batch_generation": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _I, _P],
+  "tspmi_batch_survival": [_P, _I, _P, _P, _P],
+  "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
+  "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
+  "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
 }
 
 EXPORTED = tuple(_SIGNATURES) + ("tspmi_last_error",)
@@ -318,3 +327,20 @@ def nn_init(seeds, customers, cand, work, out) -> None:
     raise ValueError("nn_init: work needs 2 * P * n int32")
   _check(_lib.tspmi_nn_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(cand), k, _ptr(work),
                             _ptr(out), _stream(torch)), "tspmi_nn_init")
+
+
+# ---- uneven multi-instance batch (raw pointers; shapes are validated by the batch engine) -------
+
+
+def batch_call(name: str, *args) -> None:
+  """Call a ``tspmi_batch_*`` entry point with device tensors / ints / floats, on the current stream."""
+  torch = require_device()
+  conv = []
+  for a in args:
+    if isinstance(a, torch.Tensor):
+      if not a.is_cuda or not a.is_contiguous():
+        raise ValueError(f"{name}: tensor arguments must be contiguous device tensors")
+      conv.append(int(a.data_ptr()))
+    else:
+      conv.append(a)
+  _check(getattr(_lib, name)(*conv, _stream(torch)), name)

@@ -1,0 +1,381 @@
+// Multi-instance ("uneven batch") kernels: many TSP instances of different n, each with its own
+// population, advanced by one launch per pipeline stage that covers every instance (gfx950).
+//
+// Instances are described by a device table of int64 records (TSPMI_BATCH_FIELDS per instance):
+//   n, P, xy offset (doubles), D offset (int32), packed-table offset (int2), kNN offset (int32),
+//   population offset (int32), row offset (sum of earlier P), pair offset (sum of earlier P/2).
+// Work is listed on the host as flat (instance, index) arrays, ordered longest-first so the
+// persistent grids below balance uneven instances.  Every stage reuses the single-instance device
+// bodies, so a batched generation is bit-identical to running each instance alone.
+#include "ga_dev.hpp"
+#include "two_opt_dev.hpp"
+
+namespace tspmi {
+
+enum : int {
+  kBN = 0,
+  kBP = 1,
+  kBXY = 2,
+  kBD = 3,
+  kBCT = 4,
+  kBCand = 5,
+  kBPop = 6,
+  kBRow = 7,
+  kBPair = 8,
+};
+
+struct Inst {
+  int n, P;
+  int64_t xy, d, ct, cand, pop, row, pair;
+};
+
+__device__ __forceinline__ Inst load_inst(const int64_t* __restrict__ desc, int i) {
+  const int64_t* r = desc + (size_t)i * TSPMI_BATCH_FIELDS;
+  return Inst{(int)r[kBN], (int)r[kBP], r[kBXY], r[kBD], r[kBCT], r[kBCand], r[kBPop], r[kBRow], r[kBPair]};
+}
+
+// D rows of every instance: one workgroup per (instance, row u).
+__global__ __launch_bounds__(256) void batch_dist_kernel(const int64_t* __restrict__ desc,
+                                                         const int32_t* __restrict__ item_inst,
+                                                         const int32_t* __restrict__ item_idx, int nitems,
+                                                         const double* __restrict__ xy_all, int32_t* __restrict__ D_all) {
+  const int it = blockIdx.x;
+  if (it >= nitems) return;
+  const Inst I = load_inst(desc, item_inst[it]);
+  const int u = item_idx[it];
+  const double* xy = xy_all + I.xy;
+  int32_t* row = D_all + I.d + (size_t)u * I.n;
+  const double xu = xy[2 * u], yu = xy[2 * u + 1];
+  for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
+#pragma clang fp contract(off)
+    const double dx = xu - xy[2 * v];
+    const double dy = yu - xy[2 * v + 1];
+    const double s = dx * dx + dy * dy;
+    row[v] = (u == v) ? 0 : (int)(__builtin_sqrt(s) + 0.5);
+  }
+}
+
+// k-NN rows and packed candidate rows: one workgroup per (instance, row u).
+__global__ __launch_bounds__(256) void batch_knn_kernel(const int64_t* __restrict__ desc,
+                                                        const int32_t* __restrict__ item_inst,
+                                                        const int32_t* __restrict__ item_idx, int nitems, int k,
+                                                        const int32_t* __restrict__ D_all, int32_t* __restrict__ cand_all,
+                                                        int2* __restrict__ ct_all) {
+  __shared__ uint64_t red[256 / kWave];
+  const int it = blockIdx.x;
+  if (it >= nitems) return;
+  const Inst I = load_inst(desc, item_inst[it]);
+  const int u = item_idx[it];
+  const int n = I.n;
+  const int ke = k < n - 1 ? k : n - 1;
+  const int32_t* row = D_all + I.d + (size_t)u * n;
+  int32_t* out = cand_all + I.cand + (size_t)u * k;
+  int2* ct = ct_all + I.ct + (size_t)u * 32;
+  uint64_t prev = 0;
+  for (int r = 0; r < ke; ++r) {
+    uint64_t best = kNoMove;
+    for (int v = threadIdx.x; v < n; v += blockDim.x) {
+      if (v == u) continue;
+      const uint64_t key = ((uint64_t)(uint32_t)row[v] << 32) | (uint32_t)v;
+      if ((r == 0 || key > prev) && key < best) best = key;
+    }
+    best = block_min_u64(best, red);
+    if (threadIdx.x == 0) {
+      const int nb = (int)(uint32_t)best;
+      out[r] = nb;
+      ct[r] = make_int2(nb, (int)(best >> 32));
+    }
+    prev = best;
+  }
+  for (int r = ke + (int)threadIdx.x; r < 32; r += blockDim.x) {
+    if (r < k) out[r] = -1;
+    ct[r] = make_int2(-1, 0);
+  }
+}
+
+// SplitMix64 Fisher-Yates rows: one thread per (instance, row).
+__global__ void batch_random_init_kernel(const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst,
+                                         const int32_t* __restrict__ item_idx, int nitems,
+                                         const uint64_t* __restrict__ seeds, int32_t* __restrict__ pop) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  const Inst I = load_inst(desc, item_inst[it]);
+  const int r = item_idx[it];
+  int32_t* row = pop + I.pop + (size_t)r * I.n;
+  for (int x = 0; x < I.n; ++x) row[x] = x;
+  uint64_t s = seeds[I.row + r];
+  for (int i = I.n - 1; i > 0; --i) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const int j = (int)(((z >> 32) * (uint64_t)(i + 1)) >> 32);
+    const int32_t t = row[i];
+    row[i] = row[j];
+    row[j] = t;
+  }
+}
+
+// Tour costs of listed rows into key slots: one wave per (instance, row).
+__global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restrict__ desc,
+                                                         const int32_t* __restrict__ item_inst,
+                                                         const int32_t* __restrict__ item_idx, int nitems,
+                                                         const int32_t* __restrict__ pop, const int32_t* __restrict__ D_all,
+                                                         double* __restrict__ keys) {
+  const int it = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (it >= nitems) return;
+  const Inst I = load_inst(desc, item_inst[it]);
+  const int r = item_idx[it];
+  const int32_t* t = pop + I.pop + (size_t)r * I.n;
+  const int32_t* D = D_all + I.d;
+  int64_t s = 0;
+  for (int x = lane; x < I.n; x += kWave) {
+    const int a = clamp_id(t[x], I.n);
+    const int b = clamp_id(t[x + 1 == I.n ? 0 : x + 1], I.n);
+    if (I.n > 1) s += D[(size_t)a * I.n + b];
+  }
+  s = wave_sum_i64(s);
+  if (lane == 0) keys[2 * I.row + r] = (double)s;
+}
+
+// Fused generation of every instance: one workgroup per (instance, child pair), persistent grid.
+__global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst,
+                                        const int32_t* __restrict__ item_idx, int nitems,
+                                        const int32_t* __restrict__ popA, double* __restrict__ keysA,
+                                        const int32_t* __restrict__ D_all, const int32_t* __restrict__ tour_cand, int k,
+                                        const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
+                                        const double* __restrict__ probs, const int32_t* __restrict__ pos1,
+                                        const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ off) {
+  extern __shared__ int32_t lds[];
+  __shared__ int64_t red[1024 / kWave];
+  __shared__ int s_w[2];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Inst I = load_inst(desc, item_inst[it]);
+    double* keys = keysA + 2 * I.row;
+    generation_pair(item_idx[it], popA + I.pop, keys, I.P, I.n, D_all + I.d, I.n, tour_cand + (size_t)I.row * k, k,
+                    cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
+                    keys + I.P, lds, red, s_w);
+    __syncthreads();
+  }
+}
+
+// (mu + lambda) survivor order of every instance: one workgroup per instance (2P <= 4096).
+__global__ __launch_bounds__(kSortBlock) void batch_survival_kernel(const int64_t* __restrict__ desc, int B,
+                                                                    const double* __restrict__ keysA,
+                                                                    int32_t* __restrict__ order) {
+  __shared__ uint64_t s_key[kSortMax];
+  __shared__ int32_t s_idx[kSortMax];
+  const int i = blockIdx.x;
+  if (i >= B) return;
+  const Inst I = load_inst(desc, i);
+  const int M = 2 * I.P;
+  int M2 = 1;
+  while (M2 < M) M2 <<= 1;
+  sort_select_body(keysA + 2 * I.row, M, M2, I.P, order + I.row, s_key, s_idx);
+}
+
+// Survivor rows into the other buffer, in rank order: one wave per (instance, destination row).
+__global__ __launch_bounds__(256) void batch_gather_kernel(const int64_t* __restrict__ desc,
+                                                           const int32_t* __restrict__ item_inst,
+                                                           const int32_t* __restrict__ item_idx, int nitems,
+                                                           const int32_t* __restrict__ popA, const double* __restrict__ keysA,
+                                                           const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ order, int32_t* __restrict__ popB,
+                                                           double* __restrict__ keysB) {
+  const int it = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (it >= nitems) return;
+  const Inst I = load_inst(desc, item_inst[it]);
+  const int d = item_idx[it];
+  const int s = order[I.row + d];
+  const int32_t* src = (s < I.P ? popA + I.pop + (size_t)s * I.n : off + I.pop + (size_t)(s - I.P) * I.n);
+  int32_t* dst = popB + I.pop + (size_t)d * I.n;
+  for (int x = lane; x < I.n; x += kWave) dst[x] = src[x];
+  if (lane == 0) keysB[2 * I.row + d] = keysA[2 * I.row + s];
+}
+
+// Elite candidate 2-opt of every instance: persistent grid over (instance, elite row) items.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __restrict__ desc,
+                                                              const int32_t* __restrict__ item_inst,
+                                                              const int32_t* __restrict__ item_idx, int nitems,
+                                                              int32_t* __restrict__ pop, const int32_t* __restrict__ D_all,
+                                                              const int2* __restrict__ ct_all, int k, int max_iter,
+                                                              int64_t* __restrict__ stats) {
+  extern __shared__ int32_t lds[];
+  __shared__ uint64_t red[BLOCK / kWave];
+  __shared__ int64_t red64[BLOCK / kWave];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Inst I = load_inst(desc, item_inst[it]);
+    const int ke = k < I.n - 1 ? k : I.n - 1;
+    if (I.n >= 4 && ke >= 1)
+      cand_two_opt_tour<BLOCK, 32, 4>(pop + I.pop + (size_t)item_idx[it] * I.n, I.n, D_all + I.d, I.n, ct_all + I.ct, ke,
+                                      max_iter, stats, lds, red, red64);
+    __syncthreads();
+  }
+}
+
+// First minimum of the first `count` keys of every instance: one wave per instance.
+__global__ __launch_bounds__(256) void batch_best_kernel(const int64_t* __restrict__ desc, int B,
+                                                         const double* __restrict__ keys, int count,
+                                                         int32_t* __restrict__ best) {
+  const int i = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (i >= B) return;
+  const Inst I = load_inst(desc, i);
+  const int c = count < I.P ? count : I.P;
+  double mv = 0.0;
+  int mi = 0x7fffffff;
+  for (int x = lane; x < c; x += kWave) {
+    const double v = keys[2 * I.row + x];
+    if (mi == 0x7fffffff || v < mv) {
+      mv = v;
+      mi = x;
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double ov = __shfl_xor(mv, m, kWave);
+    const int oi = __shfl_xor(mi, m, kWave);
+    if (oi != 0x7fffffff && (mi == 0x7fffffff || ov < mv || (ov == mv && oi < mi))) {
+      mv = ov;
+      mi = oi;
+    }
+  }
+  if (lane == 0) best[i] = mi;
+}
+
+}  // namespace tspmi
+
+using namespace tspmi;
+
+#define TSPMI_BATCH_ARGS(nm)                                                      \
+  TSPMI_CHECK_ARG(desc != nullptr && nitems >= 0 && (nitems == 0 || (item_inst && item_idx)), \
+                  nm ": bad work list")
+
+extern "C" int tspmi_batch_dist_euc2d(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                      int nitems, const double* xy_all, int32_t* D_all, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_dist_euc2d");
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(xy_all && D_all, "tspmi_batch_dist_euc2d: null pointer");
+  hipLaunchKernelGGL(batch_dist_kernel, dim3(nitems), dim3(256), 0, (hipStream_t)stream, desc, item_inst, item_idx,
+                     nitems, xy_all, D_all);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_dist_euc2d launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_knn(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                               int k, const int32_t* D_all, int32_t* cand_all, int32_t* table_all, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_knn");
+  TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_batch_knn: k=%d outside [1, 32]", k);
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(D_all && cand_all && table_all, "tspmi_batch_knn: null pointer");
+  hipLaunchKernelGGL(batch_knn_kernel, dim3(nitems), dim3(256), 0, (hipStream_t)stream, desc, item_inst, item_idx,
+                     nitems, k, D_all, cand_all, reinterpret_cast<int2*>(table_all));
+  TSPMI_CHECK_LAUNCH("tspmi_batch_knn launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                       int nitems, const uint64_t* seeds, int32_t* pop, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_random_init");
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(seeds && pop, "tspmi_batch_random_init: null pointer");
+  hipLaunchKernelGGL(batch_random_init_kernel, dim3((nitems + 63) / 64), dim3(64), 0, (hipStream_t)stream, desc,
+                     item_inst, item_idx, nitems, seeds, pop);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_random_init launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                                const int32_t* pop, const int32_t* D_all, double* keys, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_cost");
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(pop && D_all && keys, "tspmi_batch_cost: null pointer");
+  hipLaunchKernelGGL(batch_cost_kernel, dim3((nitems + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, item_inst,
+                     item_idx, nitems, pop, D_all, keys);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_cost launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                      int nitems, int max_n, const int32_t* popA, double* keysA, const int32_t* D_all,
+                                      const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                                      const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
+                                      int32_t* off, int block, int grid, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_generation");
+  TSPMI_CHECK_ARG(block == 128 || block == 256 || block == 512 || block == 1024,
+                  "tspmi_batch_generation: block %d not in {128, 256, 512, 1024}", block);
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(popA && keysA && D_all && tour_cand && cut_lo && cut_hi && probs && pos1 && pos2 && off && k >= 1,
+                  "tspmi_batch_generation: null pointer");
+  const size_t lds = (size_t)6 * max_n * sizeof(int32_t);
+  TSPMI_CHECK_ARG(max_n >= 2 && lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_generation: max_n=%d exceeds LDS", max_n);
+  const int g = grid > 0 && grid < nitems ? grid : nitems;
+  hipLaunchKernelGGL(batch_generation_kernel, dim3(g), dim3(block), lds, (hipStream_t)stream, desc, item_inst, item_idx,
+                     nitems, popA, keysA, D_all, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, off);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_generation launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_survival(const int64_t* desc, int B, const double* keysA, int32_t* order, void* stream) {
+  TSPMI_CHECK_ARG(desc && B >= 0, "tspmi_batch_survival: bad arguments");
+  if (B == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(keysA && order, "tspmi_batch_survival: null pointer");
+  hipLaunchKernelGGL(batch_survival_kernel, dim3(B), dim3(kSortBlock), 0, (hipStream_t)stream, desc, B, keysA, order);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_survival launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_gather(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
+                                  const int32_t* popA, const double* keysA, const int32_t* off, const int32_t* order,
+                                  int32_t* popB, double* keysB, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_gather");
+  if (nitems == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(popA && keysA && off && order && popB && keysB, "tspmi_batch_gather: null pointer");
+  hipLaunchKernelGGL(batch_gather_kernel, dim3((nitems + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, item_inst,
+                     item_idx, nitems, popA, keysA, off, order, popB, keysB);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_gather launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                          int nitems, int max_n, int32_t* pop, const int32_t* D_all,
+                                          const int32_t* table_all, int k, int max_iter, int block, int grid,
+                                          int64_t* stats, void* stream) {
+  TSPMI_BATCH_ARGS("tspmi_batch_two_opt_packed");
+  TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_batch_two_opt_packed: k=%d outside [1, 32]", k);
+  if (nitems == 0 || max_iter == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(pop && D_all && table_all, "tspmi_batch_two_opt_packed: null pointer");
+  const size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + 2048 <= (size_t)kMaxLds, "tspmi_batch_two_opt_packed: max_n=%d exceeds LDS", max_n);
+  const int g = grid > 0 && grid < nitems ? grid : nitems;
+  hipStream_t s = (hipStream_t)stream;
+  const int2* ct = reinterpret_cast<const int2*>(table_all);
+  if (block == 256)
+    hipLaunchKernelGGL(batch_two_opt_kernel<256>, dim3(g), dim3(256), lds, s, desc, item_inst, item_idx, nitems, pop,
+                       D_all, ct, k, max_iter, stats);
+  else if (block == 512)
+    hipLaunchKernelGGL(batch_two_opt_kernel<512>, dim3(g), dim3(512), lds, s, desc, item_inst, item_idx, nitems, pop,
+                       D_all, ct, k, max_iter, stats);
+  else if (block == 1024)
+    hipLaunchKernelGGL(batch_two_opt_kernel<1024>, dim3(g), dim3(1024), lds, s, desc, item_inst, item_idx, nitems, pop,
+                       D_all, ct, k, max_iter, stats);
+  else
+    TSPMI_CHECK_ARG(false, "tspmi_batch_two_opt_packed: block %d not in {256, 512, 1024}", block);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_two_opt_packed launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best,
+                                void* stream) {
+  TSPMI_CHECK_ARG(desc && B >= 0 && count >= 1 && count <= 4096, "tspmi_batch_best: bad arguments");
+  if (B == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(keys && best, "tspmi_batch_best: null pointer");
+  hipLaunchKernelGGL(batch_best_kernel, dim3((B + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, B, keys, count,
+                     best);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_best launch");
+  return TSPMI_OK;
+}

@@ -3,27 +3,9 @@
 //
 // Random numbers are never drawn here: the host draws them from the run's generator in the
 // reference order and passes the buffers in, so every kernel is a pure function of its inputs.
-#include "common.hpp"
+#include "ga_dev.hpp"
 
 namespace tspmi {
-
-constexpr int kRowBlock = 256;
-
-// First minimum of fitness over one tournament row (tournament_selection.py:125-127).
-__device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ cand_row, int k,
-                                                 const double* __restrict__ fitness, int P) {
-  int w = clamp_id(cand_row[0], P);
-  double best = fitness[w];
-  for (int c = 1; c < k; ++c) {
-    const int x = clamp_id(cand_row[c], P);
-    const double f = fitness[x];
-    if (f < best) {
-      best = f;
-      w = x;
-    }
-  }
-  return w;
-}
 
 // One wave per output row.
 __global__ __launch_bounds__(kRowBlock) void tournament_kernel(const int32_t* __restrict__ pop, int P, int n,
@@ -39,46 +21,6 @@ __global__ __launch_bounds__(kRowBlock) void tournament_kernel(const int32_t* __
   const int32_t* src = pop + (size_t)w * n;
   int32_t* dst = out + (size_t)r * n;
   for (int x = lane; x < n; x += kWave) dst[x] = src[x];
-}
-
-// PMX child of (p1, p2) with cuts [c1, c2] into `child` (pmx_crossover.py:49-105, open
-// permutation).  inv1/inv2 map node -> position in p1/p2.  Caller synchronises afterwards.
-__device__ void pmx_child(const int32_t* p1, const int32_t* p2, const int32_t* inv1, const int32_t* inv2, int n,
-                          int c1, int c2, int32_t* child) {
-  for (int x = threadIdx.x; x < n; x += blockDim.x) child[x] = (x >= c1 && x <= c2) ? p1[x] : -1;
-  __syncthreads();
-  const int seg_len = c2 - c1 + 1;
-  for (int idx = c1 + threadIdx.x; idx <= c2; idx += blockDim.x) {
-    const int val = p2[idx];
-    const int pv = inv1[val];
-    if (pv >= c1 && pv <= c2) continue;  // val already in the copied segment
-    int pos = idx, steps = 0;
-    while (pos >= c1 && pos <= c2 && steps <= seg_len) {
-      pos = inv2[p1[pos]];
-      ++steps;
-    }
-    if ((unsigned)pos < (unsigned)n && (pos < c1 || pos > c2)) child[pos] = val;
-  }
-  __syncthreads();
-  for (int x = threadIdx.x; x < n; x += blockDim.x)
-    if (child[x] == -1) child[x] = p2[x];
-}
-
-__device__ __forceinline__ void load_parent(const int32_t* __restrict__ src, int n, int32_t* p, int32_t* inv) {
-  for (int x = threadIdx.x; x < n; x += blockDim.x) {
-    const int v = clamp_id(src[x], n);
-    p[x] = v;
-    inv[v] = x;
-  }
-}
-
-__device__ __forceinline__ void reverse_range(int32_t* t, int lo, int hi) {
-  const int half = (hi - lo + 1) >> 1;
-  for (int q = threadIdx.x; q < half; q += blockDim.x) {
-    const int32_t tmp = t[lo + q];
-    t[lo + q] = t[hi - q];
-    t[hi - q] = tmp;
-  }
 }
 
 // One workgroup per pair (2q, 2q+1); LDS: p1, p2, inv1, inv2 (4n ints).
@@ -138,18 +80,7 @@ __global__ __launch_bounds__(kRowBlock) void inversion_kernel(int32_t* __restric
   }
 }
 
-__device__ __forceinline__ double block_cost(const int32_t* t, int n, const int32_t* __restrict__ D, int ld,
-                                             int64_t* red) {
-  int64_t s = 0;
-  for (int x = threadIdx.x; x < n; x += blockDim.x) {
-    const int nx = (x + 1 == n) ? 0 : x + 1;
-    if (n > 1) s += D[(size_t)t[x] * ld + t[nx]];
-  }
-  return (double)block_sum_i64(s, red);
-}
-
 // Fused tournament -> PMX -> inversion -> evaluate; one workgroup per child pair.
-// LDS: p1, p2, inv1, inv2, childA, childB (6n ints).
 __global__ __launch_bounds__(kRowBlock) void generation_kernel(
     const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P, int n, const int32_t* __restrict__ D,
     int ld, const int32_t* __restrict__ tour_cand, int k, const int32_t* __restrict__ cut_lo,
@@ -158,60 +89,8 @@ __global__ __launch_bounds__(kRowBlock) void generation_kernel(
   extern __shared__ int32_t lds[];
   __shared__ int64_t red[kRowBlock / kWave];
   __shared__ int s_w[2];
-  const int q = blockIdx.x;
-  const int r0 = 2 * q;
-  const bool single = (r0 + 1 >= P);
-  int32_t* p1 = lds;
-  int32_t* p2 = lds + n;
-  int32_t* inv1 = lds + 2 * n;
-  int32_t* inv2 = lds + 3 * n;
-  int32_t* cA = lds + 4 * n;
-  int32_t* cB = lds + 5 * n;
-
-  if (threadIdx.x < 2 && (threadIdx.x == 0 || !single))
-    s_w[threadIdx.x] = tournament_winner(tour_cand + (size_t)(r0 + threadIdx.x) * k, k, fitness, P);
-  __syncthreads();
-  load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
-  if (!single) load_parent(pop + (size_t)s_w[1] * n, n, p2, inv2);
-  __syncthreads();
-
-  if (single) {
-    for (int x = threadIdx.x; x < n; x += blockDim.x) cA[x] = p1[x];
-    __syncthreads();
-  } else {
-    int c1 = cut_lo[q], c2 = cut_hi[q];
-    if (c1 > c2) {
-      const int t = c1;
-      c1 = c2;
-      c2 = t;
-    }
-    c1 = c1 < 0 ? 0 : c1;
-    c2 = c2 >= n ? n - 1 : c2;
-    pmx_child(p1, p2, inv1, inv2, n, c1, c2, cA);
-    pmx_child(p2, p1, inv2, inv1, n, c1, c2, cB);
-    __syncthreads();
-  }
-
-  const int nrows = single ? 1 : 2;
-  for (int c = 0; c < nrows; ++c) {
-    const int r = r0 + c;
-    int32_t* ch = c == 0 ? cA : cB;
-    if (probs[r] < rate) {
-      int lo = pos1[r], hi = pos2[r];
-      if (lo > hi) {
-        const int t = lo;
-        lo = hi;
-        hi = t;
-      }
-      lo = lo < 0 ? 0 : lo;
-      hi = hi >= n ? n - 1 : hi;
-      reverse_range(ch, lo, hi);
-      __syncthreads();
-    }
-    const double cost = block_cost(ch, n, D, ld, red);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) offspring[(size_t)r * n + x] = ch[x];
-    if (threadIdx.x == 0) off_fit[r] = cost;
-  }
+  generation_pair(blockIdx.x, pop, fitness, P, n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring,
+                  off_fit, lds, red, s_w);
 }
 
 // Stable rank of M float64 keys; order[rank[i]] = i for rank[i] < keep.
@@ -244,44 +123,11 @@ __global__ __launch_bounds__(kRankBlock) void rank_select_kernel(const double* _
 // Stable order of up to 4096 float64 keys in one workgroup: bitonic sort of (orderable key bits,
 // index) pairs in LDS.  Indices are unique, so the network yields exactly the stable ascending
 // order that the rank kernel above computes, in O(M log^2 M) instead of O(M^2) work.
-constexpr int kSortBlock = 1024;
-constexpr int kSortMax = 4096;
-
-__device__ __forceinline__ uint64_t orderable_bits(double v) {
-  if (v == 0.0) v = 0.0;  // -0.0 and +0.0 compare equal, as in the rank kernel
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
 __global__ __launch_bounds__(kSortBlock) void sort_select_kernel(const double* __restrict__ keys, int M, int M2,
                                                                  int keep, int32_t* __restrict__ order) {
   __shared__ uint64_t s_key[kSortMax];
   __shared__ int32_t s_idx[kSortMax];
-  for (int x = threadIdx.x; x < M2; x += kSortBlock) {
-    s_key[x] = x < M ? orderable_bits(keys[x]) : ~0ull;
-    s_idx[x] = x;
-  }
-  __syncthreads();
-  for (int size = 2; size <= M2; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int p = threadIdx.x; p < (M2 >> 1); p += kSortBlock) {
-        const int i = 2 * stride * (p / stride) + (p % stride);
-        const int j = i + stride;
-        const bool up = (i & size) == 0;
-        const uint64_t ki = s_key[i], kj = s_key[j];
-        const int ii = s_idx[i], ij = s_idx[j];
-        const bool greater = (ki > kj) || (ki == kj && ii > ij);
-        if (greater == up) {
-          s_key[i] = kj;
-          s_key[j] = ki;
-          s_idx[i] = ij;
-          s_idx[j] = ii;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int x = threadIdx.x; x < keep; x += kSortBlock) order[x] = s_idx[x];
+  sort_select_body(keys, M, M2, keep, order, s_key, s_idx);
 }
 
 // One wave per destination row.
