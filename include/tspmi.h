@@ -239,6 +239,44 @@ int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* item_inst, co
 /* First minimum among the first `count` keys of every instance.  Replaces genetic_algorithm.py:708-739. */
 int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Learned candidate-edge scorer.  No counterpart exists in the reference (it extends the
+ * reference's k-nearest candidate lists of src/distances/matrix.py:140-211 and
+ * two_opt.py:117-149); the C ABI follows the same conventions.
+ * Parameters: one fp32 master buffer and one bf16 (uint16) buffer of tspmi_scorer_param_count()
+ * values: W1 [128][16] (features 12..15 zero), b1 [128], W2 [128][128], b2 [128], w3 [128], b3.
+ * ------------------------------------------------------------------------------------------- */
+int tspmi_scorer_param_count(void);
+
+/* Features X (n*32, 16) bf16 of (city u, its r-th nearest neighbour), from coordinates, the
+ * (n, 32) nearest-neighbour table and the distance matrix; n >= 33. */
+int tspmi_scorer_features(const double* xy, int n, const int32_t* nn32, const int32_t* D, int ld, uint16_t* X,
+                          void* stream);
+
+/* Labels y (n*32) float: 1 when (u, r-th neighbour) is an edge of `tour` (either direction);
+ * work holds n int32. */
+int tspmi_scorer_labels(const int32_t* tour, int n, const int32_t* nn32, int32_t* work, float* y, void* stream);
+
+/* Logits z (S) of the MLP for S feature rows (MFMA bf16, fp32 accumulation). */
+int tspmi_mlp_forward(const uint16_t* X, int S, const uint16_t* wbf, float* z, void* stream);
+
+/* Gradient of the mean BCE-with-logits over S rows (S % 8 == 0) into grad (param_count + 1
+ * floats; the last one is the loss).  work / partials sizes from tspmi_mlp_work_sizes. */
+int tspmi_mlp_grad(const uint16_t* X, const float* y, int S, const uint16_t* wbf, uint16_t* work, float* partials,
+                   float* grad, void* stream);
+int tspmi_mlp_work_sizes(int S, long* work_elems, long* partial_floats);
+
+/* One Adam step on the fp32 master weights; refreshes the bf16 copy. */
+int tspmi_adam(float* theta, float* m, float* v, const float* grad, int step, float lr, float beta1, float beta2,
+               float eps, float weight_decay, uint16_t* wbf, void* stream);
+
+/* fp32 -> bf16 (round to nearest even). */
+int tspmi_to_bf16(const float* src, int count, uint16_t* dst, void* stream);
+
+/* Learned candidate table out (n, k): the k highest-scoring of each city's 32 nearest neighbours
+ * (ties to the nearer rank), in decreasing score order; k <= 32. */
+int tspmi_scorer_topk(const float* z, const int32_t* nn32, int n, int k, int32_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -402,3 +402,14 @@ def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
   np.testing.assert_array_equal(got[rows], want)
   same, _ = c_two_opt.improve_batch(got, d, cand, "best", 1)
   np.testing.assert_array_equal(same, got)
+
+
+@pytest.mark.parametrize("n,iters,rows", [(2100, 60, 3), (3000, 40, 2), (10000, 6, 2)])
+def test_two_opt_wide_large_n_matches_c_oracle(torch_mod, nat, n, iters, rows):
+  say(f"two_opt wide n={n} iters={iters}")
+  d = compute_distance_matrix(uniform_coords(n, seed=n))
+  tours = oinit.random_init(np.arange(n), rows, oinit.draw_row_seeds(np.random.default_rng(n), rows))
+  got, stats = _run_two_opt(nat, torch_mod, tours, d, None, "best", iters)
+  want, wstats = c_two_opt.improve_batch(tours, d, None, "best", iters)
+  np.testing.assert_array_equal(got, want)
+  assert stats[1] == wstats[1]

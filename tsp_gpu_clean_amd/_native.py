@@ -53,6 +53,16 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
+  "tspmi_scorer_param_count": [],
+  "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
+  "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
+  "tspmi_mlp_forward": [_P, _I, _P, _P, _P],
+  "tspmi_mlp_grad": [_P, _P, _I, _P, _P, _P, _P, _P],
+  "tspmi_mlp_work_sizes": [_I, _P, _P],
+  "tspmi_adam": [_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                 _P, _P],
+  "tspmi_to_bf16": [_P, _I, _P, _P],
+  "tspmi_scorer_topk": [_P, _P, _I, _I, _P, _P],
 }
 
 EXPORTED = tuple(_SIGNATURES) + ("tspmi_last_error",)
@@ -344,3 +354,97 @@ def batch_call(name: str, *args) -> None:
     else:
       conv.append(a)
   _check(getattr(_lib, name)(*conv, _stream(torch)), name)
+
+
+# ---- learned candidate scorer -------------------------------------------------------------------
+
+
+def scorer_param_count() -> int:
+  return int(load_library().tspmi_scorer_param_count())
+
+
+def mlp_work_sizes(S: int) -> tuple[int, int]:
+  load_library()
+  w = ctypes.c_long()
+  p = ctypes.c_long()
+  _check(_lib.tspmi_mlp_work_sizes(int(S), ctypes.byref(w), ctypes.byref(p)), "tspmi_mlp_work_sizes")
+  return int(w.value), int(p.value)
+
+
+def scorer_features(xy, nn32, D, X) -> None:
+  torch = require_device()
+  _need(xy, torch.float64, "xy", 2)
+  _need(nn32, torch.int32, "nn32", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(X, torch.int16, "X", 2)
+  n = xy.shape[0]
+  if nn32.shape != (n, 32) or X.shape != (n * 32, 16):
+    raise ValueError("scorer_features: shape mismatch")
+  _check(_lib.tspmi_scorer_features(_ptr(xy), n, _ptr(nn32), _ptr(D), D.shape[1], _ptr(X), _stream(torch)),
+         "tspmi_scorer_features")
+
+
+def scorer_labels(tour, nn32, work, y) -> None:
+  torch = require_device()
+  _need(tour, torch.int32, "tour", 1)
+  _need(nn32, torch.int32, "nn32", 2)
+  _need(work, torch.int32, "work", 1)
+  _need(y, torch.float32, "y", 1)
+  n = tour.shape[0]
+  if nn32.shape != (n, 32) or y.shape[0] != n * 32 or work.shape[0] < n:
+    raise ValueError("scorer_labels: shape mismatch")
+  _check(_lib.tspmi_scorer_labels(_ptr(tour), n, _ptr(nn32), _ptr(work), _ptr(y), _stream(torch)), "tspmi_scorer_labels")
+
+
+def mlp_forward(X, wbf, z) -> None:
+  torch = require_device()
+  _need(X, torch.int16, "X", 2)
+  _need(wbf, torch.int16, "wbf", 1)
+  _need(z, torch.float32, "z", 1)
+  if X.shape[1] != 16 or z.shape[0] < X.shape[0]:
+    raise ValueError("mlp_forward: shape mismatch")
+  _check(_lib.tspmi_mlp_forward(_ptr(X), X.shape[0], _ptr(wbf), _ptr(z), _stream(torch)), "tspmi_mlp_forward")
+
+
+def mlp_grad(X, y, wbf, work, partials, grad) -> None:
+  torch = require_device()
+  _need(X, torch.int16, "X", 2)
+  _need(y, torch.float32, "y", 1)
+  _need(wbf, torch.int16, "wbf", 1)
+  _need(work, torch.int16, "work", 1)
+  _need(partials, torch.float32, "partials", 1)
+  _need(grad, torch.float32, "grad", 1)
+  S = X.shape[0]
+  w, p = mlp_work_sizes(S)
+  if work.numel() < w or partials.numel() < p or grad.numel() < scorer_param_count() + 1 or y.shape[0] < S:
+    raise ValueError("mlp_grad: buffers too small")
+  _check(_lib.tspmi_mlp_grad(_ptr(X), _ptr(y), S, _ptr(wbf), _ptr(work), _ptr(partials), _ptr(grad), _stream(torch)),
+         "tspmi_mlp_grad")
+
+
+def adam(theta, m, v, grad, step: int, lr: float, beta1: float, beta2: float, eps: float, weight_decay: float,
+         wbf) -> None:
+  torch = require_device()
+  for t, nm in ((theta, "theta"), (m, "m"), (v, "v"), (grad, "grad")):
+    _need(t, torch.float32, nm, 1)
+  _need(wbf, torch.int16, "wbf", 1)
+  _check(_lib.tspmi_adam(_ptr(theta), _ptr(m), _ptr(v), _ptr(grad), int(step), float(lr), float(beta1), float(beta2),
+                         float(eps), float(weight_decay), _ptr(wbf), _stream(torch)), "tspmi_adam")
+
+
+def to_bf16(src, dst) -> None:
+  torch = require_device()
+  _need(src, torch.float32, "src", 1)
+  _need(dst, torch.int16, "dst", 1)
+  _check(_lib.tspmi_to_bf16(_ptr(src), src.shape[0], _ptr(dst), _stream(torch)), "tspmi_to_bf16")
+
+
+def scorer_topk(z, nn32, k: int, out) -> None:
+  torch = require_device()
+  _need(z, torch.float32, "z", 1)
+  _need(nn32, torch.int32, "nn32", 2)
+  _need(out, torch.int32, "out", 2)
+  n = nn32.shape[0]
+  if out.shape != (n, k) or z.shape[0] < n * 32:
+    raise ValueError("scorer_topk: shape mismatch")
+  _check(_lib.tspmi_scorer_topk(_ptr(z), _ptr(nn32), n, int(k), _ptr(out), _stream(torch)), "tspmi_scorer_topk")

@@ -92,6 +92,100 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
   return best;
 }
 
+// Large-n variant of the diagonal scan: 16 waves per tour and clamped LDS indices instead of
+// padding (the padding would not fit next to a 10^4-city tour), with the explicit bound test.
+constexpr int kWideBlock = 1024;
+constexpr int kWideWaves = kWideBlock / kWave;
+
+template <int QMAX>
+__device__ __forceinline__ uint64_t scan_full_best_wide(const int32_t* s_tour, const int32_t* s_edge, int n,
+                                                        const int32_t* __restrict__ D, int ld, int wid, int lane) {
+  const int rows = n - 2;
+  const int jq0 = 2 + wid * kWave + lane;
+  int carry[QMAX], gcur[QMAX], gnext[QMAX], bd[QMAX], bi[QMAX];
+  {
+    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
+    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = jq0 + q * kWideBlock;
+      carry[q] = row_at(r0, s_tour[min(j, n)]);
+      gcur[q] = row_at(r1, s_tour[min(j + 1, n)]);
+      bd[q] = 0;
+      bi[q] = 0;
+    }
+  }
+  for (int i = 0; i < rows; ++i) {
+    if (i + 2 + kWave * wid >= n) break;
+    const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) gnext[q] = row_at(rn, s_tour[min(i + jq0 + 2 + q * kWideBlock, n)]);
+    const int eab = s_edge[i];
+    const int lim = (i == 0) ? n - 1 : n;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = i + jq0 + q * kWideBlock;
+      const int delta = carry[q] + gcur[q] - eab - s_edge[min(j, n - 1)];
+      const bool take = (j < lim) & (delta < bd[q]);
+      bd[q] = take ? delta : bd[q];
+      bi[q] = take ? i : bi[q];
+      carry[q] = gcur[q];
+      gcur[q] = gnext[q];
+    }
+  }
+  uint64_t best = kNoMove;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (bd[q] < 0) {
+      const uint32_t r = (uint32_t)bi[q];
+      const uint64_t key = pack_key(bd[q], r * (uint32_t)n + r + (uint32_t)(jq0 + q * kWideBlock));
+      best = key < best ? key : best;
+    }
+  }
+  return best;
+}
+
+template <int QMAX>
+__global__ __launch_bounds__(kWideBlock) void two_opt_wide_kernel(int32_t* __restrict__ tours, int n,
+                                                                  const int32_t* __restrict__ D, int ld, int max_iter,
+                                                                  int64_t* __restrict__ stats) {
+  extern __shared__ int32_t lds[];
+  __shared__ uint64_t red[kWideWaves];
+  int32_t* s_tour = lds;          // n + 1 (s_tour[n] mirrors s_tour[0])
+  int32_t* s_edge = lds + n + 1;  // n
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+  for (int x = tid; x < n; x += kWideBlock) {
+    const int v = clamp_id(tour[x], ld);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += kWideBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
+  __syncthreads();
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = scan_full_best_wide<QMAX>(s_tour, s_edge, n, D, ld, wid, lane);
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)n);
+    const int j = (int)(idx - (uint32_t)i * (uint32_t)n);
+    __syncthreads();
+    apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
+    ++moves;
+  }
+  for (int x = tid; x < n; x += kWideBlock) tour[x] = s_tour[x];
+  if (stats != nullptr && tid == 0) {
+    atomicAdd((unsigned long long*)&stats[0], (unsigned long long)((int64_t)passes * n * (n - 3) / 2));
+    atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+    atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+  }
+}
+
 template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restrict__ tours, int n,
                                                                const int32_t* __restrict__ D, int ld,
@@ -313,7 +407,14 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   const size_t lds = ((size_t)(use_cand ? 3 : 2) * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + sizeof(TwoOptShared) <= (size_t)kMaxLds,
                   "tspmi_two_opt: n=%d needs %zu B of LDS (max %d)", n, lds, kMaxLds);
-  if (q == 4) {
+  if (!use_cand && mode == TSPMI_MODE_BEST && q == 0 && n - 2 <= kWideWaves * 16 * kWave &&
+      (size_t)(2 * n + 1) * sizeof(int32_t) + 1024 <= (size_t)kMaxLds) {
+    const size_t wl = (size_t)(2 * n + 1) * sizeof(int32_t);
+    if (n - 2 <= kWideWaves * 10 * kWave)
+      hipLaunchKernelGGL(two_opt_wide_kernel<10>, dim3(P), dim3(kWideBlock), wl, s, tours, n, D, ld, max_iter, stats);
+    else
+      hipLaunchKernelGGL(two_opt_wide_kernel<16>, dim3(P), dim3(kWideBlock), wl, s, tours, n, D, ld, max_iter, stats);
+  } else if (q == 4) {
     hipLaunchKernelGGL(two_opt_kernel<4>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, nullptr, 0, mode,
                        max_iter, pad, stats);
   } else if (q == kQ) {
