@@ -175,6 +175,15 @@ int tspmi_random_init(const uint64_t* seeds, int P, const int32_t* customers, in
 int tspmi_nn_init(const uint64_t* seeds, int P, const int32_t* customers, int n, const int32_t* cand,
                   int k, int32_t* work, int32_t* out, void* stream);
 
+/*
+ * Greedy nearest-neighbour tours over cities 0..n-1: row r starts at starts[r] and always moves to
+ * the nearest unvisited city (lowest index on equal distances).  One workgroup per row.
+ * No counterpart in the reference (its nearest-neighbour initialiser is the randomised one
+ * above); used to build common start tours when comparing candidate-list schemes.
+ */
+int tspmi_nn_greedy_init(const int32_t* starts, int P, int n, const int32_t* D, int ld, int32_t* out,
+                         void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Uneven multi-instance batch (many instances of different n, each with its own population).
  *

@@ -130,3 +130,23 @@ def nn_init(customers, population_size: int, candidates: np.ndarray, row_seeds: 
   for r in range(population_size):
     pop[r] = nn_row(cust, candidates, int(row_seeds[r]))
   return pop
+
+
+def greedy_nn(D: np.ndarray, starts) -> np.ndarray:
+  """Greedy nearest-neighbour tours: from each start, always the nearest unvisited city, lowest
+  index on equal distances.  No reference counterpart (textbook nearest-neighbour heuristic);
+  pinned by hand-worked line instances in tests/test_oracle_operators.py."""
+  D = np.asarray(D, dtype=np.int64)
+  n = D.shape[0]
+  out = np.empty((len(starts), n), dtype=np.int32)
+  big = np.iinfo(np.int64).max
+  for r, s in enumerate(starts):
+    seen = np.zeros(n, dtype=bool)
+    cur = int(s)
+    out[r, 0] = cur
+    seen[cur] = True
+    for step in range(1, n):
+      cur = int(np.argmin(np.where(seen, big, D[cur])))
+      out[r, step] = cur
+      seen[cur] = True
+  return out

@@ -143,10 +143,11 @@ def adam(theta, m, v, g, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
     wmask[OFF_W2:OFF_B2] = True
     wmask[OFF_W3:OFF_B3] = True
     g = np.where(wmask, g + np.float32(wd) * th, g).astype(np.float32)
-  m = (np.float32(b1) * m + np.float32(1 - b1) * g).astype(np.float32)
-  v = (np.float32(b2) * v + np.float32(1 - b2) * g * g).astype(np.float32)
-  mh = m / np.float32(1 - b1**step)
-  vh = v / np.float32(1 - b2**step)
+  one, b1f, b2f = np.float32(1), np.float32(b1), np.float32(b2)
+  m = (b1f * m + (one - b1f) * g).astype(np.float32)
+  v = (b2f * v + (one - b2f) * g * g).astype(np.float32)
+  mh = m / (one - b1f ** np.float32(step))
+  vh = v / (one - b2f ** np.float32(step))
   th = (th - np.float32(lr) * mh / (np.sqrt(vh) + np.float32(eps))).astype(np.float32)
   th[pad] = 0.0
   return th, m, v

@@ -350,6 +350,16 @@ def test_nn_init_matches_oracle(torch_mod, nat, berlin52_coords, k):
   np.testing.assert_array_equal(out.cpu().numpy(), oinit.nn_init(np.arange(n), P, cand, seeds))
 
 
+@pytest.mark.parametrize("n,lattice", [(300, False), (400, True), (1, False), (2, False)])
+def test_nn_greedy_init_matches_oracle(torch_mod, nat, n, lattice):
+  say(f"nn_greedy_init n={n} lattice={lattice}")
+  d = lattice_matrix(n) if lattice else compute_distance_matrix(uniform_coords(n, seed=n))
+  starts = np.random.default_rng(n).integers(0, n, 6).astype(np.int32)
+  out = torch_mod.empty((6, n), dtype=torch_mod.int32, device="cuda")
+  nat.nn_greedy_init(cu(starts), cu(d), out)
+  np.testing.assert_array_equal(out.cpu().numpy(), oinit.greedy_nn(d, starts))
+
+
 # ---- packed candidate 2-opt ------------------------------------------------------------------------
 
 

@@ -3,7 +3,8 @@
 Bars: bf16 conversion, features (as bf16 bits), labels and top-k bit-exact; the MLP forward
 within 1e-3 * (1 + |z|) per sample (fp32 accumulation against float64 at the same bf16 rounding
 points; a rare one-ulp bf16 flip of an activation is absorbed by that tolerance); the gradient
-within relative L2 error 1e-2 per parameter block; Adam within 1e-6 relative (float32 both).
+within relative L2 error 1e-2 per parameter block; Adam within 1e-5 relative (float32 both sides,
+the device fusing multiply-adds).
 """
 
 from __future__ import annotations
@@ -132,10 +133,13 @@ def test_adam_matches_oracle(env):
   wbf = torch.empty(mlp.NPARAM, dtype=torch.int16, device="cuda")
   nat.adam(dth, dm, dv, grad, 3, 1e-3, 0.9, 0.999, 1e-8, 1e-4, wbf)
   th2, m2, v2 = mlp.adam(th, m, v, g, 3, lr=1e-3, wd=1e-4)
+  # float32 both sides; the device may fuse a multiply-add (one rounding fewer)
   assert np.allclose(dth.cpu().numpy(), th2, rtol=1e-6, atol=1e-7)
-  assert np.allclose(dm.cpu().numpy(), m2, rtol=1e-6, atol=1e-9)
-  assert np.allclose(dv.cpu().numpy(), v2, rtol=1e-6, atol=1e-12)
+  assert np.allclose(dm.cpu().numpy(), m2, rtol=1e-5, atol=1e-8)
+  assert np.allclose(dv.cpu().numpy(), v2, rtol=1e-5, atol=1e-11)
   assert np.array_equal(wbf.cpu().numpy().view(np.uint16), mlp.bf16_bits(dth.cpu().numpy()))
+  pad = (np.arange(mlp.H * mlp.F) % mlp.F) >= mlp.NFEAT
+  assert (dth.cpu().numpy()[mlp.OFF_W1 : mlp.OFF_B1][pad] == 0).all()
 
 
 def _instance(n: int, seed: int, clustered: bool):
@@ -215,3 +219,24 @@ def test_training_reduces_loss(env):
   base = -(prior * np.log(prior) + (1 - prior) * np.log(1 - prior))
   assert losses[-1] < losses[0]
   assert losses[-1] < base
+
+
+def test_learned_candidates_gain_over_knn_at_equal_width(env):
+  """What the trained network gains: with the same 8 candidates per city, 2-opt on the learned
+  lists reaches better local optima than on the 8 nearest neighbours (mean over 32 start tours,
+  same starts), measured against full-neighbourhood 2-opt from those starts."""
+  torch, nat = env
+  from tsp_gpu_clean_amd.scorer import LearnedCandidateScorer, build_training_set, evaluate_gap, training_instances
+
+  inst = training_instances(count=24, seed=21, lo=300, hi=1200)
+  ts = build_training_set(inst, population=8, generations=2, chunk=24)
+  sc = LearnedCandidateScorer(seed=1)
+  losses = sc.train(ts.X, ts.y, steps=400, batch=16384, lr=3e-3, log_every=100)
+  say(f"gain test: {ts.X.shape[0]} samples, losses {losses}")
+  res = evaluate_gap(n=2000, P=32, scorer=sc, k_learned=8, seed=77, full_chunk=100, cand_chunk=50)
+  say(f"gain test result: {res}")
+  full = res["full"]["mean"]
+  learned = res["methods"]["learned8"]["mean"]
+  knn8 = res["methods"]["knn8"]["mean"]
+  assert learned < knn8, (learned, knn8, full)
+  assert res["methods"]["learned8"]["best"] <= 1.05 * res["full"]["best"]

@@ -112,3 +112,16 @@ def test_nn_init_rows_are_permutations_and_follow_neighbours(berlin52_coords):
   # most steps go to one of the 5 nearest neighbours
   hits = sum(int(pop[r, s + 1] in cand[pop[r, s]]) for r in range(8) for s in range(51))
   assert hits > 0.7 * 8 * 51
+
+
+def test_greedy_nn_hand_worked():
+  from oracle.init import greedy_nn
+
+  x = np.array([0, 1, 3, 6])
+  D = np.abs(x[:, None] - x[None, :]).astype(np.int32)
+  got = greedy_nn(D, [0, 2, 3])
+  assert got.tolist() == [[0, 1, 2, 3], [2, 1, 0, 3], [3, 2, 1, 0]]
+  # equal distances: the lower index wins
+  x = np.array([-1, 0, 1])
+  D = np.abs(x[:, None] - x[None, :]).astype(np.int32)
+  assert greedy_nn(D, [1]).tolist() == [[1, 0, 2]]

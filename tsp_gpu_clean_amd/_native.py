@@ -53,6 +53,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
+  "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_scorer_param_count": [],
   "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
   "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
@@ -337,6 +338,18 @@ def nn_init(seeds, customers, cand, work, out) -> None:
     raise ValueError("nn_init: work needs 2 * P * n int32")
   _check(_lib.tspmi_nn_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(cand), k, _ptr(work),
                             _ptr(out), _stream(torch)), "tspmi_nn_init")
+
+
+def nn_greedy_init(starts, D, out) -> None:
+  torch = require_device()
+  _need(starts, torch.int32, "starts", 1)
+  _need(D, torch.int32, "D", 2)
+  _need(out, torch.int32, "out", 2)
+  P, n = out.shape
+  if starts.shape[0] != P or D.shape[0] < n:
+    raise ValueError("nn_greedy_init: shape mismatch")
+  _check(_lib.tspmi_nn_greedy_init(_ptr(starts), P, n, _ptr(D), D.shape[1], _ptr(out), _stream(torch)),
+         "tspmi_nn_greedy_init")
 
 
 # ---- uneven multi-instance batch (raw pointers; shapes are validated by the batch engine) -------

@@ -104,6 +104,46 @@ __global__ __launch_bounds__(kInitBlock) void nn_init_kernel(const uint64_t* __r
   }
 }
 
+
+// Greedy nearest-neighbour tours (always the nearest unvisited city, lowest index on ties), one
+// workgroup per tour: each step is a workgroup argmin of (D[current][j], j) over the unvisited j,
+// with the visited set as an LDS bitmap of ceil(n / 32) words.
+constexpr int kGreedyBlock = 256;
+
+__global__ __launch_bounds__(kGreedyBlock) void nn_greedy_kernel(const int32_t* __restrict__ starts, int n,
+                                                                 const int32_t* __restrict__ D, int ld,
+                                                                 int32_t* __restrict__ out) {
+  extern __shared__ uint32_t visited[];
+  __shared__ uint64_t red[kGreedyBlock / kWave];
+  const int words = (n + 31) >> 5;
+  for (int w = threadIdx.x; w < words; w += kGreedyBlock) visited[w] = 0u;
+  __syncthreads();
+  int32_t* row = out + (size_t)blockIdx.x * n;
+  int cur = clamp_id(starts[blockIdx.x], n);
+  if (threadIdx.x == 0) {
+    row[0] = cur;
+    visited[cur >> 5] |= 1u << (cur & 31);
+  }
+  __syncthreads();
+  for (int step = 1; step < n; ++step) {
+    const int32_t* drow = D + (size_t)cur * ld;
+    uint64_t best = ~0ull;
+    for (int j = threadIdx.x; j < n; j += kGreedyBlock) {
+      if (!((visited[j >> 5] >> (j & 31)) & 1u)) {
+        const uint64_t key = ((uint64_t)(uint32_t)drow[j] << 32) | (uint32_t)j;
+        best = key < best ? key : best;
+      }
+    }
+    best = block_min_u64(best, red);
+    cur = (int)(uint32_t)best;
+    if (threadIdx.x == 0) {
+      row[step] = cur;
+      visited[cur >> 5] |= 1u << (cur & 31);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace tspmi
 
 using namespace tspmi;
@@ -127,5 +167,17 @@ extern "C" int tspmi_nn_init(const uint64_t* seeds, int P, const int32_t* custom
   hipLaunchKernelGGL(nn_init_kernel, dim3((P + kInitBlock - 1) / kInitBlock), dim3(kInitBlock), 0,
                      (hipStream_t)stream, seeds, P, customers, n, cand, k, work, out);
   TSPMI_CHECK_LAUNCH("tspmi_nn_init launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_nn_greedy_init(const int32_t* starts, int P, int n, const int32_t* D, int ld, int32_t* out,
+                                    void* stream) {
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_nn_greedy_init: negative shape");
+  if (P == 0 || n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(starts && D && out && ld >= n, "tspmi_nn_greedy_init: null pointer or ld < n");
+  const size_t lds = (size_t)((n + 31) / 32) * sizeof(uint32_t);
+  TSPMI_CHECK_ARG(lds + 64 <= (size_t)kMaxLds, "tspmi_nn_greedy_init: n=%d exceeds the LDS visited bitmap", n);
+  hipLaunchKernelGGL(nn_greedy_kernel, dim3(P), dim3(kGreedyBlock), lds, (hipStream_t)stream, starts, n, D, ld, out);
+  TSPMI_CHECK_LAUNCH("tspmi_nn_greedy_init launch");
   return TSPMI_OK;
 }

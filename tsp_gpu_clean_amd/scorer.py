@@ -15,6 +15,7 @@ The reference has no learned component; its candidate lists are k-nearest neighb
 
 from __future__ import annotations
 
+import os
 import time
 from dataclasses import dataclass
 
@@ -30,6 +31,7 @@ OFF_W3 = OFF_B2 + H
 OFF_B3 = OFF_W3 + H
 NPARAM = OFF_B3 + 1
 NEIGHBOURS = 32
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "scorer_weights.npz")
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -82,8 +84,11 @@ class LearnedCandidateScorer:
 
   # ---- training ---------------------------------------------------------------------------------
   def train(self, X, y, steps: int = 300, batch: int = 65536, lr: float = 2e-3, seed: int = 0,
-            weight_decay: float = 0.0, log_every: int = 50) -> list[float]:
-    """Minibatch Adam on (X, y); minibatch rows drawn on the host with a seeded generator."""
+            weight_decay: float = 0.0, log_every: int = 50, lr_final: float | None = None) -> list[float]:
+    """Minibatch Adam on (X, y); minibatch rows drawn on the host with a seeded generator.
+
+    The learning rate follows a cosine from ``lr`` to ``lr_final`` (constant when None).
+    """
     torch = _native.require_device()
     S = int(X.shape[0])
     batch = min(batch, S - S % 8)
@@ -103,7 +108,8 @@ class LearnedCandidateScorer:
       _native.gather_rows(y32, None, None, None, idx, yb.view(torch.int32).view(batch, 1), None)
       _native.mlp_grad(Xb, yb, self.wbf, work, partials, grad)
       self.step += 1
-      _native.adam(self.theta, self.m, self.v, grad, self.step, lr, 0.9, 0.999, 1e-8, weight_decay, self.wbf)
+      lr_t = lr if lr_final is None else lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * it / max(1, steps - 1)))
+      _native.adam(self.theta, self.m, self.v, grad, self.step, lr_t, 0.9, 0.999, 1e-8, weight_decay, self.wbf)
       if log_every and (it % log_every == 0 or it == steps - 1):
         losses.append(float(grad[NPARAM].item()))
     self.history.extend(losses)
@@ -112,14 +118,20 @@ class LearnedCandidateScorer:
   def state(self) -> np.ndarray:
     return self.theta.cpu().numpy()
 
-  def save(self, path: str) -> None:
-    np.savez(path, theta=self.state(), step=self.step)
+  def save(self, path: str, info: dict | None = None) -> None:
+    import json
+
+    np.savez(path, theta=self.state(), step=self.step, info=np.array(json.dumps(info or {})))
 
   @classmethod
-  def load(cls, path: str) -> "LearnedCandidateScorer":
+  def load(cls, path: str = DEFAULT_WEIGHTS) -> "LearnedCandidateScorer":
+    """Weights saved by ``save`` (default: the shipped scorer trained on 2000 generated instances)."""
+    import json
+
     data = np.load(path, allow_pickle=False)
     s = cls(theta=data["theta"])
     s.step = int(data["step"])
+    s.info = json.loads(str(data["info"])) if "info" in data.files else {}
     return s
 
 
@@ -191,3 +203,149 @@ def build_training_set(instances, population: int = 8, generations: int = 3, chu
     del ga
   positives = int(y.sum().item())
   return TrainingSet(X, y, len(instances), positives, time.perf_counter() - t0)
+
+
+def train_on_generated(count: int = 2000, steps: int = 3000, batch: int = 65536, lr: float = 2e-3, seed: int = 0,
+                       population: int = 8, generations: int = 3, chunk: int = 64, progress=None):
+  """Generate ``count`` instances, label them with GA + 2-opt tours and train a scorer on them.
+
+  Returns (scorer, info) where info records the data-generation and training times and losses.
+  """
+  torch = _native.require_device()
+  inst = training_instances(count=count, seed=11 + seed)
+  ts = build_training_set(inst, population=population, generations=generations, chunk=chunk, seed=5 + seed,
+                          progress=progress)
+  sc = LearnedCandidateScorer(seed=seed)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  losses = sc.train(ts.X, ts.y, steps=steps, batch=batch, lr=lr, seed=seed, log_every=max(1, steps // 20),
+                    lr_final=lr / 20)
+  torch.cuda.synchronize()
+  info = {"instances": count, "samples": int(ts.X.shape[0]), "positives": ts.positives,
+          "data_seconds": round(ts.seconds, 2), "train_seconds": round(time.perf_counter() - t0, 2),
+          "steps": steps, "batch": batch, "lr": lr, "losses": [round(x, 5) for x in losses]}
+  return sc, info
+
+
+# ---- evaluation: time to a 2 % gap against full-neighbourhood 2-opt -----------------------------
+
+
+def uniform_coords(n: int, seed: int) -> np.ndarray:
+  rng = np.random.default_rng(seed)
+  return rng.integers(0, 1_000_001, (n, 2)).astype(np.float64)
+
+
+def greedy_start(D, P: int, seed: int):
+  """P greedy nearest-neighbour tours from distinct random start cities (P <= n)."""
+  torch = _native.require_device()
+  n = int(D.shape[0])
+  starts = np.random.default_rng(seed).choice(n, size=P, replace=P > n).astype(np.int32)
+  out = torch.empty((P, n), dtype=torch.int32, device="cuda")
+  _native.nn_greedy_init(torch.from_numpy(starts).cuda(), D, out)
+  return out
+
+
+def ls_trace(tours0, D, table=None, k: int = 0, chunk: int = 50, limit_s: float = 600.0, on_chunk=None):
+  """2-opt (best improvement) to the local optimum in chunks of ``chunk`` iterations.
+
+  ``table`` None runs the full neighbourhood.  Returns (trace, tours) with trace a list of
+  (seconds, best cost, mean cost) after each chunk, starting at (0, start costs).
+  """
+  torch = _native.require_device()
+  tours = tours0.clone()
+  P = int(tours.shape[0])
+  cost = torch.empty(P, dtype=torch.float64, device="cuda")
+  stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  _native.tour_cost(tours, D, cost)
+  trace = [(0.0, float(cost.min().item()), float(cost.mean().item()))]
+  torch.cuda.synchronize()
+  t = 0.0
+  while t < limit_s:
+    moves0 = int(stats[1].item())
+    t0 = time.perf_counter()
+    if table is None:
+      _native.two_opt(tours, D, None, 0, chunk, stats)
+    else:
+      _native.two_opt_packed(tours, D, table, k, chunk, stats)
+    _native.tour_cost(tours, D, cost)
+    best, mean, moves = cost.min().item(), cost.mean().item(), int(stats[1].item())
+    t += time.perf_counter() - t0
+    trace.append((t, float(best), float(mean)))
+    if on_chunk is not None:
+      on_chunk(trace[-1], moves)
+    if moves == moves0:
+      break
+  return trace, tours
+
+
+def time_to(trace, target: float, offset: float = 0.0):
+  for t, best, _ in trace:
+    if best <= target:
+      return t + offset
+  return None
+
+
+def candidate_tables(D, xy, scorer, k_learned: int = 8, knn_ks=(8, 20)):
+  """Device candidate tables with their build times: {'knn{k}': (table, k, s), 'learned{k}': ...}."""
+  torch = _native.require_device()
+  n = int(D.shape[0])
+  out = {}
+  for k in knn_ks:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cand = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    _native.knn(D, k, cand)
+    tab = torch.empty((n, _native.candidate_lanes(k), 2), dtype=torch.int32, device="cuda")
+    _native.candidate_table(cand, D, tab)
+    torch.cuda.synchronize()
+    out[f"knn{k}"] = (tab, k, time.perf_counter() - t0)
+  if scorer is not None:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nn = torch.empty((n, NEIGHBOURS), dtype=torch.int32, device="cuda")
+    _native.knn(D, NEIGHBOURS, nn)
+    X = torch.empty((n * NEIGHBOURS, F), dtype=torch.int16, device="cuda")
+    _native.scorer_features(xy, nn, D, X)
+    z = scorer.scores(X)
+    cand = torch.empty((n, k_learned), dtype=torch.int32, device="cuda")
+    _native.scorer_topk(z, nn, k_learned, cand)
+    tab = torch.empty((n, _native.candidate_lanes(k_learned), 2), dtype=torch.int32, device="cuda")
+    _native.candidate_table(cand, D, tab)
+    torch.cuda.synchronize()
+    out[f"learned{k_learned}"] = (tab, k_learned, time.perf_counter() - t0)
+  return out
+
+
+def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, gap: float = 0.02, seed: int = 2024,
+                 full_chunk: int = 200, cand_chunk: int = 50, full_limit_s: float = 600.0, progress=None) -> dict:
+  """Time for each candidate scheme to bring the population's best tour within ``gap`` of the
+  full-neighbourhood 2-opt local optimum reached from the same start tours."""
+  torch = _native.require_device()
+  xy = torch.from_numpy(uniform_coords(n, seed)).cuda()
+  D = torch.empty((n, n), dtype=torch.int32, device="cuda")
+  _native.dist_euc2d(xy, D)
+  start = greedy_start(D, P, seed + 1)
+  tables = candidate_tables(D, xy, scorer, k_learned)
+  res = {"n": n, "P": P, "gap": gap, "methods": {}}
+  tick = None
+  if progress:
+    tick = lambda p, m: progress("full-chunk", {"seconds": round(p[0], 2), "best": p[1], "moves": m})  # noqa: E731
+  full_trace, _ = ls_trace(start, D, None, 0, full_chunk, full_limit_s, tick)
+  c_full = full_trace[-1][1]
+  target = (1.0 + gap) * c_full
+  res["full"] = {"best": c_full, "mean": full_trace[-1][2], "seconds_to_local_optimum": round(full_trace[-1][0], 4),
+                 "seconds_to_gap": time_to(full_trace, target), "start_best": full_trace[0][1],
+                 "chunks": len(full_trace) - 1}
+  if progress:
+    progress("full", res["full"])
+  for name, (tab, k, prep) in tables.items():
+    trace, _ = ls_trace(start, D, tab, k, cand_chunk)
+    best = trace[-1][1]
+    t_gap = time_to(trace, target, prep)
+    res["methods"][name] = {"k": k, "prep_seconds": round(prep, 5), "best": best, "mean": trace[-1][2],
+                            "gap_at_optimum": round(best / c_full - 1.0, 5),
+                            "seconds_to_gap": None if t_gap is None else round(t_gap, 5),
+                            "seconds_to_local_optimum": round(trace[-1][0] + prep, 5)}
+    if progress:
+      progress(name, res["methods"][name])
+  return res
