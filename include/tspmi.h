@@ -31,6 +31,7 @@ extern "C" {
 
 #define TSPMI_MODE_BEST 0
 #define TSPMI_MODE_FIRST 1
+#define TSPMI_MODE_MULTI 2
 
 /* Library identification and device probe. */
 int tspmi_version(void);
@@ -65,8 +66,11 @@ int tspmi_tour_cost(const int32_t* tours, int P, int n, const int32_t* D, int ld
 
 /*
  * Population-batched 2-opt on cyclic TSP tours, refined in place.
- *   mode: TSPMI_MODE_BEST (steepest, ties -> lowest (i,j) / (i,r) scan index) or
- *         TSPMI_MODE_FIRST (first improving pair in scan order).
+ *   mode: TSPMI_MODE_BEST (steepest, ties -> lowest (i,j) / (i,r) scan index),
+ *         TSPMI_MODE_FIRST (first improving pair in scan order) or
+ *         TSPMI_MODE_MULTI (every improving pair of the pass, steepest first with ties in scan
+ *         order, applied unless its window [i, j] meets an already accepted one; two_opt.py:491-563;
+ *         needs about 30 n bytes of LDS, so n <= ~5000).
  *   cand: (n, k) candidate table or NULL/k=0 for the full neighbourhood.
  *   max_iter: passes per tour; < 0 runs each tour to its local optimum.
  *   stats: device int64[3] accumulated with {deltas evaluated, moves applied, passes}

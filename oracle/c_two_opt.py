@@ -53,7 +53,7 @@ def improve_batch(routes, dist, candidates=None, mode="best", max_iterations=10)
     cand, k, cptr = None, 0, None
   stats = np.zeros(3, dtype=np.int64)
   rc = lib.oracle_two_opt(tours2.ctypes.data, tours2.shape[0], tours2.shape[1], d.ctypes.data,
-                          d.shape[1], cptr, k, {"best": 0, "first": 1}[mode], int(max_iterations),
+                          d.shape[1], cptr, k, {"best": 0, "first": 1, "multi_move": 2}[mode], int(max_iterations),
                           stats.ctypes.data)
   if rc != 0:
     raise ValueError("oracle_two_opt rejected its arguments")

@@ -145,11 +145,12 @@ def test_tour_cost_known_answers_and_random(torch_mod, nat):
 def _run_two_opt(nat, torch, tours, d, cand, mode, iters):
   t = cu(tours.astype(np.int32))
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
-  nat.two_opt(t, cu(d), None if cand is None else cu(cand), {"best": 0, "first": 1}[mode], iters, stats)
+  nat.two_opt(t, cu(d), None if cand is None else cu(cand), {"best": 0, "first": 1, "multi_move": 2}[mode], iters,
+              stats)
   return t.cpu().numpy(), stats.cpu().numpy()
 
 
-@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("mode", ["best", "first", "multi_move"])
 @pytest.mark.parametrize("k", [None, 3, 20])
 @pytest.mark.parametrize("n", [4, 5, 6, 8, 10, 17, 40])
 def test_two_opt_small_matches_numpy_oracle(torch_mod, nat, mode, k, n):
@@ -163,7 +164,7 @@ def test_two_opt_small_matches_numpy_oracle(torch_mod, nat, mode, k, n):
     np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, mode, iters))
 
 
-@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("mode", ["best", "first", "multi_move"])
 @pytest.mark.parametrize("k", [None, 3])
 @pytest.mark.parametrize("n", [9, 16, 25, 64])
 def test_two_opt_lattice_ties_match_numpy_oracle(torch_mod, nat, mode, k, n):
@@ -185,13 +186,13 @@ def test_two_opt_duplicate_candidates(torch_mod, nat):
   cand = nearest(d, 6)
   cand[:, 3] = cand[:, 1]  # repeated neighbours: pairs appear twice, first occurrence wins ties
   tours = np.stack([np.random.default_rng(t).permutation(n) for t in range(8)]).astype(np.int32)
-  for mode in ("best", "first"):
+  for mode in ("best", "first", "multi_move"):
     got, _ = _run_two_opt(nat, torch_mod, tours, d, cand, mode, 20)
     np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, mode, 20))
 
 
 @pytest.mark.parametrize("k", [None, 20])
-@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("mode", ["best", "first", "multi_move"])
 def test_two_opt_n1000_to_local_optimum_matches_c_oracle(torch_mod, nat, k, mode):
   say(f"two_opt n=1000 to optimum k={k} mode={mode}")
   xy = uniform_coords(1000)
@@ -203,6 +204,8 @@ def test_two_opt_n1000_to_local_optimum_matches_c_oracle(torch_mod, nat, k, mode
   want, wstats = c_two_opt.improve_batch(tours, d, cand, mode, -1)
   np.testing.assert_array_equal(got, want)
   assert stats[1] == wstats[1]  # same number of moves
+  if mode != "first":
+    assert list(stats) == list(wstats)
   np.testing.assert_array_equal(ops.tour_costs(got, d), ops.tour_costs(want, d))
 
 

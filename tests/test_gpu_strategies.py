@@ -70,7 +70,7 @@ def test_residency_shape_and_no_aliasing(t):
 
 
 @pytest.mark.parametrize("k", [None, 20])
-@pytest.mark.parametrize("mode", ["batch", "first"])
+@pytest.mark.parametrize("mode", ["batch", "first", "multi_move"])
 def test_local_search_strategy_matches_oracle(t, k, mode):
   say(f"LocalSearchGPUTourTwoOpt k={k} mode={mode}")
   ctx = t.ProblemContext(c1_problem(t, 300, seed=7, hi=1_000_000), seed=3)
@@ -78,10 +78,10 @@ def test_local_search_strategy_matches_oracle(t, k, mode):
   ls = t.LocalSearchGPUTourTwoOpt(max_iterations=10, mode=mode, candidate_k=k)
   got = ls.improve_batch(routes, ctx)
   cand = None if k is None else ctx.get_cpu_candidate_list(k)
-  want = o2.improve_batch(routes, ctx.get_cpu_distances(), cand, "best" if mode == "batch" else "first", 10)
+  want = o2.improve_batch(routes, ctx.get_cpu_distances(), cand, "best" if mode == "batch" else mode, 10)
   np.testing.assert_array_equal(got, want)
   deltas, moves, passes = ls.read_stats()
-  assert moves > 0 and passes >= moves
+  assert moves > 0 and (passes >= moves or mode == "multi_move")
 
 
 def test_operator_strategies_match_oracle_with_same_generator(t):

@@ -158,7 +158,7 @@ def test_moves_change_cost_by_delta_and_converge_to_local_optimum(k):
     assert all(_spec_delta(t, dist, i, j) >= 0 for i, j in _spec_pairs(n, t, dist, None))
 
 
-@pytest.mark.parametrize("mode", ["best", "first"])
+@pytest.mark.parametrize("mode", ["best", "first", "multi_move"])
 @pytest.mark.parametrize("k", [None, 4, 20])
 def test_c_oracle_matches_numpy_oracle(mode, k):
   n = 70
@@ -167,9 +167,13 @@ def test_c_oracle_matches_numpy_oracle(mode, k):
   rng = np.random.default_rng(5)
   routes = np.stack([rng.permutation(n) for _ in range(6)]).astype(np.int32)
   for iters in (1, 7, 1000):
-    want = o2.improve_batch(routes, dist, cand, mode, iters)
+    counter = o2.new_counter()
+    want = o2.improve_batch(routes, dist, cand, mode, iters, counter)
     got, stats = c_two_opt.improve_batch(routes, dist, cand, mode, iters)
     np.testing.assert_array_equal(got, want)
+    assert [stats[1], stats[2]] == [counter.moves, counter.passes]
+    if mode != "first":  # first-improvement scans stop at different points (numpy evaluates whole blocks)
+      assert stats[0] == counter.deltas
   got, stats = c_two_opt.improve_batch(routes, dist, cand, mode, -1)
   assert stats[1] > 0 and stats[2] >= 6
 
@@ -179,7 +183,7 @@ def test_c_oracle_lattice_ties():
   dist = lattice_matrix(n)
   rng = np.random.default_rng(9)
   routes = np.stack([rng.permutation(n) for _ in range(5)]).astype(np.int32)
-  for mode in ("best", "first"):
+  for mode in ("best", "first", "multi_move"):
     want = o2.improve_batch(routes, dist, None, mode, 50)
     got, _ = c_two_opt.improve_batch(routes, dist, None, mode, 50)
     np.testing.assert_array_equal(got, want)

@@ -381,13 +381,186 @@ __global__ void cand_table_kernel(const int32_t* __restrict__ cand, int n, int k
   out[idx] = v;
 }
 
+
+// Multi-move acceptance (two_opt.py:491-563): of all improving pairs of one scan, greedily accept
+// windows [i, j] in (delta, scan index) order unless they intersect an accepted window, then apply
+// every accepted reversal.  The greedy is computed without materialising the improving pairs:
+// the first accepted window is the scan's overall best pair; accepting [a, b] inside a free run
+// [L, R] leaves the runs [L, a-1] and [b+1, R], whose greedy choices are independent of each
+// other (no window can span both), so each round takes the best pair inside every live run, in
+// parallel, until no run holds an improving pair.  Deltas stay those of the step's start tour
+// (reversals are applied after the last round), as in the reference.
+//
+// LDS per position p: run[p] = start L of p's free run (-1 once covered or dead), rend[L] = end R
+// of the run starting at L, key[L] = best (delta, index) inside that run; moves[] = accepted
+// windows (at most n / 3: each covers >= 3 positions).
+constexpr int kMultiBlock = 256;
+constexpr int kMultiWaves = kMultiBlock / kWave;
+
+__global__ __launch_bounds__(kMultiBlock) void two_opt_multi_kernel(int32_t* __restrict__ tours, int n,
+                                                                    const int32_t* __restrict__ D, int ld,
+                                                                    const int32_t* __restrict__ cand, int k,
+                                                                    int max_iter, int64_t* __restrict__ stats) {
+  extern __shared__ int4 lds_multi[];
+  __shared__ int s_count, s_total;
+  __shared__ int64_t red64[kMultiWaves];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  int4* s_moves = lds_multi;                                             // n / 3 + 1 windows (a, b, L, R)
+  uint64_t* s_key = reinterpret_cast<uint64_t*>(s_moves + n / 3 + 1);  // n
+  int32_t* s_tour = reinterpret_cast<int32_t*>(s_key + n);              // n + 1
+  int32_t* s_edge = s_tour + n + 1;                                     // n
+  int32_t* s_run = s_edge + n;                                          // n
+  int32_t* s_rend = s_run + n;                                          // n (also the reversal copy)
+  int32_t* s_pos = s_rend + n;                                          // n (candidate mode)
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+  for (int x = tid; x < n; x += kMultiBlock) {
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  __syncthreads();
+  const int rows = n - 2;
+  int64_t my_deltas = 0;
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    for (int x = tid; x < n; x += kMultiBlock) {
+      const int a = s_tour[x];
+      s_edge[x] = D[(size_t)a * ld + s_tour[x + 1]];
+      if (cand != nullptr) s_pos[a] = x;
+      s_run[x] = 0;
+    }
+    if (tid == 0) {
+      s_rend[0] = n - 1;
+      s_total = 0;
+    }
+    __syncthreads();
+    for (int round = 0;; ++round) {
+      for (int x = tid; x < n; x += kMultiBlock)
+        if (s_run[x] == x) s_key[x] = kNoMove;
+      if (tid == 0) s_count = 0;
+      __syncthreads();
+      // best improving pair inside each live run
+      for (int i = wid; i < rows; i += kMultiWaves) {
+        const int L = s_run[i];
+        if (L < 0) continue;
+        const int R = s_rend[L];
+        const int lim = (i == 0) ? n - 1 : n;
+        const int jmax = min(lim - 1, R);
+        if (jmax < i + 2) continue;
+        const int a = s_tour[i], b = s_tour[i + 1];
+        const int eab = s_edge[i];
+        const int32_t* __restrict__ ra = D + (size_t)a * ld;
+        const int32_t* __restrict__ rb = D + (size_t)b * ld;
+        uint64_t best = kNoMove;
+        if (cand == nullptr) {
+          const uint32_t base = (uint32_t)i * (uint32_t)n;
+          for (int j = i + 2 + lane; j <= jmax; j += kWave) {
+            const int delta = ra[s_tour[j]] + rb[s_tour[j + 1]] - eab - s_edge[j];
+            if (delta < 0) {
+              const uint64_t key = pack_key(delta, base + (uint32_t)j);
+              best = key < best ? key : best;
+            }
+          }
+          if (round == 0 && lane == 0) my_deltas += jmax - i - 1;
+        } else {
+          for (int r = lane; r < k; r += kWave) {
+            const int nb = cand[(size_t)a * k + r];
+            const int j = (nb >= 0 && nb < n) ? s_pos[nb] : -1;
+            if (j > i + 1 && j <= jmax) {
+              if (round == 0) ++my_deltas;
+              const int delta = ra[nb] + rb[s_tour[j + 1]] - eab - s_edge[j];
+              if (delta < 0) {
+                const uint64_t key = pack_key(delta, (uint32_t)i * (uint32_t)k + (uint32_t)r);
+                best = key < best ? key : best;
+              }
+            }
+          }
+        }
+        best = wave_min_u64(best);
+        if (lane == 0 && best != kNoMove) atomicMin((unsigned long long*)&s_key[L], (unsigned long long)best);
+      }
+      __syncthreads();
+      // decode every run's choice into its window (a, b), packed as (a << 32 | b)
+      for (int x = tid; x < n; x += kMultiBlock) {
+        if (s_run[x] == x && s_key[x] != kNoMove) {
+          const uint32_t idx = key_index(s_key[x]);
+          int a, b;
+          if (cand == nullptr) {
+            a = (int)(idx / (uint32_t)n);
+            b = (int)(idx - (uint32_t)a * (uint32_t)n);
+          } else {
+            a = (int)(idx / (uint32_t)k);
+            b = s_pos[cand[(size_t)s_tour[a] * k + (int)(idx - (uint32_t)a * (uint32_t)k)]];
+          }
+          s_key[x] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+        }
+      }
+      __syncthreads();
+      // relabel positions; the first position of each accepted window records the move
+      for (int x = tid; x < n; x += kMultiBlock) {
+        const int L = s_run[x];
+        if (L < 0) continue;
+        const uint64_t w = s_key[L];
+        if (w == kNoMove) {
+          s_run[x] = -1;  // run without improving pairs: done
+          continue;
+        }
+        const int a = (int)(w >> 32), b = (int)(uint32_t)w;
+        if (x == a) {
+          const int m = atomicAdd(&s_count, 1);
+          s_moves[s_total + m] = make_int4(a, b, L, s_rend[L]);
+        }
+        s_run[x] = x < a ? L : (x <= b ? -1 : b + 1);
+      }
+      __syncthreads();
+      const int cnt = s_count;
+      if (cnt == 0) break;
+      for (int m = tid; m < cnt; m += kMultiBlock) {
+        const int4 mv = s_moves[s_total + m];
+        if (mv.x > mv.z) s_rend[mv.z] = mv.x - 1;
+        if (mv.y < mv.w) s_rend[mv.y + 1] = mv.w;
+      }
+      __syncthreads();
+      if (tid == 0) s_total += cnt;
+      __syncthreads();
+    }
+    const int total = s_total;
+    if (total == 0) break;
+    moves += total;
+    // apply the disjoint reversals of positions a+1 .. b
+    for (int x = tid; x < n; x += kMultiBlock) s_rend[x] = s_tour[x];
+    __syncthreads();
+    for (int m = 0; m < total; ++m) {
+      const int a = s_moves[m].x, b = s_moves[m].y;
+      for (int p = a + 1 + tid; p <= b; p += kMultiBlock) s_tour[p] = s_rend[a + 1 + b - p];
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < n; x += kMultiBlock) tour[x] = s_tour[x];
+  if (stats != nullptr) {
+    const int64_t all = block_sum_i64(my_deltas, red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)all);
+      atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+      atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+    }
+  }
+}
+
+size_t multi_lds_bytes(int n, bool use_cand) {
+  return (size_t)n * sizeof(uint64_t) + (size_t)(4 * n + 1) * sizeof(int32_t) + (size_t)(n / 3 + 1) * sizeof(int4) +
+         (use_cand ? (size_t)n * sizeof(int32_t) : 0) + 16;
+}
+
 }  // namespace tspmi
 
 extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* cand, int k,
                              int mode, int max_iter, int64_t* stats, void* stream) {
   using namespace tspmi;
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt: negative shape P=%d n=%d", P, n);
-  TSPMI_CHECK_ARG(mode == TSPMI_MODE_BEST || mode == TSPMI_MODE_FIRST, "tspmi_two_opt: bad mode %d", mode);
+  TSPMI_CHECK_ARG(mode == TSPMI_MODE_BEST || mode == TSPMI_MODE_FIRST || mode == TSPMI_MODE_MULTI,
+                  "tspmi_two_opt: bad mode %d", mode);
   if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;  // reference: n < 4 or no budget is a copy
   TSPMI_CHECK_ARG(tours != nullptr && D != nullptr, "tspmi_two_opt: null tours or distances");
   TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt: ld=%d < n=%d", ld, n);
@@ -398,6 +571,15 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
     TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt: (n-2)*k too large");
   }
   hipStream_t s = (hipStream_t)stream;
+  if (mode == TSPMI_MODE_MULTI) {
+    const size_t ml = multi_lds_bytes(n, use_cand);
+    TSPMI_CHECK_ARG(ml <= (size_t)kMaxLds - 512, "tspmi_two_opt: multi_move at n=%d needs %zu B of LDS (max %d)", n, ml,
+                    kMaxLds);
+    hipLaunchKernelGGL(two_opt_multi_kernel, dim3(P), dim3(kMultiBlock), ml, s, tours, n, D, ld, use_cand ? cand : nullptr,
+                       use_cand ? k : 0, max_iter, stats);
+    TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
+    return TSPMI_OK;
+  }
   // The diagonal scan reads up to QMAX * 256 + 64 entries past the tour/edge arrays (zero padding).
   constexpr int kQ = 8;
   const int q = (!use_cand && mode == TSPMI_MODE_BEST) ? (n - 2 <= kTwoOptWaves * 4 * kWave ? 4

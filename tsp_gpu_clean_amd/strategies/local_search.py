@@ -12,12 +12,17 @@ from .base import StrategyBase
 
 TSPMI_MODE_BEST = 0
 TSPMI_MODE_FIRST = 1
+TSPMI_MODE_MULTI = 2
 
 #: Accepted modes.  The reference GPU kernel families "batch" and "naive" both apply the single
 #: steepest move per pass; here they share the steepest kernel, whose ties follow the reference
 #: CPU scan order (lowest (i, j)).  "best"/"first" name the CPU acceptance rules directly.
+#: "multi_move" applies every mutually disjoint improving move of a pass by the exact CPU rule
+#: (two_opt.py:491-563: steepest first, ties in scan order); the reference's GPU kernel of that
+#: name offers only one candidate per thread (two_opt.py:649-655), so it is the CPU rule that
+#: this device kernel reproduces bit for bit.
 _MODES: dict[str, int] = {"batch": TSPMI_MODE_BEST, "naive": TSPMI_MODE_BEST, "best": TSPMI_MODE_BEST,
-                          "first": TSPMI_MODE_FIRST}
+                          "first": TSPMI_MODE_FIRST, "multi_move": TSPMI_MODE_MULTI}
 
 
 class LocalSearchGPUTourTwoOpt(StrategyBase):
