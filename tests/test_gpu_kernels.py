@@ -417,7 +417,8 @@ def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
   np.testing.assert_array_equal(same, got)
 
 
-@pytest.mark.parametrize("n,iters,rows", [(2100, 60, 3), (3000, 40, 2), (10000, 6, 2)])
+@pytest.mark.parametrize("n,iters,rows", [(2100, 60, 3), (3000, 40, 2), (6147, 10, 2), (10000, 6, 2), (10243, 4, 1),
+                                           (12000, 3, 1)])
 def test_two_opt_wide_large_n_matches_c_oracle(torch_mod, nat, n, iters, rows):
   say(f"two_opt wide n={n} iters={iters}")
   d = compute_distance_matrix(uniform_coords(n, seed=n))
@@ -426,3 +427,15 @@ def test_two_opt_wide_large_n_matches_c_oracle(torch_mod, nat, n, iters, rows):
   want, wstats = c_two_opt.improve_batch(tours, d, None, "best", iters)
   np.testing.assert_array_equal(got, want)
   assert stats[1] == wstats[1]
+
+
+def test_two_opt_large_n_lattice_ties_match_c_oracle(torch_mod, nat):
+  """Row-staged scan on a tie-heavy lattice (n = 2116): ties resolve to the lowest (i, j)."""
+  say("two_opt staged lattice n=2116")
+  n = 2116
+  d = lattice_matrix(n)
+  tours = oinit.random_init(np.arange(n), 2, oinit.draw_row_seeds(np.random.default_rng(5), 2))
+  got, stats = _run_two_opt(nat, torch_mod, tours, d, None, "best", 40)
+  want, wstats = c_two_opt.improve_batch(tours, d, None, "best", 40)
+  np.testing.assert_array_equal(got, want)
+  assert list(stats) == list(wstats)

@@ -186,6 +186,136 @@ __global__ __launch_bounds__(kWideBlock) void two_opt_wide_kernel(int32_t* __res
   }
 }
 
+// Row-staged variant for rows too long for the L1 (n > ~2000): the diagonal scan reads the new
+// row D[t_{i+2}][*] at n random columns per step; here the whole row is first copied into LDS with
+// coalesced loads (issued two steps ahead into registers), so the random accesses hit LDS and
+// every 128-byte line of the row is fetched once per step.  Two workgroup barriers per step.
+// Needs (3n + 1) int32 of LDS and n - 2 <= QMAX * 1024 + 1 (QMAX <= 10 keeps it within 128 VGPRs).
+template <int E>
+__device__ __forceinline__ void fetch_row(int (&r)[E], const int32_t* __restrict__ D, int ld, int node, int n) {
+  const int32_t* __restrict__ row = D + (size_t)node * ld;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = threadIdx.x + e * kWideBlock;
+    r[e] = x < n ? row[x] : 0;
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void stage_row(const int (&r)[E], int32_t* s_row, int n) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = threadIdx.x + e * kWideBlock;
+    if (x < n) s_row[x] = r[e];
+  }
+}
+
+template <int QMAX>
+__device__ __forceinline__ uint64_t scan_full_best_staged(const int32_t* s_tour, const int32_t* s_edge, int32_t* s_row,
+                                                          int n, const int32_t* __restrict__ D, int ld, int wid,
+                                                          int lane) {
+  constexpr int E = QMAX + 1;
+  const int rows = n - 2;
+  const int jq0 = 2 + wid * kWave + lane;
+  int carry[QMAX], gcur[QMAX], bd[QMAX], bi[QMAX];
+  {
+    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
+    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = jq0 + q * kWideBlock;
+      carry[q] = row_at(r0, s_tour[min(j, n)]);
+      gcur[q] = row_at(r1, s_tour[min(j + 1, n)]);
+      bd[q] = 0;
+      bi[q] = 0;
+    }
+  }
+  int ra[E], rb[E];
+  fetch_row(ra, D, ld, __builtin_amdgcn_readfirstlane(s_tour[2]), n);
+  fetch_row(rb, D, ld, __builtin_amdgcn_readfirstlane(s_tour[min(3, n)]), n);
+  auto step = [&](int i) {
+    if (i + 2 + kWave * wid >= n) return;  // every column of this wave lies past the tour
+    const int eab = s_edge[i];
+    const int lim = (i == 0) ? n - 1 : n;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = i + jq0 + q * kWideBlock;
+      const int gnext = s_row[s_tour[min(j + 2, n)]];
+      const int delta = carry[q] + gcur[q] - eab - s_edge[min(j, n - 1)];
+      const bool take = (j < lim) & (delta < bd[q]);
+      bd[q] = take ? delta : bd[q];
+      bi[q] = take ? i : bi[q];
+      carry[q] = gcur[q];
+      gcur[q] = gnext;
+    }
+  };
+  for (int i = 0; i < rows; i += 2) {
+    __syncthreads();
+    stage_row(ra, s_row, n);
+    if (i + 4 < n) fetch_row(ra, D, ld, __builtin_amdgcn_readfirstlane(s_tour[i + 4]), n);
+    __syncthreads();
+    step(i);
+    if (i + 1 >= rows) break;
+    __syncthreads();
+    stage_row(rb, s_row, n);
+    if (i + 5 < n) fetch_row(rb, D, ld, __builtin_amdgcn_readfirstlane(s_tour[i + 5]), n);
+    __syncthreads();
+    step(i + 1);
+  }
+  uint64_t best = kNoMove;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (bd[q] < 0) {
+      const uint32_t r = (uint32_t)bi[q];
+      const uint64_t key = pack_key(bd[q], r * (uint32_t)n + r + (uint32_t)(jq0 + q * kWideBlock));
+      best = key < best ? key : best;
+    }
+  }
+  return best;
+}
+
+template <int QMAX>
+__global__ __launch_bounds__(kWideBlock) void two_opt_staged_kernel(int32_t* __restrict__ tours, int n,
+                                                                    const int32_t* __restrict__ D, int ld,
+                                                                    int max_iter, int64_t* __restrict__ stats) {
+  extern __shared__ int32_t lds[];
+  __shared__ uint64_t red[kWideWaves];
+  int32_t* s_tour = lds;          // n + 1 (s_tour[n] mirrors s_tour[0])
+  int32_t* s_edge = lds + n + 1;  // n
+  int32_t* s_row = s_edge + n;    // n
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+  for (int x = tid; x < n; x += kWideBlock) {
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += kWideBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
+  __syncthreads();
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = scan_full_best_staged<QMAX>(s_tour, s_edge, s_row, n, D, ld, wid, lane);
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)n);
+    const int j = (int)(idx - (uint32_t)i * (uint32_t)n);
+    __syncthreads();
+    apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
+    ++moves;
+  }
+  for (int x = tid; x < n; x += kWideBlock) tour[x] = s_tour[x];
+  if (stats != nullptr && tid == 0) {
+    atomicAdd((unsigned long long*)&stats[0], (unsigned long long)((int64_t)passes * n * (n - 3) / 2));
+    atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+    atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+  }
+}
+
 template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restrict__ tours, int n,
                                                                const int32_t* __restrict__ D, int ld,
@@ -589,7 +719,16 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   const size_t lds = ((size_t)(use_cand ? 3 : 2) * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + sizeof(TwoOptShared) <= (size_t)kMaxLds,
                   "tspmi_two_opt: n=%d needs %zu B of LDS (max %d)", n, lds, kMaxLds);
-  if (!use_cand && mode == TSPMI_MODE_BEST && q == 0 && n - 2 <= kWideWaves * 16 * kWave &&
+  const size_t staged_lds = (size_t)(3 * n + 1) * sizeof(int32_t);
+  if (!use_cand && mode == TSPMI_MODE_BEST && q == 0 && n - 2 <= 10 * kWideBlock + 1 &&
+      staged_lds + 1024 <= (size_t)kMaxLds) {
+    if (n - 2 <= 6 * kWideBlock + 1)
+      hipLaunchKernelGGL(two_opt_staged_kernel<6>, dim3(P), dim3(kWideBlock), staged_lds, s, tours, n, D, ld, max_iter,
+                         stats);
+    else
+      hipLaunchKernelGGL(two_opt_staged_kernel<10>, dim3(P), dim3(kWideBlock), staged_lds, s, tours, n, D, ld, max_iter,
+                         stats);
+  } else if (!use_cand && mode == TSPMI_MODE_BEST && q == 0 && n - 2 <= kWideWaves * 16 * kWave &&
       (size_t)(2 * n + 1) * sizeof(int32_t) + 1024 <= (size_t)kMaxLds) {
     const size_t wl = (size_t)(2 * n + 1) * sizeof(int32_t);
     if (n - 2 <= kWideWaves * 10 * kWave)
