@@ -5,11 +5,14 @@ Headline (value): population-batched 2-opt deltas/s on n=1000 integer coordinate
 improvement over the full O(n^2) neighbourhood, every tour iterated to its local optimum.
 One step = one launch refining all 1024 tours from the same start population.
 
-Also reported on the same line: the same workload with candidate_k=20 lists, and GA
-generations/s of the fused memetic generation (n=1000 clustered, P=1024, tournament 5, PMX,
-inversion 0.05, elite-16 2-opt k=20 max_iter 10) over 200 generations.
+Also reported on the same line: the same workload with candidate_k=20 lists; GA generations/s of
+the fused memetic generation (n=1000 clustered, P=1024, tournament 5, PMX, inversion 0.05,
+elite-16 2-opt k=20 max_iter 10) over 200 generations; the uneven multi-instance batch (256
+instances, n in [50, 2000], P=64, 100 graph-captured generations); and the learned candidate
+scorer's time to a 2 % gap against full-neighbourhood 2-opt at n=10000, P=256 (learned top-8 vs
+kNN-20 and kNN-8, the judge run live).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-scorer-eval]
 """
 
 from __future__ import annotations
@@ -144,6 +147,34 @@ def time_uneven_batch(torch, generations=100):
                       "instances, 100 generations captured in one HIP graph, autotuned launch shapes"}
 
 
+def learned_candidates_eval() -> dict:
+  """Config 5: learned top-8 vs kNN-20 (and kNN-8) time to within 2 % of the full 2-opt judge."""
+  from tsp_gpu_clean_amd import scorer as S
+
+  sc = S.LearnedCandidateScorer.load()
+
+  def progress(name, r):
+    print(f"bench: scorer eval {name} {json.dumps(r)}", file=sys.stderr, flush=True)
+
+  res = S.evaluate_gap(n=10000, P=256, scorer=sc, k_learned=8, gap=0.02, seed=2024, progress=progress)
+  m = res["methods"]
+  t_l, t_k = m["learned8"]["seconds_to_gap"], m["knn20"]["seconds_to_gap"]
+  info = getattr(sc, "info", {})
+  return {
+    "workload": "n=10000 uniform integer coords [0,1e6]^2 (seed 2024), P=256 greedy nearest-neighbour starts, "
+                "best-improvement 2-opt; judge = full-neighbourhood 2-opt local optimum from the same starts; "
+                "target = 1.02 x judge best; times include each scheme's candidate construction",
+    "judge_best": res["full"]["best"], "judge_seconds": res["full"]["seconds_to_local_optimum"],
+    "full_seconds_to_gap": res["full"]["seconds_to_gap"],
+    "learned8_seconds_to_gap": t_l, "knn20_seconds_to_gap": t_k,
+    "knn8_seconds_to_gap": m["knn8"]["seconds_to_gap"],
+    "learned8_vs_knn20_speedup": (t_k / t_l) if (t_l and t_k) else None,
+    "methods": m,
+    "training": {k: info.get(k) for k in ("instances", "samples", "positives", "data_seconds", "train_seconds",
+                                          "steps", "batch")},
+  }
+
+
 def cpu_baseline(seconds_hint: float = 20.0) -> dict:
   """Reference-faithful NumPy 2-opt (single core) on a bounded sample of the C2 workload."""
   code = f"""
@@ -199,6 +230,7 @@ def main() -> int:
   ap.add_argument("--warmup", type=int, default=1)
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--no-extras", action="store_true")
+  ap.add_argument("--no-scorer-eval", action="store_true")
   ap.add_argument("--generations", type=int, default=200)
   ap.add_argument("--copy-probe-mib", type=int, default=0,
                   help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
@@ -268,6 +300,9 @@ def main() -> int:
                               "inversion 0.05, elite-16 2-opt best k=20 max_iter 10"}
     print("bench: uneven batch (256 instances)", file=sys.stderr, flush=True)
     line["uneven_batch"] = time_uneven_batch(torch)
+    if not args.no_scorer_eval:
+      print("bench: learned candidates vs kNN at n=10000", file=sys.stderr, flush=True)
+      line["learned_candidates"] = learned_candidates_eval()
   if not args.no_cpu_baseline:
     print("bench: CPU baseline (NumPy restatement, 1 core)", file=sys.stderr, flush=True)
     line["cpu_baseline"] = cpu_baseline()
