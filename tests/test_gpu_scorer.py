@@ -240,3 +240,26 @@ def test_learned_candidates_gain_over_knn_at_equal_width(env):
   knn8 = res["methods"]["knn8"]["mean"]
   assert learned < knn8, (learned, knn8, full)
   assert res["methods"]["learned8"]["best"] <= 1.05 * res["full"]["best"]
+
+
+def test_local_search_strategy_with_learned_candidates(env):
+  """The drop-in strategy with the shipped scorer: bit-exact against the oracle 2-opt run on the
+  same learned lists, in best and multi_move modes."""
+  torch, nat = env
+  import tsp_gpu_clean_amd as t
+  from oracle import two_opt as o2
+  from tsp_gpu_clean_amd.scorer import LearnedCandidateScorer
+
+  rng = np.random.default_rng(12)
+  xy = rng.integers(0, 1_000_001, (600, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("u600", xy), seed=1)
+  sc = LearnedCandidateScorer.load()
+  routes = np.stack([rng.permutation(600) for _ in range(8)]).astype(np.int32)
+  for mode in ("best", "multi_move"):
+    ls = t.LocalSearchGPUTourTwoOpt(max_iterations=25, mode=mode, candidate_k=8, candidate_scorer=sc)
+    got = ls.improve_batch(routes, ctx)
+    cand = ctx.get_gpu_learned_candidates(sc, 8)[0].cpu().numpy()
+    want = o2.improve_batch(routes, ctx.get_cpu_distances(), cand, mode, 25)
+    np.testing.assert_array_equal(got, want)
+  knn = ctx.get_cpu_candidate_list(8)
+  assert not np.array_equal(np.sort(cand, axis=1), np.sort(knn, axis=1))  # the lists really differ

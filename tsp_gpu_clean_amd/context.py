@@ -138,6 +138,27 @@ class ProblemContext:
       self._gpu_candidates[key] = table
     return table
 
+  def get_gpu_learned_candidates(self, scorer, k: int = 8):
+    """(list (n, k), packed table) from a learned candidate scorer, built once per (scorer, k).
+
+    The scorer ranks each city's 32 nearest neighbours; an instance with n <= 32 keeps its k-NN
+    lists, since every other city is already a neighbour."""
+    torch = _native.require_device()
+    key = ("learned", k, id(scorer))
+    hit = self._gpu_candidates.get(key)
+    if hit is None:
+      n = self.dimension
+      if n <= 32:
+        cand = self.get_gpu_candidate_list(k)
+        hit = (cand, self.get_gpu_candidate_table(k))
+      else:
+        cand = scorer.candidates(self, k)
+        table = torch.empty((n, _native.candidate_lanes(k), 2), dtype=torch.int32, device="cuda")
+        _native.candidate_table(cand, self.get_gpu_distances(), table)
+        hit = (cand, table)
+      self._gpu_candidates[key] = hit
+    return hit
+
   def get_cpu_candidate_list(self, k: int = 20) -> np.ndarray:
     cached = self._cpu_candidates.get(k)
     if cached is None:

@@ -37,7 +37,10 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     max_shared_tour_cities: int = 13_000,
     candidate_k: int | None = None,
     seed: Any = None,
+    candidate_scorer: Any = None,
   ) -> None:
+    """``candidate_scorer`` (a ``LearnedCandidateScorer``) replaces the k-nearest lists with the
+    k best-scored of each city's 32 nearest neighbours; None keeps the reference's k-NN lists."""
     super().__init__(seed)
     if mode not in _MODES:
       raise ValueError(f"Invalid 2-opt GPU mode {mode!r}; expected one of {sorted(_MODES)}")
@@ -49,6 +52,9 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     self.max_matrix_dimension = max_matrix_dimension
     self.max_shared_tour_cities = max_shared_tour_cities
     self.candidate_k = candidate_k
+    self.candidate_scorer = candidate_scorer
+    if candidate_scorer is not None and not (candidate_k and 1 <= candidate_k <= 32):
+      raise ValueError("a candidate scorer needs candidate_k in [1, 32]")
     self.last_stats: tuple[int, int, int] | None = None
     self._stats = None
 
@@ -81,13 +87,19 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     tours, _, up = to_device_int32(batch, copy=True)
     self.h2d_bytes += up
     dist = context.get_gpu_distances()
-    cand = context.get_gpu_candidate_list(int(self.candidate_k)) if self._uses_candidates(context, n) else None
+    cand = table = None
+    if self._uses_candidates(context, n):
+      if self.candidate_scorer is not None:
+        cand, table = context.get_gpu_learned_candidates(self.candidate_scorer, int(self.candidate_k))
+      else:
+        cand = context.get_gpu_candidate_list(int(self.candidate_k))
     if self._stats is None:
       self._stats = torch.zeros(3, dtype=torch.int64, device="cuda")
     else:
       self._stats.zero_()
     if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
-      table = context.get_gpu_candidate_table(int(self.candidate_k))
+      if table is None:
+        table = context.get_gpu_candidate_table(int(self.candidate_k))
       _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
     else:
       _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
