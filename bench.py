@@ -109,13 +109,15 @@ def time_generations(torch, native, generations=200, warmup=5):
   generations -= generations % 2
   graph = loop.capture(draws, generations, best, start=warmup)
   loop.load(start_pop, start_fit)
+  loop.stats.zero_()
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   graph.replay()
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   best_cost = float(loop.fitness.min().item())
-  return generations / wall, wall, best_cost
+  elite_deltas = int(loop.stats[0].item()) / generations
+  return generations / wall, wall, best_cost, elite_deltas
 
 
 def time_uneven_batch(torch, generations=100):
@@ -133,15 +135,19 @@ def time_uneven_batch(torch, generations=100):
   shapes = ga.autotune(d, g=generations)
   ga.run(d, 2, start=generations)  # warm-up generations on the spare draw slots
   graph = ga.capture(d, generations)
+  ga.stats.zero_()
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   graph.replay()
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   ns = np.array([c.shape[0] for c in coords])
+  gbytes = 16 * 64 * int(ns.sum()) + 8 * int(ga.stats[0].item()) / generations
   return {"instances": 256, "n_range": [int(ns.min()), int(ns.max())], "population": 64, "generations": generations,
           "wall_s": wall, "gens_per_s": generations / wall, "instance_gens_per_s": 256 * generations / wall,
           "setup_s": setup, "distance_bytes": int(4 * (ns * ns).sum()), "shapes": shapes,
+          "bytes_per_generation": gbytes, "achieved_GBps": gbytes * generations / wall / 1e9,
+          "hbm_frac": gbytes * generations / wall / 1e9 / HBM_PEAK_GBS,
           "workload": "256 instances, n uniform-integer [50,2000], uniform coords [0,1e6]^2, P=64 each, tournament 5, "
                       "PMX, inversion 0.05, elite-16 2-opt best k=20 max_iter 10, one launch per stage for all "
                       "instances, 100 generations captured in one HIP graph, autotuned launch shapes"}
@@ -157,6 +163,7 @@ def learned_candidates_eval() -> dict:
     print(f"bench: scorer eval {name} {json.dumps(r)}", file=sys.stderr, flush=True)
 
   res = S.evaluate_gap(n=10000, P=256, scorer=sc, k_learned=8, gap=0.02, seed=2024, progress=progress)
+  mlp = mlp_throughput(sc)
   m = res["methods"]
   t_l, t_k = m["learned8"]["seconds_to_gap"], m["knn20"]["seconds_to_gap"]
   info = getattr(sc, "info", {})
@@ -172,46 +179,169 @@ def learned_candidates_eval() -> dict:
     "methods": m,
     "training": {k: info.get(k) for k in ("instances", "samples", "positives", "data_seconds", "train_seconds",
                                           "steps", "batch")},
+    "mlp": mlp,
   }
 
 
-def cpu_baseline(seconds_hint: float = 20.0) -> dict:
-  """Reference-faithful NumPy 2-opt (single core) on a bounded sample of the C2 workload."""
-  code = f"""
-import os, sys, time, json
-os.sched_setaffinity(0, {{sorted(os.sched_getaffinity(0))[0]}})
-sys.path.insert(0, {ROOT!r})
+MLP_FLOP_FWD = 2 * (12 * 128 + 128 * 128 + 128)  # per (city, neighbour) candidate
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+
+
+def mlp_throughput(sc, reps: int = 20) -> dict:
+  """Forward over the 320,000 candidates of an n=10000 instance, and one training step (forward,
+  backward, weight gradients, reduction, Adam) on 65,536 samples; random bf16 features."""
+  import torch
+
+  from tsp_gpu_clean_amd import _native
+  from tsp_gpu_clean_amd import scorer as S
+
+  g = torch.Generator(device="cpu").manual_seed(0)
+  out = {}
+  for S_, kind in ((320_000, "forward"), (65_536, "train_step")):
+    x = torch.randn((S_, 16), generator=g)
+    x[:, 12:] = 0
+    X = x.to(torch.bfloat16).view(torch.int16).cuda()
+    stream = torch.cuda.current_stream()
+    if kind == "forward":
+      z = torch.empty(S_, dtype=torch.float32, device="cuda")
+      fn = lambda: _native.mlp_forward(X, sc.wbf, z)  # noqa: E731
+      flop = S_ * MLP_FLOP_FWD
+    else:
+      y = (torch.rand(S_, generator=g) < 0.06).float().cuda()
+      w, p = _native.mlp_work_sizes(S_)
+      work = torch.empty(w, dtype=torch.int16, device="cuda")
+      part = torch.empty(p, dtype=torch.float32, device="cuda")
+      grad = torch.empty(S.NPARAM + 1, dtype=torch.float32, device="cuda")
+      th, m, v, wb = sc.theta.clone(), torch.zeros_like(sc.m), torch.zeros_like(sc.v), sc.wbf.clone()
+
+      def fn():
+        _native.mlp_grad(X, y, wb, work, part, grad)
+        _native.adam(th, m, v, grad, 1, 1e-3, 0.9, 0.999, 1e-8, 0.0, wb)
+      flop = S_ * 3 * MLP_FLOP_FWD
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+      fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tf = flop / (ms * 1e-3) / 1e12
+    out[kind] = {"samples": S_, "ms": ms, "TFLOPs": tf, "mfma_frac": tf / BF16_PEAK_TFLOPS,
+                 "flop_per_sample": flop // S_}
+  return out
+
+
+_CPU_SCRIPT = r"""
+import json, os, sys, time
+sys.path.insert(0, ROOT)
 import numpy as np
-from oracle import two_opt as o2, init as oinit
-from oracle.distances import compute_distance_matrix
-xy = np.random.default_rng(42).integers(0, 1_000_001, ({N_CITIES}, 2)).astype(np.float64)
-d = compute_distance_matrix(xy)
-tours = oinit.random_init(np.arange({N_CITIES}), {POP}, oinit.draw_row_seeds(np.random.default_rng(42), {POP}))
-c = o2.new_counter()
-t0 = time.perf_counter(); rows = 0
-while rows < {POP} and (time.perf_counter() - t0) < {seconds_hint}:
-  o2.improve_batch(tours[rows:rows + 1], d, None, "best", 10**9, c); rows += 1
-wall = time.perf_counter() - t0
-print(json.dumps({{"deltas": c.deltas, "wall": wall, "rows": rows, "passes": c.passes}}))
+from oracle import two_opt as o2, init as oinit, ga as oga
+from oracle.distances import compute_candidate_list, compute_distance_matrix
+
+def c2_tours():
+  xy = np.random.default_rng(42).integers(0, 1_000_001, (N, 2)).astype(np.float64)
+  d = compute_distance_matrix(xy)
+  return d, oinit.random_init(np.arange(N), P, oinit.draw_row_seeds(np.random.default_rng(42), P))
+
+def refine_row(args):
+  os.sched_setaffinity(0, {args[2]})
+  d, tours = c2_tours()
+  c = o2.new_counter()
+  out = o2.improve_batch(tours[args[0]:args[1]], d, None, "best", 10**9, c)
+  return out, c.deltas, c.passes
+
+if MODE == "single":
+  os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+  d, tours = c2_tours()
+  c = o2.new_counter()
+  t0 = time.perf_counter(); rows = 0; outs = []
+  while rows < P and (time.perf_counter() - t0) < SECONDS:
+    outs.append(o2.improve_batch(tours[rows:rows + 1], d, None, "best", 10**9, c)); rows += 1
+  wall = time.perf_counter() - t0
+  np.save(OUT, np.concatenate(outs))
+  print(json.dumps({"deltas": c.deltas, "wall": wall, "rows": rows, "passes": c.passes, "first_row": 0}))
+elif MODE == "all":
+  import multiprocessing as mp
+  cores = sorted(os.sched_getaffinity(0))[:WORKERS]
+  jobs = [(FIRST + i, FIRST + i + 1, c) for i, c in enumerate(cores)]
+  t0 = time.perf_counter()
+  with mp.get_context("fork").Pool(len(jobs)) as pool:
+    res = pool.map(refine_row, jobs)
+  wall = time.perf_counter() - t0
+  np.save(OUT, np.concatenate([r[0] for r in res]))
+  print(json.dumps({"deltas": int(sum(r[1] for r in res)), "wall": wall, "rows": len(jobs),
+                    "passes": int(sum(r[2] for r in res)), "first_row": FIRST, "workers": len(jobs)}))
+else:
+  os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+  rng = np.random.default_rng(42)
+  centres = rng.uniform(0, 1e6, (20, 2))
+  lab = rng.integers(0, 20, N)
+  xy = np.rint(np.clip(centres[lab] + rng.normal(0, 0.02 * 1e6, (N, 2)), 0, 1e6))
+  d = compute_distance_matrix(xy)
+  grng = np.random.default_rng(42)
+  pop = oga.init_population(grng, d, N, P, "random")
+  ls = oga.LocalSearchSpec("best", 10, compute_candidate_list(d, 20))
+  t0 = time.perf_counter()
+  r = oga.evolve(grng, d, pop, max_generations=GENS, mutation_rate=0.05, tournament_k=5, local_search=ls,
+                 elite_count=16, patience=10**9)
+  wall = time.perf_counter() - t0
+  print(json.dumps({"generations": r.stats["generations_completed"], "wall": wall, "best": r.stats["best_fitness"]}))
 """
+
+
+def _run_cpu(mode: str, out_path: str, seconds: float = 20.0, first: int = 0, workers: int = 16, gens: int = 10):
+  head = (f"ROOT = {ROOT!r}\nN = {N_CITIES}\nP = {POP}\nMODE = {mode!r}\nOUT = {out_path!r}\n"
+          f"SECONDS = {seconds}\nFIRST = {first}\nWORKERS = {workers}\nGENS = {gens}\n")
   env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-  out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600, check=True)
-  r = json.loads(out.stdout.strip().splitlines()[-1])
+  out = subprocess.run([sys.executable, "-c", head + _CPU_SCRIPT], capture_output=True, text=True, env=env,
+                       timeout=900, check=True)
+  return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(gpu_tours=None, seconds_hint: float = 20.0) -> dict:
+  """Reference-faithful NumPy 2-opt on bounded samples of the C2 workload: one pinned core for
+  about ``seconds_hint`` s, then one tour per core on up to 16 cores; the refined sample rows are
+  compared with the GPU's rows of the same start population.  Also the oracle GA (C3) on one core
+  for 10 generations."""
+  import tempfile
+
+  cores = len(os.sched_getaffinity(0))
+  workers = min(16, cores)
+  with tempfile.TemporaryDirectory() as tmp:
+    single = _run_cpu("single", os.path.join(tmp, "single.npy"), seconds=seconds_hint)
+    allc = _run_cpu("all", os.path.join(tmp, "all.npy"), first=single["rows"], workers=workers)
+    ga = _run_cpu("ga", os.path.join(tmp, "ga.npy"), gens=10)
+    exact = None
+    if gpu_tours is not None:
+      got = np.concatenate([np.load(os.path.join(tmp, "single.npy")), np.load(os.path.join(tmp, "all.npy"))])
+      rows = np.arange(got.shape[0])
+      exact = bool(np.array_equal(got, gpu_tours[rows]))
   cpu = "unknown"
   try:
     with open("/proc/cpuinfo", encoding="utf-8") as fh:
       cpu = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "unknown")
   except OSError:
     pass
+  if exact is False:
+    raise RuntimeError("CPU baseline rows differ from the GPU's refined rows")
   return {
-    "value": r["deltas"] / r["wall"],
+    "value": single["deltas"] / single["wall"],
     "unit": "deltas/s",
     "cores": 1,
     "kind": "port",
-    "sample": (f"{r['rows']} tour(s) of the same workload (n=1000, full neighbourhood, best, to local optimum; "
-               f"{r['passes']} passes, {r['deltas']} deltas in {r['wall']:.1f} s) with the NumPy restatement of "
-               f"the reference CPU scan, 1 thread pinned to 1 core of {cpu}; host has "
-               f"{len(os.sched_getaffinity(0))} usable cores"),
+    "sample": (f"{single['rows']} tour(s) of the same workload (n=1000, full neighbourhood, best, to local optimum; "
+               f"{single['passes']} passes, {single['deltas']} deltas in {single['wall']:.1f} s) with the NumPy "
+               f"restatement of the reference CPU scan, 1 thread pinned to 1 core of {cpu}; host has {cores} usable "
+               f"cores"),
+    "all_cores": {"value": allc["deltas"] / allc["wall"], "unit": "deltas/s", "cores": allc["workers"],
+                  "sample": f"rows {allc['first_row']}..{allc['first_row'] + allc['rows'] - 1}, one tour per pinned "
+                            f"core, {allc['deltas']} deltas in {allc['wall']:.1f} s"},
+    "bit_exact_rows": (single["rows"] + allc["rows"]) if exact else None,
+    "ga_gens_per_s": {"value": ga["generations"] / ga["wall"], "unit": "generations/s", "cores": 1,
+                      "sample": f"oracle GA on the C3 workload (P=1024, n=1000 clustered, elite-16 2-opt k=20), "
+                                f"{ga['generations']} generations in {ga['wall']:.1f} s"},
   }
 
 
@@ -255,7 +385,7 @@ def main() -> int:
     return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
   _, D, start = c2_inputs(torch, _native)
-  sec, deltas, moves, kms, _ = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
+  sec, deltas, moves, kms, refined = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
   print(f"bench: full neighbourhood {deltas / sec:.3e} deltas/s, {kms:.1f} ms/launch", file=sys.stderr, flush=True)
   value = deltas / sec
   achieved = deltas * BYTES_PER_DELTA / (kms * 1e-3) / 1e9
@@ -293,9 +423,12 @@ def main() -> int:
     line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
                              "deltas_per_step": d20}
     print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)
-    gps, gwall, gbest = time_generations(torch, _native, args.generations)
+    gps, gwall, gbest, gdel = time_generations(torch, _native, args.generations)
     line["gens_per_s"] = gps
+    gbytes = 16 * POP * N_CITIES + 8 * gdel
     line["ga"] = {"generations": args.generations, "wall_s": gwall, "best_cost": gbest,
+                  "bytes_per_generation": gbytes, "elite_deltas_per_generation": gdel,
+                  "achieved_GBps": gbytes * gps / 1e9, "hbm_frac": gbytes * gps / 1e9 / HBM_PEAK_GBS,
                   "workload": "n=1000 clustered (20 Gaussian clusters, sigma 2% of 1e6), P=1024, tournament 5, PMX, "
                               "inversion 0.05, elite-16 2-opt best k=20 max_iter 10"}
     print("bench: uneven batch (256 instances)", file=sys.stderr, flush=True)
@@ -305,7 +438,7 @@ def main() -> int:
       line["learned_candidates"] = learned_candidates_eval()
   if not args.no_cpu_baseline:
     print("bench: CPU baseline (NumPy restatement, 1 core)", file=sys.stderr, flush=True)
-    line["cpu_baseline"] = cpu_baseline()
+    line["cpu_baseline"] = cpu_baseline(refined.cpu().numpy())
   print(json.dumps(line), flush=True)
   return 0
 

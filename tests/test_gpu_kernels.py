@@ -403,7 +403,7 @@ def test_packed_candidate_ties_and_duplicates(torch_mod, nat):
   np.testing.assert_array_equal(got, o2.improve_batch(tours, d, cand, "best", 30))
 
 
-@pytest.mark.parametrize("P", [24, 512])  # 1024-thread (few tours) and 256-thread workgroups
+@pytest.mark.parametrize("P", [24, 300, 600])  # 1024-, 512- and 256-thread workgroups on 256 CUs
 def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
   say(f"two_opt_packed n=1000 P={P} to optimum")
   d = compute_distance_matrix(uniform_coords(1000))

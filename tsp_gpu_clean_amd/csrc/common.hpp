@@ -16,6 +16,9 @@ void set_error(const char* fmt, ...);
 // Map a HIP status to the library status, recording the message.
 int hip_fail(hipError_t err, const char* what);
 
+// Compute units of the current device (cached per device; 256 if the query fails).
+int device_cu_count();
+
 #define TSPMI_CHECK_ARG(cond, ...)      \
   do {                                  \
     if (!(cond)) {                      \

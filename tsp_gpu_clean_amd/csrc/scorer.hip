@@ -29,9 +29,10 @@ constexpr int kScorerParams = kOffB3 + 1;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// fp32 -> bf16, round to nearest even (gfx950 v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)lo) | ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)hi) << 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -51,28 +52,29 @@ __device__ __forceinline__ bf16x8 load_perm(const uint16_t* W, int ld, int row, 
 
 // Pack the accumulator slots 8c..8c+7 (converted to bf16) into one B operand.
 __device__ __forceinline__ bf16x8 pack_slots(const float* v) {
-  uint4 u;
-  u.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-  u.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-  u.z = (uint32_t)f2bf(v[4]) | ((uint32_t)f2bf(v[5]) << 16);
-  u.w = (uint32_t)f2bf(v[6]) | ((uint32_t)f2bf(v[7]) << 16);
-  return as_bf16x8(u);
+  return as_bf16x8(make_uint4(pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]), pack_bf16x2(v[4], v[5]),
+                              pack_bf16x2(v[6], v[7])));
 }
 
-struct alignas(16) ScorerLds {
+// Weights of the forward pass in LDS (bf16 matrices, fp32 biases).
+struct alignas(16) FwdLds {
   uint16_t w1[kH * kF];
   uint16_t w2[kH * kH];
-  uint16_t w2t[kH * kH];
   float b1[kH], b2[kH], w3[kH];
   float b3;
 };
 
-__device__ void load_scorer(ScorerLds& L, const uint16_t* __restrict__ wbf, bool need_t) {
+// Forward weights plus W2 transposed for the backward pass.
+struct alignas(16) ScorerLds : FwdLds {
+  uint16_t w2t[kH * kH];
+};
+
+__device__ void load_fwd(FwdLds& L, const uint16_t* __restrict__ wbf, uint16_t* w2t) {
   for (int x = threadIdx.x; x < kH * kF; x += blockDim.x) L.w1[x] = wbf[kOffW1 + x];
   for (int x = threadIdx.x; x < kH * kH; x += blockDim.x) {
     const uint16_t v = wbf[kOffW2 + x];
     L.w2[x] = v;
-    if (need_t) L.w2t[(x % kH) * kH + x / kH] = v;
+    if (w2t != nullptr) w2t[(x % kH) * kH + x / kH] = v;
   }
   for (int x = threadIdx.x; x < kH; x += blockDim.x) {
     L.b1[x] = bf2f(wbf[kOffB1 + x]);
@@ -83,6 +85,8 @@ __device__ void load_scorer(ScorerLds& L, const uint16_t* __restrict__ wbf, bool
   __syncthreads();
 }
 
+__device__ void load_scorer(ScorerLds& L, const uint16_t* __restrict__ wbf) { load_fwd(L, wbf, L.w2t); }
+
 // Forward of 32 samples (rows base .. base+31 of X, bf16 [S][16]) by one wave.
 // Returns z for sample s (valid in every lane; both halves agree); fills pre-activations.
 struct Fwd {
@@ -91,7 +95,7 @@ struct Fwd {
   float z;
 };
 
-__device__ __forceinline__ void forward32(const ScorerLds& L, const uint16_t* __restrict__ X, int base, int S, Fwd& f) {
+__device__ __forceinline__ void forward32(const FwdLds& L, const uint16_t* __restrict__ X, int base, int S, Fwd& f) {
   const int lane = threadIdx.x & 63;
   const int s = lane & 31, h = lane >> 5;
   const int row = base + s < S ? base + s : S - 1;
@@ -139,17 +143,56 @@ __device__ __forceinline__ void forward32(const ScorerLds& L, const uint16_t* __
 
 constexpr int kMlpBlock = 256;  // 4 waves, 32 samples each per step
 
-__global__ __launch_bounds__(kMlpBlock) void mlp_forward_kernel(const uint16_t* __restrict__ X, int S,
+// Inference forward of 32 samples by one wave, same arithmetic as forward32 but keeping only the
+// packed layer-1 activations (no pre-activation arrays), so the kernel runs at high occupancy.
+__device__ __forceinline__ float forward32_z(const FwdLds& L, const uint16_t* __restrict__ X, int base, int S) {
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 31, h = lane >> 5;
+  const int row = base + s < S ? base + s : S - 1;
+  const bf16x8 xb = as_bf16x8(*reinterpret_cast<const uint4*>(X + (size_t)row * kF + 8 * h));
+  bf16x8 bop[4][2];
+#pragma unroll
+  for (int T = 0; T < 4; ++T) {
+    const bf16x8 a = as_bf16x8(*reinterpret_cast<const uint4*>(L.w1 + (32 * T + s) * kF + 8 * h));
+    f32x16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xb, acc, 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = fmaxf(acc[8 * c + q] + L.b1[dunit(T, h, 8 * c + q)], 0.0f);
+      bop[T][c] = pack_slots(v);
+    }
+  }
+  float part = 0.0f;
+#pragma unroll 1
+  for (int To = 0; To < 4; ++To) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int T = 0; T < 4; ++T)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2, kH, 32 * To + s, 32 * T + 16 * c + 4 * h), bop[T][c],
+                                                      acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int u = dunit(To, h, i);
+      part += L.w3[u] * fmaxf(acc[i] + L.b2[u], 0.0f);
+    }
+  }
+  return part + __shfl_xor(part, 32, 64) + L.b3;
+}
+
+__global__ __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(3))) void mlp_forward_kernel(const uint16_t* __restrict__ X, int S,
                                                                 const uint16_t* __restrict__ wbf, float* __restrict__ z) {
-  __shared__ ScorerLds L;
-  load_scorer(L, wbf, false);
+  __shared__ FwdLds L;
+  load_fwd(L, wbf, nullptr);
   const int wave = blockIdx.x * (kMlpBlock / 64) + threadIdx.x / 64;
   const int waves = gridDim.x * (kMlpBlock / 64);
   const int lane = threadIdx.x & 63;
   for (int base = wave * 32; base < S; base += waves * 32) {
-    Fwd f;
-    forward32(L, X, base, S, f);
-    if (lane < 32 && base + lane < S) z[base + lane] = f.z;
+    const float zz = forward32_z(L, X, base, S);
+    if (lane < 32 && base + lane < S) z[base + lane] = zz;
   }
 }
 
@@ -163,7 +206,7 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_backward_kernel(const uint16_t*
                                                                  uint16_t* __restrict__ d1T, float* __restrict__ part) {
   __shared__ ScorerLds L;
   __shared__ float s_w3[kMlpBlock / 64][kH + 2];
-  load_scorer(L, wbf, true);
+  load_scorer(L, wbf);
   const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
   const int s = lane & 31, h = lane >> 5;
   const int base = (blockIdx.x * (kMlpBlock / 64) + wid) * 32;
@@ -253,7 +296,7 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_backward_kernel(const uint16_t*
 //   dW1[u][f] = sum_s d1T[u][s] X[s][f]      (VALU)
 //   db1[u] = sum_s d1T[u][s], db2[u] = sum_s d2T[u][s]
 // Writes a partial gradient (dW1, db1, dW2, db2) per block.
-constexpr int kGradChunk = 2048;
+constexpr int kGradChunk = 256;
 
 __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __restrict__ X, int S,
                                                               const uint16_t* __restrict__ a1T,
@@ -295,43 +338,57 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __
   for (int Ti = 0; Ti < 4; ++Ti)
 #pragma unroll
     for (int i = 0; i < 16; ++i) out[kOffW2 + dunit(wid, h, i) * kH + 32 * Ti + s] = acc[Ti][i];
-  // dW1, db1, db2 on VALU: thread t covers unit u = t / 2 and features (t % 2) * 8 .. +7 (12 used)
+  // dW1, db1, db2 on VALU: thread t covers unit u = t / 2 and features (t % 2) * 8 .. +7 (12 used);
+  // samples are read 8 at a time (S and the chunk bounds are multiples of 8).
   const int u = threadIdx.x >> 1;
   const int f0 = (threadIdx.x & 1) * 8;
+  const bool odd = (threadIdx.x & 1) != 0;
   float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  float gb1 = 0.0f, gb2 = 0.0f;
-  for (int k = lo; k < hi; ++k) {
-    const float d1 = bf2f(d1T[(size_t)u * S + k]);
-    const uint4 xv = *reinterpret_cast<const uint4*>(X + (size_t)k * kF + f0);
-    const uint32_t xs[4] = {xv.x, xv.y, xv.z, xv.w};
+  float gb = 0.0f;  // db1 (even threads) or db2 (odd threads)
+  for (int k = lo; k < hi; k += 8) {
+    const uint4 dv = *reinterpret_cast<const uint4*>(d1T + (size_t)u * S + k);
+    const uint4 ev = *reinterpret_cast<const uint4*>((odd ? d2T : d1T) + (size_t)u * S + k);
+    const uint32_t ds[4] = {dv.x, dv.y, dv.z, dv.w};
+    const uint32_t es[4] = {ev.x, ev.y, ev.z, ev.w};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      g[2 * q] += d1 * bf2f((uint16_t)(xs[q] & 0xffff));
-      g[2 * q + 1] += d1 * bf2f((uint16_t)(xs[q] >> 16));
+    for (int e = 0; e < 8; ++e) {
+      const float d1 = bf2f((uint16_t)(ds[e >> 1] >> (16 * (e & 1))));
+      gb += bf2f((uint16_t)(es[e >> 1] >> (16 * (e & 1))));
+      const uint4 xv = *reinterpret_cast<const uint4*>(X + (size_t)(k + e) * kF + f0);
+      const uint32_t xs[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        g[2 * q] += d1 * bf2f((uint16_t)(xs[q] & 0xffff));
+        g[2 * q + 1] += d1 * bf2f((uint16_t)(xs[q] >> 16));
+      }
     }
-    if ((threadIdx.x & 1) == 0) gb1 += d1;
-    else gb2 += bf2f(d2T[(size_t)u * S + k]);
   }
 #pragma unroll
   for (int q = 0; q < 8; ++q) out[kOffW1 + u * kF + f0 + q] = (f0 + q < kFeat) ? g[q] : 0.0f;
-  if ((threadIdx.x & 1) == 0) out[kOffB1 + u] = gb1;
-  else out[kOffB2 + u] = gb2;
+  out[(odd ? kOffB2 : kOffB1) + u] = gb;
 }
 
-// grad = sum of partials in block order (deterministic); returns the loss in grad[kScorerParams].
-__global__ void mlp_grad_sum_kernel(const float* __restrict__ wpart, int nw, const float* __restrict__ p1part, int n1,
-                                    float* __restrict__ grad) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+// grad = sum of the per-block partials, returning the loss in grad[kScorerParams].  A workgroup
+// covers 64 parameters with 4 lanes each; lane group y sums partial sets y, y+4, y+8, ... and the
+// four group sums are added in a fixed order, so the result does not depend on scheduling.
+constexpr int kSumParams = 64;
+
+__global__ __launch_bounds__(256) void mlp_grad_sum_kernel(const float* __restrict__ wpart, int nw,
+                                                           const float* __restrict__ p1part, int n1,
+                                                           float* __restrict__ grad) {
+  __shared__ float acc[4][kSumParams];
+  const int tx = threadIdx.x % kSumParams, ty = threadIdx.x / kSumParams;
+  const int x = blockIdx.x * kSumParams + tx;
+  float v = 0.0f;
   if (x < kOffW3) {
-    float v = 0.0f;
-    for (int b = 0; b < nw; ++b) v += wpart[(size_t)b * kOffW3 + x];
-    grad[x] = v;
+    for (int b = ty; b < nw; b += 4) v += wpart[(size_t)b * kOffW3 + x];
   } else if (x < kScorerParams + 1) {
     const int t = x - kOffW3;  // w3 [0, 128), b3 128, loss 129
-    float v = 0.0f;
-    for (int b = 0; b < n1; ++b) v += p1part[(size_t)b * (kH + 2) + t];
-    grad[x] = v;
+    for (int b = ty; b < n1; b += 4) v += p1part[(size_t)b * (kH + 2) + t];
   }
+  acc[ty][tx] = v;
+  __syncthreads();
+  if (ty == 0 && x < kScorerParams + 1) grad[x] = ((acc[0][tx] + acc[1][tx]) + acc[2][tx]) + acc[3][tx];
 }
 
 // Adam on the fp32 master weights, then refresh the bf16 copy (padding features stay zero).
@@ -464,9 +521,12 @@ extern "C" int tspmi_mlp_forward(const uint16_t* X, int S, const uint16_t* wbf, 
   TSPMI_CHECK_ARG(S >= 0, "tspmi_mlp_forward: S < 0");
   if (S == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(X && wbf && z && ((uintptr_t)X % 16) == 0, "tspmi_mlp_forward: null or misaligned pointer");
+  // Persistent grid: the weights are staged into LDS once per workgroup (36 KB, four workgroups
+  // per CU), and every wave then walks groups of 32 samples.
   const int waves = (S + 31) / 32;
   int blocks = (waves + 3) / 4;
-  blocks = blocks > 4096 ? 4096 : blocks;
+  const int cap = 3 * device_cu_count();
+  blocks = blocks > cap ? cap : blocks;
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(kMlpBlock), 0, (hipStream_t)stream, X, S, wbf, z);
   TSPMI_CHECK_LAUNCH("tspmi_mlp_forward launch");
   return TSPMI_OK;
@@ -487,7 +547,8 @@ extern "C" int tspmi_mlp_grad(const uint16_t* X, const float* y, int S, const ui
   hipLaunchKernelGGL(mlp_backward_kernel, dim3(n1), dim3(kMlpBlock), 0, s, X, y, S, wbf, 1.0f / (float)S, a1T, d2T, d1T,
                      p1);
   hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(nw), dim3(kMlpBlock), 0, s, X, S, a1T, d2T, d1T, pw);
-  hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3((kScorerParams + 1 + 255) / 256), dim3(256), 0, s, pw, nw, p1, n1, grad);
+  hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3((kScorerParams + 1 + kSumParams - 1) / kSumParams), dim3(256), 0, s, pw,
+                     nw, p1, n1, grad);
   TSPMI_CHECK_LAUNCH("tspmi_mlp_grad launch");
   return TSPMI_OK;
 }
