@@ -101,6 +101,18 @@ int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld,
                          int max_iter, int64_t* stats, void* stream);
 
 /*
+ * EUC_2D variants that compute every distance from the coordinates xy (n, 2) float64 (16-byte
+ * aligned) with the FP64 nint of tspmi_dist_euc2d instead of reading a matrix: same tours, moves
+ * and counters as tspmi_two_opt(mode = BEST, full neighbourhood) and tspmi_two_opt_packed on the
+ * matrix those coordinates produce.  The full-neighbourhood form needs n <= 8194.
+ * Replaces: src/algorithms/strategies/local_search/two_opt.py:831-899 (best mode) for EUC_2D
+ *           problems, reading coordinates where the reference reads context.get_gpu_distances().
+ */
+int tspmi_two_opt_euc2d(int32_t* tours, int P, int n, const double* xy, int max_iter, int64_t* stats, void* stream);
+int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
+                               int max_iter, int64_t* stats, void* stream);
+
+/*
  * Tournament selection: out[r] = pop[w_r] with w_r = cand[r, argmin_c fitness[cand[r, c]]]
  * (first minimum).  cand is (num_parents, k) int32 drawn on the host.
  * Replaces: src/algorithms/strategies/selection/tournament_selection.py:78-128 and
@@ -248,6 +260,13 @@ int tspmi_batch_gather(const int64_t* desc, const int32_t* item_inst, const int3
 int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
                                int max_n, int32_t* pop, const int32_t* D_all, const int32_t* table_all, int k,
                                int max_iter, int block, int grid, int64_t* stats, void* stream);
+
+/* As tspmi_batch_two_opt_packed; instances with n >= coord_min_n compute EUC_2D distances from
+ * xy_all (descriptor field [2]) instead of reading their matrix (same tours). */
+int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                     int nitems, int max_n, int32_t* pop, const int32_t* D_all, const double* xy_all,
+                                     const int32_t* table_all, int k, int max_iter, int block, int grid,
+                                     int coord_min_n, int64_t* stats, void* stream);
 
 /* First minimum among the first `count` keys of every instance.  Replaces genetic_algorithm.py:708-739. */
 int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best, void* stream);

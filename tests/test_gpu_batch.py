@@ -25,8 +25,9 @@ def _coords(ns, seed=5):
   return [rng.integers(0, 1_000_001, (int(n), 2)).astype(np.float64) for n in ns]
 
 
-@pytest.mark.parametrize("kc", [8, 20])
-def test_batch_matches_single_instance_loops(kc):
+@pytest.mark.parametrize("kc,ns", [(8, [6, 300, 57, 120, 9, 211, 64, 33]), (20, [6, 300, 57, 120, 9, 211, 64, 33]),
+                                   (8, [4200, 300, 57])])  # the last batch mixes coordinate and matrix scans
+def test_batch_matches_single_instance_loops(kc, ns):
   import torch
 
   import tsp_gpu_clean_amd as t
@@ -34,8 +35,7 @@ def test_batch_matches_single_instance_loops(kc):
   from tsp_gpu_clean_amd.engine import DeviceMemeticLoop, draw_generations, upload
   from tsp_gpu_clean_amd.rng import draw_row_seeds
 
-  say(f"batch vs single loops kc={kc}")
-  ns = [6, 300, 57, 120, 9, 211, 64, 33]
+  say(f"batch vs single loops kc={kc} ns={ns}")
   coords = _coords(ns)
   P, G, E = 16, 6, 4
   seeds = [100 + i for i in range(len(ns))]

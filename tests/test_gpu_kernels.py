@@ -439,3 +439,60 @@ def test_two_opt_large_n_lattice_ties_match_c_oracle(torch_mod, nat):
   want, wstats = c_two_opt.improve_batch(tours, d, None, "best", 40)
   np.testing.assert_array_equal(got, want)
   assert list(stats) == list(wstats)
+
+
+# ---- EUC_2D coordinate-mode 2-opt ---------------------------------------------------------------
+
+
+def lattice_coords(n):
+  side = int(np.ceil(np.sqrt(n)))
+  return np.array([[x, y] for x in range(side) for y in range(side)][:n], dtype=np.float64)
+
+
+@pytest.mark.parametrize("n,P,iters,kind", [(5, 6, -1, "uniform"), (40, 8, -1, "float"), (300, 16, -1, "lattice"),
+                                            (1000, 64, -1, "uniform"), (1000, 300, -1, "float"),
+                                            (2100, 300, 15, "uniform"), (5000, 4, 6, "float"), (8194, 2, 3, "uniform")])
+def test_two_opt_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, iters, kind):
+  """Distances computed from coordinates give the matrix kernel's tours and counters exactly."""
+  say(f"two_opt_euc2d n={n} P={P} iters={iters} {kind}")
+  xy = {"uniform": lambda: uniform_coords(n, seed=n), "float": lambda: np.random.default_rng(n).random((n, 2)) * 1000.0,
+        "lattice": lambda: lattice_coords(n)}[kind]()
+  d = compute_distance_matrix(xy)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + 1), P))
+  want, wstats = _run_two_opt(nat, torch_mod, tours, d, None, "best", iters)
+  t = cu(tours)
+  stats = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  nat.two_opt_euc2d(t, cu(xy), iters, stats)
+  np.testing.assert_array_equal(t.cpu().numpy(), want)
+  assert stats.cpu().tolist() == list(wstats)
+  if n <= 1000:
+    ref, _ = c_two_opt.improve_batch(tours[:2], d, None, "best", iters)
+    np.testing.assert_array_equal(want[:2], ref)
+
+
+@pytest.mark.parametrize("n,P,k,iters", [(40, 6, 8, -1), (1000, 24, 20, -1), (1000, 300, 20, -1), (1000, 600, 32, -1),
+                                         (10000, 4, 8, 40)])
+def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iters):
+  say(f"two_opt_packed_euc2d n={n} P={P} k={k}")
+  xy = uniform_coords(n, seed=3 * n)
+  D, _ = device_dist(nat, torch_mod, xy)
+  cand = torch_mod.empty((n, k), dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, k, cand)
+  table = torch_mod.empty((n, nat.candidate_lanes(k), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cand, D, table)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + 2), P))
+  a, b = cu(tours), cu(tours)
+  sa = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  sb = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  nat.two_opt_packed(a, D, table, k, iters, sa)
+  nat.two_opt_packed_euc2d(b, cu(xy), table, k, iters, sb)
+  np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy())
+  assert sb.cpu().tolist() == sa.cpu().tolist()
+
+
+def test_two_opt_euc2d_rejects_bad_input(torch_mod, nat):
+  t = cu(np.arange(9000, dtype=np.int32)[None, :])
+  with pytest.raises(ValueError):
+    nat.two_opt_euc2d(t, cu(uniform_coords(9000)), 1)  # above the full-neighbourhood size limit
+  with pytest.raises(ValueError):
+    nat.two_opt_euc2d(cu(np.arange(10, dtype=np.int32)[None, :]), cu(uniform_coords(11)), 1)

@@ -221,3 +221,22 @@ def test_device_loop_graph_replays_match_eager(t):
     np.testing.assert_array_equal(loop.fitness.cpu().numpy(), want_fit)
     np.testing.assert_array_equal(best.cpu().numpy(), want_best)
     _ = (loop.population.float() * 2).sum().item()  # unrelated eager work between replays
+
+
+def test_local_search_strategy_large_n_computes_distances_from_coordinates(t):
+  """n = 5000 > 4096: the strategy's candidate scan reads coordinates, with the matrix kernel's tours."""
+  import torch
+
+  from tsp_gpu_clean_amd import _native
+
+  say("LocalSearchGPUTourTwoOpt n=5000 k=8 (coordinate candidate scan)")
+  rng = np.random.default_rng(31)
+  xy = rng.integers(0, 1_000_001, (5000, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("u5000", xy), seed=2)
+  routes = np.stack([rng.permutation(5000) for _ in range(4)]).astype(np.int32)
+  ls = t.LocalSearchGPUTourTwoOpt(max_iterations=40, mode="best", candidate_k=8)
+  got = ls.improve_batch(routes, ctx)
+  want = torch.from_numpy(routes).cuda()
+  _native.two_opt_packed(want, ctx.get_gpu_distances(), ctx.get_gpu_candidate_table(8), 8, 40, None)
+  np.testing.assert_array_equal(got, want.cpu().numpy())
+  assert ls.read_stats()[1] > 0

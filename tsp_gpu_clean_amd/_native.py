@@ -52,8 +52,11 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_survival": [_P, _I, _P, _P, _P],
   "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
+  "tspmi_batch_two_opt_packed_mixed": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
   "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
+  "tspmi_two_opt_euc2d": [_P, _I, _I, _P, _I, _P, _P],
+  "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
   "tspmi_scorer_param_count": [],
   "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
   "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
@@ -338,6 +341,35 @@ def nn_init(seeds, customers, cand, work, out) -> None:
     raise ValueError("nn_init: work needs 2 * P * n int32")
   _check(_lib.tspmi_nn_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(cand), k, _ptr(work),
                             _ptr(out), _stream(torch)), "tspmi_nn_init")
+
+
+def two_opt_euc2d(tours, xy, max_iter: int = 10, stats=None) -> None:
+  """Full-neighbourhood best-improvement 2-opt with distances computed from EUC_2D coordinates."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(xy, torch.float64, "xy", 2)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  P, n = tours.shape
+  if xy.shape != (n, 2):
+    raise ValueError(f"two_opt_euc2d: xy {tuple(xy.shape)} != ({n}, 2)")
+  _check(_lib.tspmi_two_opt_euc2d(_ptr(tours), P, n, _ptr(xy), int(max_iter), _ptr(stats), _stream(torch)),
+         "tspmi_two_opt_euc2d")
+
+
+def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None) -> None:
+  """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(xy, torch.float64, "xy", 2)
+  _need(table, torch.int32, "table", 3)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  P, n = tours.shape
+  if xy.shape != (n, 2) or table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+    raise ValueError("two_opt_packed_euc2d: xy / table do not match n / k")
+  _check(_lib.tspmi_two_opt_packed_euc2d(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter), _ptr(stats),
+                                         _stream(torch)), "tspmi_two_opt_packed_euc2d")
 
 
 def nn_greedy_init(starts, D, out) -> None:

@@ -19,10 +19,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
+from .context import coordinate_candidates_pay
 from .rng import draw_row_seeds, make_rng
 
 FIELDS = 9  # TSPMI_BATCH_FIELDS
 KP = 32  # packed candidate row width used by the batch kernels
+COORD_MIN_N = 4097  # instances this large compute candidate-scan distances from coordinates
 
 
 @dataclass
@@ -171,8 +173,12 @@ class BatchedMemeticGA:
                        self.rate, self.off, block, grid)
 
   def _elite_ls(self, pop, block: int, grid: int) -> None:
-    _native.batch_call("tspmi_batch_two_opt_packed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.table,
-                       self.kc, self.ls_iters, block, grid, self.stats)
+    if coordinate_candidates_pay(self.max_n):  # large instances compute distances from coordinates
+      _native.batch_call("tspmi_batch_two_opt_packed_mixed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.xy,
+                         self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
+    else:
+      _native.batch_call("tspmi_batch_two_opt_packed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.table,
+                         self.kc, self.ls_iters, block, grid, self.stats)
 
   def step(self, d: BatchDraws, g: int, best=None) -> None:
     a, b = self.cur, 1 - self.cur

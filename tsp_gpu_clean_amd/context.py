@@ -22,6 +22,14 @@ MAX_DISTANCE = (1 << 29) - 1
 DEFAULT_MAX_MATRIX_DIMENSION = 10_000
 
 
+def coordinate_candidates_pay(n: int) -> bool:
+  """Whether the candidate-list 2-opt should compute EUC_2D distances from coordinates instead of
+  gathering them from the matrix: measured on an MI355X, the matrix form is faster while the
+  n x n int32 matrix stays cache-resident (n = 1000: 63 vs 85 ms per C2 k=20 launch) and the
+  coordinate form once it does not (n = 10000, k = 8, 256 tours, 200 passes: 9.4 vs 35.2 ms)."""
+  return n > 4096
+
+
 class ProblemContext:
   def __init__(
     self,
@@ -70,6 +78,12 @@ class ProblemContext:
       self._gpu_coordinates = torch.from_numpy(xy).to("cuda")
       self.h2d_bytes += int(xy.nbytes)
     return self._gpu_coordinates
+
+  def get_gpu_euc2d_coordinates(self):
+    """Device (n, 2) float64 coordinates when the distances are EUC_2D of them, else None."""
+    if self.problem.edge_type != "EUC_2D" or self.problem.coordinates is None:
+      return None
+    return self.get_gpu_coordinates()
 
   def get_gpu_distances(self):
     """Dense (n, n) int32 matrix on the device, built once (problem_context.py:337-397)."""

@@ -211,8 +211,36 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __r
     const Inst I = load_inst(desc, item_inst[it]);
     const int ke = k < I.n - 1 ? k : I.n - 1;
     if (I.n >= 4 && ke >= 1)
-      cand_two_opt_tour<BLOCK, 32, 4>(pop + I.pop + (size_t)item_idx[it] * I.n, I.n, D_all + I.d, I.n, ct_all + I.ct, ke,
+      cand_two_opt_tour<BLOCK, 32, 4>(pop + I.pop + (size_t)item_idx[it] * I.n, I.n, MatrixDist{D_all + I.d, I.n},
+                                      ct_all + I.ct, ke,
                                       max_iter, stats, lds, red, red64);
+    __syncthreads();
+  }
+}
+
+// As batch_two_opt_kernel, but instances with n >= coord_min_n compute their distances from the
+// EUC_2D coordinates (same tours) instead of gathering them from a matrix that no longer fits the
+// caches.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
+    const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst, const int32_t* __restrict__ item_idx,
+    int nitems, int32_t* __restrict__ pop, const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
+    const int2* __restrict__ ct_all, int k, int max_iter, int coord_min_n, int64_t* __restrict__ stats) {
+  extern __shared__ int32_t lds[];
+  __shared__ uint64_t red[BLOCK / kWave];
+  __shared__ int64_t red64[BLOCK / kWave];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Inst I = load_inst(desc, item_inst[it]);
+    const int ke = k < I.n - 1 ? k : I.n - 1;
+    int32_t* tour = pop + I.pop + (size_t)item_idx[it] * I.n;
+    if (I.n >= 4 && ke >= 1) {
+      if (I.n >= coord_min_n)
+        cand_two_opt_tour<BLOCK, 32, 4>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
+                                        ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+      else
+        cand_two_opt_tour<BLOCK, 32, 4>(tour, I.n, MatrixDist{D_all + I.d, I.n}, ct_all + I.ct, ke, max_iter, stats,
+                                        lds, red, red64);
+    }
     __syncthreads();
   }
 }
@@ -377,5 +405,35 @@ extern "C" int tspmi_batch_best(const int64_t* desc, int B, const double* keys, 
   hipLaunchKernelGGL(batch_best_kernel, dim3((B + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, B, keys, count,
                      best);
   TSPMI_CHECK_LAUNCH("tspmi_batch_best launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
+                                                int nitems, int max_n, int32_t* pop, const int32_t* D_all,
+                                                const double* xy_all, const int32_t* table_all, int k, int max_iter,
+                                                int block, int grid, int coord_min_n, int64_t* stats, void* stream) {
+  TSPMI_CHECK_ARG(desc && nitems >= 0 && max_n >= 0 && k >= 1 && k <= 32, "tspmi_batch_two_opt_packed_mixed: bad args");
+  if (nitems == 0 || max_iter == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(item_inst && item_idx && pop && D_all && xy_all && table_all,
+                  "tspmi_batch_two_opt_packed_mixed: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_two_opt_packed_mixed: xy must be 16-byte aligned");
+  const size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_two_opt_packed_mixed: n=%d exceeds the LDS limit",
+                  max_n);
+  const int g = grid > 0 && grid < nitems ? grid : nitems;
+  hipStream_t s = (hipStream_t)stream;
+  const int2* ct = reinterpret_cast<const int2*>(table_all);
+  if (block == 256)
+    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<256>, dim3(g), dim3(256), lds, s, desc, item_inst, item_idx, nitems,
+                       pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
+  else if (block == 512)
+    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<512>, dim3(g), dim3(512), lds, s, desc, item_inst, item_idx, nitems,
+                       pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
+  else if (block == 1024)
+    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<1024>, dim3(g), dim3(1024), lds, s, desc, item_inst, item_idx,
+                       nitems, pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
+  else
+    TSPMI_CHECK_ARG(false, "tspmi_batch_two_opt_packed_mixed: block %d not in {256, 512, 1024}", block);
+  TSPMI_CHECK_LAUNCH("tspmi_batch_two_opt_packed_mixed launch");
   return TSPMI_OK;
 }

@@ -100,4 +100,17 @@ constexpr uint64_t kNoMove = ~0ull;
 // Node ids outside [0, limit) are an invalid input; clamping keeps every access in bounds.
 __device__ __forceinline__ int clamp_id(int v, int limit) { return (unsigned)v < (unsigned)limit ? v : 0; }
 
+// EUC_2D TSPLIB nint with FP64 kept for the sqrt-and-round step only:
+// d = (int)(sqrt(dx*dx + dy*dy) + 0.5), products and sum rounded separately (no FMA
+// contraction), as NumPy evaluates src/distances/matrix.py:84-86.  Every kernel that needs an
+// EUC_2D distance calls this one function, so a distance computed on the fly is bit-identical to
+// the matrix entry built by tspmi_dist_euc2d.
+__device__ __forceinline__ int euc2d_nint(double xu, double yu, double xv, double yv) {
+#pragma clang fp contract(off)
+  const double dx = xu - xv;
+  const double dy = yu - yv;
+  const double s = dx * dx + dy * dy;
+  return (int)(__builtin_sqrt(s) + 0.5);
+}
+
 }  // namespace tspmi

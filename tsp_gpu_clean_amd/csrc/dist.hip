@@ -3,17 +3,6 @@
 
 namespace tspmi {
 
-// EUC_2D TSPLIB nint with FP64 kept for the sqrt-and-round step only:
-// d = (int)(sqrt(dx*dx + dy*dy) + 0.5), products and sum rounded separately (no FMA
-// contraction), as NumPy evaluates src/distances/matrix.py:84-86.
-__device__ __forceinline__ int euc2d_nint(double xu, double yu, double xv, double yv) {
-#pragma clang fp contract(off)
-  const double dx = xu - xv;
-  const double dy = yu - yv;
-  const double s = dx * dx + dy * dy;
-  return (int)(__builtin_sqrt(s) + 0.5);
-}
-
 constexpr int kDistTile = 256;
 
 __global__ __launch_bounds__(kDistTile) void dist_euc2d_kernel(const double* __restrict__ xy, int n,

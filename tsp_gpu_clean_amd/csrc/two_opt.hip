@@ -493,8 +493,8 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, D, ld, ct, k, max_iter, stats, lds, red,
-                                       red64);
+  cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, MatrixDist{D, ld}, ct, k, max_iter, stats,
+                                       lds, red, red64);
 }
 
 // Packed candidate table: out[u, r] = {N[u, r], D[u, N[u, r]]} for r < k, {-1, 0} for k <= r < KP.

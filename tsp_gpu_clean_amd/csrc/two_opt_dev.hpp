@@ -16,9 +16,26 @@ struct TwoOptShared {
 
 __device__ __forceinline__ int wrap_next(int x, int n) { return (x + 1 == n) ? 0 : x + 1; }
 
+// Distance policies: a row-major int32 matrix, or EUC_2D coordinates with the distance computed
+// on the fly by the same FP64 function that builds the matrix (bit-identical values).
+struct MatrixDist {
+  const int32_t* __restrict__ D;
+  int ld;
+  __device__ __forceinline__ int operator()(int u, int v) const { return D[(size_t)u * ld + v]; }
+};
+
+struct CoordDist {
+  const double2* __restrict__ xy;
+  __device__ __forceinline__ int operator()(int u, int v) const {
+    const double2 a = xy[u], b = xy[v];
+    return euc2d_nint(a.x, a.y, b.x, b.y);
+  }
+};
+
 // Reverse s_tour[i+1 .. j] and refresh edge costs (and positions) of the touched range.
+template <class Dist>
 __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int32_t* s_pos, int i, int j, int n,
-                                           const int32_t* __restrict__ D, int ld) {
+                                           const Dist& dist) {
   const int half = (j - i) >> 1;
   for (int t = threadIdx.x; t < half; t += blockDim.x) {
     const int lo = i + 1 + t, hi = j - t;
@@ -29,10 +46,15 @@ __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int
   __syncthreads();
   for (int x = i + threadIdx.x; x <= j; x += blockDim.x) {
     const int a = s_tour[x];
-    s_edge[x] = D[(size_t)a * ld + s_tour[wrap_next(x, n)]];
+    s_edge[x] = dist(a, s_tour[wrap_next(x, n)]);
     if (s_pos != nullptr && x > i) s_pos[a] = x;
   }
   __syncthreads();
+}
+
+__device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int32_t* s_pos, int i, int j, int n,
+                                           const int32_t* __restrict__ D, int ld) {
+  apply_move(s_tour, s_edge, s_pos, i, j, n, MatrixDist{D, ld});
 }
 
 __device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node) {
@@ -47,10 +69,10 @@ __device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node)
 // division; UNROLL steps are issued together so their loads overlap.  Within a lane the pairs
 // are visited in increasing (i, r) scan order, so a strict `<` keeps the lane's first minimum and
 // the packed (delta, i*k + r) key restores the reference tie-break across lanes.
-template <int BLOCK, int KP, int UNROLL>
-__device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const int32_t* __restrict__ D, int ld,
-                                  const int2* __restrict__ ct, int k, int max_iter, int64_t* __restrict__ stats,
-                                  int32_t* lds, uint64_t* red, int64_t* red64) {
+template <int BLOCK, int KP, int UNROLL, class Dist>
+__device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+                                  int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
+                                  int64_t* red64) {
   constexpr int NW = BLOCK / kWave;
   constexpr int RPW = kWave / KP;  // rows per wave step
   int32_t* s_tour = lds;           // n + 1 (s_tour[n] mirrors s_tour[0])
@@ -69,7 +91,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const int32
   __syncthreads();
   for (int x = tid; x < n; x += BLOCK) {
     const int a = s_tour[x];
-    s_edge[x] = D[(size_t)a * ld + s_tour[x + 1]];
+    s_edge[x] = dist(a, s_tour[x + 1]);
     s_pos[a] = x;
   }
   __syncthreads();
@@ -100,7 +122,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const int32
         j = (r < k && i < rows && j > i + 1 && j < lim) ? j : -1;
         jj[u] = j;
         const int b = s_tour[i < rows ? i + 1 : 0];
-        g[u] = D[(size_t)b * ld + s_tour[j >= 0 ? j + 1 : 0]];
+        g[u] = dist(b, s_tour[j >= 0 ? j + 1 : 0]);
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
@@ -123,7 +145,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const int32
     const int i = (int)(idx / (uint32_t)k);
     const int j = s_pos[ct[(size_t)s_tour[i] * KP + (int)(idx - (uint32_t)i * (uint32_t)k)].x];
     __syncthreads();
-    apply_move(s_tour, s_edge, s_pos, i, j, n, D, ld);
+    apply_move(s_tour, s_edge, s_pos, i, j, n, dist);
     ++moves;
   }
   for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];

@@ -23,6 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
+from .context import coordinate_candidates_pay
 
 TSPMI_MODE_BEST = 0
 
@@ -89,6 +90,8 @@ class DeviceMemeticLoop:
     self.cand_k = 0 if self.cand is None else int(self.cand.shape[1])
     self.table = (context.get_gpu_candidate_table(ls_candidate_k)
                   if self.cand is not None and 1 <= self.cand_k <= 32 else None)
+    # Above a few thousand cities the candidate scan computes distances from the coordinates.
+    self.xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(self.n) else None
     P, n = self.P, self.n
     dev = "cuda"
     self.pop = [torch.empty((P, n), dtype=torch.int32, device=dev) for _ in range(2)]
@@ -133,7 +136,9 @@ class DeviceMemeticLoop:
     if self.elite:
       E = self.elite
       elites = self.pop[b][:E]
-      if self.table is not None:
+      if self.table is not None and self.xy is not None:
+        _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats)
+      elif self.table is not None:
         _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
       else:
         _native.two_opt(elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)

@@ -245,11 +245,13 @@ def greedy_start(D, P: int, seed: int):
   return out
 
 
-def ls_trace(tours0, D, table=None, k: int = 0, chunk: int = 50, limit_s: float = 600.0, on_chunk=None):
+def ls_trace(tours0, D, table=None, k: int = 0, chunk: int = 50, limit_s: float = 600.0, on_chunk=None, xy=None):
   """2-opt (best improvement) to the local optimum in chunks of ``chunk`` iterations.
 
-  ``table`` None runs the full neighbourhood.  Returns (trace, tours) with trace a list of
-  (seconds, best cost, mean cost) after each chunk, starting at (0, start costs).
+  ``table`` None runs the full neighbourhood.  With ``xy`` (EUC_2D coordinates) the candidate
+  scan computes its distances from the coordinates (same tours, faster once the matrix no longer
+  fits the caches).  Returns (trace, tours) with trace a list of (seconds, best cost, mean cost)
+  after each chunk, starting at (0, start costs).
   """
   torch = _native.require_device()
   tours = tours0.clone()
@@ -265,6 +267,8 @@ def ls_trace(tours0, D, table=None, k: int = 0, chunk: int = 50, limit_s: float 
     t0 = time.perf_counter()
     if table is None:
       _native.two_opt(tours, D, None, 0, chunk, stats)
+    elif xy is not None:
+      _native.two_opt_packed_euc2d(tours, xy, table, k, chunk, stats)
     else:
       _native.two_opt_packed(tours, D, table, k, chunk, stats)
     _native.tour_cost(tours, D, cost)
@@ -317,16 +321,22 @@ def candidate_tables(D, xy, scorer, k_learned: int = 8, knn_ks=(8, 20)):
 
 
 def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, gap: float = 0.02, seed: int = 2024,
-                 full_chunk: int = 200, cand_chunk: int = 50, full_limit_s: float = 600.0, progress=None) -> dict:
+                 full_chunk: int = 200, cand_chunk: int = 50, full_limit_s: float = 600.0, progress=None,
+                 coords: bool | None = None) -> dict:
   """Time for each candidate scheme to bring the population's best tour within ``gap`` of the
-  full-neighbourhood 2-opt local optimum reached from the same start tours."""
+  full-neighbourhood 2-opt local optimum reached from the same start tours.
+
+  Every candidate scheme runs the same kernel: the coordinate form of the candidate scan when
+  ``coords`` (default: when the matrix outgrows the caches, n > 4096), else the matrix form."""
+  from .context import coordinate_candidates_pay
   torch = _native.require_device()
   xy = torch.from_numpy(uniform_coords(n, seed)).cuda()
   D = torch.empty((n, n), dtype=torch.int32, device="cuda")
   _native.dist_euc2d(xy, D)
   start = greedy_start(D, P, seed + 1)
   tables = candidate_tables(D, xy, scorer, k_learned)
-  res = {"n": n, "P": P, "gap": gap, "methods": {}}
+  coords = coordinate_candidates_pay(n) if coords is None else bool(coords)
+  res = {"n": n, "P": P, "gap": gap, "candidate_kernel": "coordinates" if coords else "matrix", "methods": {}}
   tick = None
   if progress:
     tick = lambda p, m: progress("full-chunk", {"seconds": round(p[0], 2), "best": p[1], "moves": m})  # noqa: E731
@@ -339,7 +349,7 @@ def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, 
   if progress:
     progress("full", res["full"])
   for name, (tab, k, prep) in tables.items():
-    trace, _ = ls_trace(start, D, tab, k, cand_chunk)
+    trace, _ = ls_trace(start, D, tab, k, cand_chunk, xy=xy if coords else None)
     best = trace[-1][1]
     t_gap = time_to(trace, target, prep)
     res["methods"][name] = {"k": k, "prep_seconds": round(prep, 5), "best": best, "mean": trace[-1][2],

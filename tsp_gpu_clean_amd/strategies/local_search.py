@@ -7,6 +7,7 @@ from typing import Any
 
 from .. import _native
 from ..route_helpers import ensure_routes_batch, is_gpu_array, restore_routes_shape, to_device_int32
+from ..context import coordinate_candidates_pay
 from ..timing import time_strategy
 from .base import StrategyBase
 
@@ -100,7 +101,11 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
       if table is None:
         table = context.get_gpu_candidate_table(int(self.candidate_k))
-      _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
+      xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(n) else None
+      if xy is not None:  # same tours; distances from coordinates once the matrix leaves the caches
+        _native.two_opt_packed_euc2d(tours, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
+      else:
+        _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
     else:
       _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
     self.kernel_launches += 1
