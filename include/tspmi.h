@@ -139,6 +139,24 @@ int tspmi_inversion(int32_t* routes, int P, int n, const double* probs, const in
                     const int32_t* pos2, double rate, void* stream);
 
 /*
+ * Order crossover (OX) of pairs (2q, 2q+1) with shared cuts [lo[q], hi[q]]: child keeps its own
+ * parent's segment and takes the other parent's remaining genes in that parent's order from
+ * position hi + 1 on (wrapping); an odd last row is copied.
+ * Replaces: src/algorithms/strategies/crossover/order_crossover.py:49-176 and
+ *           crossover/kernels/order_crossover.cu.
+ */
+int tspmi_ox(const int32_t* parents, int P, int n, const int32_t* cut_lo, const int32_t* cut_hi, int32_t* out,
+             void* stream);
+
+/*
+ * Swap mutation in place: rows with probs[r] < rate exchange positions pos1[r] and pos2[r].
+ * Replaces: src/algorithms/strategies/mutation/swap_mutation.py:84-105 (uniform positions) and
+ *           mutation/kernels/swap_mutation.cu.
+ */
+int tspmi_swap(int32_t* routes, int P, int n, const double* probs, const int32_t* pos1, const int32_t* pos2,
+               double rate, void* stream);
+
+/*
  * Fused generation: tournament -> PMX -> inversion -> evaluate, one workgroup per child pair.
  * Inputs are the host-drawn buffers of one generation: tour_cand (P, k), cut_lo/cut_hi (P/2),
  * probs/pos1/pos2 (P).  Writes offspring (P, n) and their costs off_fit (P).

@@ -56,6 +56,8 @@ def evolve(
   patience: int | None = None,
   optimal_cost: float | None = None,
   record: bool = False,
+  crossover: str = "pmx",
+  mutation: str = "inversion",
 ) -> GAResult:
   pop = np.array(population, dtype=np.int32, copy=True)
   p, n = pop.shape
@@ -82,8 +84,8 @@ def evolve(
       pop = _ls(pop)
       fit = ops.tour_costs(pop, dist)
     offspring = ops.tournament_select(rng, pop, fit, tournament_k, p)
-    offspring = ops.pmx_batch(rng, offspring)
-    offspring = ops.inversion_batch(rng, offspring, mutation_rate)
+    offspring = (ops.ox_batch if crossover == "ox" else ops.pmx_batch)(rng, offspring)
+    offspring = (ops.swap_batch if mutation == "swap" else ops.inversion_batch)(rng, offspring, mutation_rate)
     off_fit = ops.tour_costs(offspring, dist)
     pop, fit, _ = ops.survivors(pop, fit, offspring, off_fit, p)
     fit = fit.copy()

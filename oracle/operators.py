@@ -156,6 +156,61 @@ def inversion_batch(rng, batch, rate):
   return inversion_apply(batch, probs, p1, p2, rate)
 
 
+def ox_pair(p1: np.ndarray, p2: np.ndarray, c1: int, c2: int) -> np.ndarray:
+  """Order crossover of one child, open permutation, order_crossover.py:49-81 (is_closed False):
+  child[c1..c2] = p1[c1..c2]; the other positions, from c2+1 onward (wrapping), take p2's
+  genes outside that segment in p2's order starting from position c2+1 (wrapping)."""
+  n = len(p1)
+  child = np.full(n, -1, dtype=np.int32)
+  child[c1 : c2 + 1] = p1[c1 : c2 + 1]
+  in_seg = np.zeros(max(n, int(p1.max()) + 1, int(p2.max()) + 1), dtype=bool)
+  in_seg[p1[c1 : c2 + 1]] = True
+  shift = (c2 + 1) % n
+  p2_rot = np.roll(p2, -shift)
+  remaining = p2_rot[~in_seg[p2_rot]]
+  rot_pos = (np.arange(n) + shift) % n
+  empty = child[rot_pos] == -1
+  child[rot_pos[empty]] = remaining
+  return child
+
+
+def ox_apply(batch: np.ndarray, cut_lo: np.ndarray, cut_hi: np.ndarray) -> np.ndarray:
+  """Pairs (2k, 2k+1) share cuts; odd last row copied, order_crossover.py:129-176."""
+  batch = np.asarray(batch, dtype=np.int32)
+  out = batch.copy()
+  for q in range(batch.shape[0] // 2):
+    i = 2 * q
+    out[i] = ox_pair(batch[i], batch[i + 1], int(cut_lo[q]), int(cut_hi[q]))
+    out[i + 1] = ox_pair(batch[i + 1], batch[i], int(cut_lo[q]), int(cut_hi[q]))
+  return out
+
+
+def ox_batch(rng, batch):
+  """CrossoverCPUPermOrder.crossover_batch, order_crossover.py:129-176 (same draws as PMX)."""
+  batch = np.asarray(batch, dtype=np.int32)
+  if batch.shape[0] < 2 or batch.shape[1] < 2:
+    return batch.copy()
+  lo, hi = pmx_draw(rng, batch.shape[0], batch.shape[1])
+  return ox_apply(batch, lo, hi)
+
+
+def swap_apply(batch, probs, pos1, pos2, rate):
+  """Swap positions pos1/pos2 of the rows with probs < rate, swap_mutation.py:84-105."""
+  out = np.array(batch, dtype=np.int32, copy=True)
+  act = np.flatnonzero(probs < rate)
+  out[act, pos1[act]], out[act, pos2[act]] = out[act, pos2[act]], out[act, pos1[act]]
+  return out
+
+
+def swap_batch(rng, batch, rate):
+  """MutationCPUPermSwap.mutate_batch without candidate lists: probs, then the position pairs."""
+  batch = np.asarray(batch, dtype=np.int32)
+  if batch.shape[1] < 2:
+    return batch.copy()
+  probs, p1, p2 = inversion_draw(rng, batch.shape[0], batch.shape[1])
+  return swap_apply(batch, probs, p1, p2, rate)
+
+
 def survivors(old_pop, old_fit, new_pop, new_fit, pop_size):
   """(mu + lambda) survival with all-feasible, zero-violation TSP assessments.
 

@@ -496,3 +496,31 @@ def test_two_opt_euc2d_rejects_bad_input(torch_mod, nat):
     nat.two_opt_euc2d(t, cu(uniform_coords(9000)), 1)  # above the full-neighbourhood size limit
   with pytest.raises(ValueError):
     nat.two_opt_euc2d(cu(np.arange(10, dtype=np.int32)[None, :]), cu(uniform_coords(11)), 1)
+
+
+# ---- order crossover and swap mutation ------------------------------------------------------------
+
+
+@pytest.mark.parametrize("P,n", [(1, 50), (2, 2), (7, 3), (64, 50), (1024, 1000), (6, 5000)])
+def test_ox_matches_oracle(torch_mod, nat, P, n):
+  say(f"ox P={P} n={n}")
+  rng = np.random.default_rng(P * 7 + n)
+  pop = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+  lo, hi = ops.pmx_draw(rng, P, n) if P >= 2 else (np.zeros(1, np.int32), np.zeros(1, np.int32))
+  if P >= 6:  # extreme cuts: whole tour, single gene at both ends
+    lo[:3] = [0, 0, n - 1]
+    hi[:3] = [n - 1, 0, n - 1]
+  out = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  nat.ox(cu(pop), cu(lo), cu(hi), out)
+  np.testing.assert_array_equal(out.cpu().numpy(), ops.ox_apply(pop, lo, hi))
+
+
+@pytest.mark.parametrize("P,n", [(1, 2), (100, 50), (1024, 1000)])
+def test_swap_matches_oracle(torch_mod, nat, P, n):
+  say(f"swap P={P} n={n}")
+  rng = np.random.default_rng(P + n)
+  pop = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+  probs, p1, p2 = ops.inversion_draw(rng, P, n)
+  t = cu(pop)
+  nat.swap(t, cu(probs), cu(p1), cu(p2), 0.3)
+  np.testing.assert_array_equal(t.cpu().numpy(), ops.swap_apply(pop, probs, p1, p2, 0.3))

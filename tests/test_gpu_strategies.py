@@ -240,3 +240,45 @@ def test_local_search_strategy_large_n_computes_distances_from_coordinates(t):
   _native.two_opt_packed(want, ctx.get_gpu_distances(), ctx.get_gpu_candidate_table(8), 8, 40, None)
   np.testing.assert_array_equal(got, want.cpu().numpy())
   assert ls.read_stats()[1] > 0
+
+
+def test_chapter4_operators_match_oracle_with_same_generator(t):
+  """Order crossover and swap mutation strategies: the same draws and children as the oracle."""
+  say("OX / swap strategies")
+  prob = c1_problem(t, 120, seed=9, hi=1_000_000)
+  ctx = t.ProblemContext(prob, seed=321)
+  rng = np.random.default_rng(321)
+  pop = np.stack([np.random.default_rng(200 + i).permutation(120) for i in range(65)]).astype(np.int32)
+  kids = t.CrossoverGPUPermOrder().crossover_batch(pop, ctx)
+  want = ops.ox_batch(rng, pop)
+  np.testing.assert_array_equal(kids, want)
+  mut = t.MutationGPUPermSwap().mutate_batch(kids, 0.4, ctx)
+  np.testing.assert_array_equal(mut, ops.swap_batch(rng, want, 0.4))
+  child = t.CrossoverGPUPermOrder().crossover(pop[0], pop[1], ctx)
+  cuts = ops.sample_distinct_pairs(rng, 1, 0, 120)[0]
+  np.testing.assert_array_equal(child, ops.ox_pair(pop[0], pop[1], min(cuts), max(cuts)))
+  assert ctx.rng.bit_generator.state == rng.bit_generator.state
+
+
+def test_chapter4_ga_end_to_end_matches_oracle(t):
+  """The config-1 memetic pipeline with OX and swap in place of PMX and inversion."""
+  say("GA with OX + swap end to end")
+  prob = c1_problem(t)
+  ctx = t.ProblemContext(prob, seed=42)
+  ls = t.LocalSearchGPUTourTwoOpt(mode="best", max_iterations=10, candidate_k=20)
+  ev = t.EvaluationGPUCost()
+  ga = t.GeneticAlgorithm(
+    population_size=128, mutation_rate=0.1, seed=42,
+    initialization=t.InitializationGPUTourNearestNeighbor(k=5, seed=42),
+    selection=t.SelectionGPUTournament(k=5, seed=42), crossover=t.CrossoverGPUPermOrder(seed=42),
+    mutation=t.MutationGPUPermSwap(seed=42), local_search=ls, evaluation=ev, local_search_stage="none",
+    pipeline=[t.PostInitLocalSearchStep(ls, ev), t.EliteOnlyLocalSearchStep(ls, ev, elite_count=16)])
+  best, stats = ga.evolve(ctx, list(range(52)), 30)
+  rng = np.random.default_rng(42)
+  pop0 = oinit.nn_init(np.arange(52), 128, ctx.get_cpu_candidate_list(5), oinit.draw_row_seeds(rng, 128))
+  res = oga.evolve(rng, ctx.get_cpu_distances(), pop0, max_generations=30, mutation_rate=0.1, tournament_k=5,
+                   local_search=oga.LocalSearchSpec("best", 10, ctx.get_cpu_candidate_list(20)), post_init=True,
+                   elite_count=16, crossover="ox", mutation="swap")
+  assert stats["convergence_history"] == res.stats["convergence_history"]
+  assert stats["stop_reason"] == res.stats["stop_reason"]
+  np.testing.assert_array_equal(best, res.best_tour)

@@ -125,3 +125,33 @@ def test_greedy_nn_hand_worked():
   x = np.array([-1, 0, 1])
   D = np.abs(x[:, None] - x[None, :]).astype(np.int32)
   assert greedy_nn(D, [1]).tolist() == [[1, 0, 2]]
+
+
+def test_ox_hand_worked_and_invariants():
+  p1 = np.arange(9, dtype=np.int32)
+  p2 = np.array([8, 2, 6, 7, 1, 5, 4, 0, 3], dtype=np.int32)
+  # Davis' order crossover: segment [3, 5] from p1, the rest in p2's order from position 6 on
+  assert ops.ox_pair(p1, p2, 3, 5).tolist() == [6, 7, 1, 3, 4, 5, 0, 8, 2]
+  assert ops.ox_pair(p1, p2, 0, 8).tolist() == p1.tolist()  # whole segment: copy of p1
+  assert ops.ox_pair(p1, p2, 4, 4).tolist() == [2, 6, 7, 1, 4, 5, 0, 3, 8]
+  rng = np.random.default_rng(3)
+  pop = np.stack([rng.permutation(50) for _ in range(7)]).astype(np.int32)
+  out = ops.ox_batch(np.random.default_rng(1), pop)
+  assert (np.sort(out, axis=1) == np.arange(50)).all()
+  np.testing.assert_array_equal(out[6], pop[6])  # odd last row copied
+  lo, hi = ops.pmx_draw(np.random.default_rng(1), 7, 50)
+  for q in range(3):
+    c1, c2 = lo[q], hi[q]
+    np.testing.assert_array_equal(out[2 * q, c1 : c2 + 1], pop[2 * q, c1 : c2 + 1])
+    np.testing.assert_array_equal(out[2 * q + 1, c1 : c2 + 1], pop[2 * q + 1, c1 : c2 + 1])
+    # outside the segment the child keeps the other parent's relative order
+    seg = set(pop[2 * q, c1 : c2 + 1].tolist())
+    rest = [g for g in np.roll(pop[2 * q + 1], -(c2 + 1)) if g not in seg]
+    got = [out[2 * q, (c2 + 1 + m) % 50] for m in range(len(rest))]
+    assert got == rest
+
+
+def test_swap_hand_worked():
+  pop = np.tile(np.arange(6, dtype=np.int32), (3, 1))
+  out = ops.swap_apply(pop, np.array([0.01, 0.9, 0.02]), np.array([0, 1, 5]), np.array([4, 2, 2]), 0.05)
+  assert out.tolist() == [[4, 1, 2, 3, 0, 5], [0, 1, 2, 3, 4, 5], [0, 1, 5, 3, 4, 2]]

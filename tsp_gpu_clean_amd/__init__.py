@@ -18,6 +18,7 @@ from .pipeline_steps import (
 from .problem import Problem, load_tsplib_file
 from .strategies import (
   STRATEGY_REGISTRY,
+  CrossoverGPUPermOrder,
   CrossoverGPUPermPartiallyMapped,
   EvaluationGPUCost,
   InitializationGPUPermRandom,
@@ -25,6 +26,7 @@ from .strategies import (
   LocalSearchGPUTourTwoOpt,
   LocalSearchNoOp,
   MutationGPUPermInversion,
+  MutationGPUPermSwap,
   SelectionGPUTournament,
   StrategyBase,
   register_strategy,
@@ -35,6 +37,7 @@ __all__ = [
   "STAGE_SPECS",
   "STRATEGY_REGISTRY",
   "CrossoverGPUPermPartiallyMapped",
+  "CrossoverGPUPermOrder",
   "EliteOnlyLocalSearchStep",
   "EvaluationGPUCost",
   "GPUUnavailableError",
@@ -45,6 +48,7 @@ __all__ = [
   "LocalSearchGPUTourTwoOpt",
   "LocalSearchNoOp",
   "MutationGPUPermInversion",
+  "MutationGPUPermSwap",
   "PeriodicLocalSearchStep",
   "PostInitLocalSearchStep",
   "Problem",

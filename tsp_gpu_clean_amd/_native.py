@@ -55,6 +55,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_two_opt_packed_mixed": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
   "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
+  "tspmi_ox": [_P, _I, _I, _P, _P, _P, _P],
+  "tspmi_swap": [_P, _I, _I, _P, _P, _P, _D, _P],
   "tspmi_two_opt_euc2d": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
   "tspmi_scorer_param_count": [],
@@ -275,6 +277,34 @@ def inversion(routes, probs, pos1, pos2, rate: float) -> None:
     raise ValueError("inversion: need P probabilities and positions")
   _check(_lib.tspmi_inversion(_ptr(routes), P, n, _ptr(probs), _ptr(pos1), _ptr(pos2), float(rate), _stream(torch)),
          "tspmi_inversion")
+
+
+def ox(parents, cut_lo, cut_hi, out) -> None:
+  torch = require_device()
+  _need(parents, torch.int32, "parents", 2)
+  _need(out, torch.int32, "out", 2)
+  P, n = parents.shape
+  if out.shape != parents.shape:
+    raise ValueError("ox: out shape != parents shape")
+  if P >= 2:
+    _need(cut_lo, torch.int32, "cut_lo", 1)
+    _need(cut_hi, torch.int32, "cut_hi", 1)
+    if cut_lo.shape[0] < P // 2 or cut_hi.shape[0] < P // 2:
+      raise ValueError("ox: need P // 2 cut pairs")
+  _check(_lib.tspmi_ox(_ptr(parents), P, n, _ptr(cut_lo), _ptr(cut_hi), _ptr(out), _stream(torch)), "tspmi_ox")
+
+
+def swap(routes, probs, pos1, pos2, rate: float) -> None:
+  torch = require_device()
+  _need(routes, torch.int32, "routes", 2)
+  _need(probs, torch.float64, "probs", 1)
+  _need(pos1, torch.int32, "pos1", 1)
+  _need(pos2, torch.int32, "pos2", 1)
+  P, n = routes.shape
+  if min(probs.shape[0], pos1.shape[0], pos2.shape[0]) < P:
+    raise ValueError("swap: need P probabilities and positions")
+  _check(_lib.tspmi_swap(_ptr(routes), P, n, _ptr(probs), _ptr(pos1), _ptr(pos2), float(rate), _stream(torch)),
+         "tspmi_swap")
 
 
 def generation(pop, fitness, D, tour_cand, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring, off_fit) -> None:

@@ -158,6 +158,76 @@ __global__ __launch_bounds__(kRowBlock) void scatter_rows_kernel(const int32_t* 
   if (dst_fit != nullptr && src_fit != nullptr && lane == 0) dst_fit[d] = src_fit[r];
 }
 
+// Order crossover (OX) of one child per wave: child r of pair q = r / 2 keeps parent r's segment
+// [lo, hi] and takes parent r ^ 1's other genes, in that parent's order starting at position
+// hi + 1 (wrapping), into positions hi + 1, hi + 2, ... (wrapping) -- order_crossover.py:49-81
+// with is_closed false.  The segment's genes are marked in an LDS bitmap; the fill is a wave-wide
+// stream compaction (ballot + popcount) over 64 genes at a time.  An odd last row is copied.
+constexpr int kOxBlock = 64;
+
+__global__ __launch_bounds__(kOxBlock) void ox_kernel(const int32_t* __restrict__ parents, int P, int n,
+                                                      const int32_t* __restrict__ cut_lo,
+                                                      const int32_t* __restrict__ cut_hi, int32_t* __restrict__ out) {
+  extern __shared__ uint32_t seg_bits[];
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t* p1 = parents + (size_t)r * n;
+  int32_t* child = out + (size_t)r * n;
+  if ((P & 1) && r == P - 1) {
+    for (int x = lane; x < n; x += kOxBlock) child[x] = p1[x];
+    return;
+  }
+  const int32_t* p2 = parents + (size_t)(r ^ 1) * n;
+  const int q = r >> 1;
+  int lo = cut_lo[q], hi = cut_hi[q];
+  lo = lo < 0 ? 0 : (lo >= n ? n - 1 : lo);
+  hi = hi < lo ? lo : (hi >= n ? n - 1 : hi);
+  const int words = (n + 31) >> 5;
+  for (int w = lane; w < words; w += kOxBlock) seg_bits[w] = 0u;
+  __syncthreads();
+  for (int x = lo + lane; x <= hi; x += kOxBlock) {
+    const int g = clamp_id(p1[x], n);
+    child[x] = g;
+    atomicOr(&seg_bits[g >> 5], 1u << (g & 31));
+  }
+  __syncthreads();
+  const int shift = hi + 1 == n ? 0 : hi + 1;
+  int filled = 0;
+  for (int t0 = 0; t0 < n; t0 += kOxBlock) {
+    const int t = t0 + lane;
+    int g = 0;
+    bool keep = false;
+    if (t < n) {
+      const int src = shift + t < n ? shift + t : shift + t - n;
+      g = clamp_id(p2[src], n);
+      keep = !((seg_bits[g >> 5] >> (g & 31)) & 1u);
+    }
+    const uint64_t mask = __ballot(keep);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    if (keep) {
+      int dst = hi + 1 + filled + rank;
+      dst = dst >= n ? dst - n : dst;
+      child[dst] = g;
+    }
+    filled += __popcll(mask);
+  }
+}
+
+// Swap mutation in place: rows with probs[r] < rate exchange positions pos1[r] and pos2[r]
+// (swap_mutation.py:84-105).  One thread per row.
+__global__ __launch_bounds__(256) void swap_kernel(int32_t* __restrict__ routes, int P, int n,
+                                                   const double* __restrict__ probs, const int32_t* __restrict__ pos1,
+                                                   const int32_t* __restrict__ pos2, double rate) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P || !(probs[r] < rate)) return;
+  const int a = pos1[r], b = pos2[r];
+  if ((unsigned)a >= (unsigned)n || (unsigned)b >= (unsigned)n) return;
+  int32_t* t = routes + (size_t)r * n;
+  const int32_t va = t[a];
+  t[a] = t[b];
+  t[b] = va;
+}
+
 inline int row_blocks(int rows) { return (rows + kRowBlock / kWave - 1) / (kRowBlock / kWave); }
 
 }  // namespace tspmi
@@ -252,5 +322,28 @@ extern "C" int tspmi_scatter_rows(const int32_t* src, const double* src_fit, con
   hipLaunchKernelGGL(scatter_rows_kernel, dim3(row_blocks(rows)), dim3(kRowBlock), 0, (hipStream_t)stream, src, src_fit,
                      idx, rows, n, dst, dst_fit);
   TSPMI_CHECK_LAUNCH("tspmi_scatter_rows launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_ox(const int32_t* parents, int P, int n, const int32_t* cut_lo, const int32_t* cut_hi,
+                        int32_t* out, void* stream) {
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_ox: negative shape");
+  if (P == 0 || n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(parents && out && (P < 2 || (cut_lo && cut_hi)), "tspmi_ox: null pointer");
+  const size_t lds = (size_t)((n + 31) / 32) * sizeof(uint32_t);
+  TSPMI_CHECK_ARG(lds <= (size_t)kMaxLds, "tspmi_ox: n=%d exceeds the LDS bitmap", n);
+  hipLaunchKernelGGL(ox_kernel, dim3(P), dim3(kOxBlock), lds, (hipStream_t)stream, parents, P, n, cut_lo, cut_hi, out);
+  TSPMI_CHECK_LAUNCH("tspmi_ox launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_swap(int32_t* routes, int P, int n, const double* probs, const int32_t* pos1, const int32_t* pos2,
+                          double rate, void* stream) {
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_swap: negative shape");
+  if (P == 0 || n < 2) return TSPMI_OK;
+  TSPMI_CHECK_ARG(routes && probs && pos1 && pos2, "tspmi_swap: null pointer");
+  hipLaunchKernelGGL(swap_kernel, dim3((P + 255) / 256), dim3(256), 0, (hipStream_t)stream, routes, P, n, probs, pos1,
+                     pos2, rate);
+  TSPMI_CHECK_LAUNCH("tspmi_swap launch");
   return TSPMI_OK;
 }

@@ -16,21 +16,23 @@ from ..protocols import (
   SelectionStrategy,
 )
 from .base import StrategyBase
-from .crossover import CrossoverGPUPermPartiallyMapped
+from .crossover import CrossoverGPUPermOrder, CrossoverGPUPermPartiallyMapped
 from .evaluation import EvaluationGPUCost
 from .initialization import InitializationGPUPermRandom, InitializationGPUTourNearestNeighbor
 from .local_search import LocalSearchGPUTourTwoOpt, LocalSearchNoOp
-from .mutation import MutationGPUPermInversion
+from .mutation import MutationGPUPermInversion, MutationGPUPermSwap
 from .selection import SelectionGPUTournament
 
 STRATEGY_REGISTRY: dict[str, type] = {
   "CrossoverGPUPermPartiallyMapped": CrossoverGPUPermPartiallyMapped,
+  "CrossoverGPUPermOrder": CrossoverGPUPermOrder,
   "EvaluationGPUCost": EvaluationGPUCost,
   "InitializationGPUPermRandom": InitializationGPUPermRandom,
   "InitializationGPUTourNearestNeighbor": InitializationGPUTourNearestNeighbor,
   "LocalSearchGPUTourTwoOpt": LocalSearchGPUTourTwoOpt,
   "LocalSearchNoOp": LocalSearchNoOp,
   "MutationGPUPermInversion": MutationGPUPermInversion,
+  "MutationGPUPermSwap": MutationGPUPermSwap,
   "SelectionGPUTournament": SelectionGPUTournament,
 }
 
@@ -68,12 +70,14 @@ def register_strategy(name: str, strategy_cls: type, *, overwrite: bool = False)
 __all__ = [
   "STRATEGY_REGISTRY",
   "CrossoverGPUPermPartiallyMapped",
+  "CrossoverGPUPermOrder",
   "EvaluationGPUCost",
   "InitializationGPUPermRandom",
   "InitializationGPUTourNearestNeighbor",
   "LocalSearchGPUTourTwoOpt",
   "LocalSearchNoOp",
   "MutationGPUPermInversion",
+  "MutationGPUPermSwap",
   "SelectionGPUTournament",
   "StrategyBase",
   "register_strategy",
