@@ -48,6 +48,14 @@ int tspmi_device_count(int* count);
 int tspmi_dist_euc2d(const double* xy, int n, int32_t* D, int ld, int32_t* max_out, void* stream);
 
 /*
+ * Distance matrix of another TSPLIB 2D metric: metric 0 EUC_2D, 1 CEIL_2D, 2 MAN_2D, 3 MAX_2D,
+ * 4 ATT, 5 GEO (xy as DDD.MM latitude/longitude), each with the operations of the NumPy
+ * expression it replaces; diagonal 0; max_out as for tspmi_dist_euc2d.  n <= 65535.
+ * Replaces: src/distances/matrix.py:77-131 and pairwise.py:62-102 (2D metrics).
+ */
+int tspmi_dist_metric(const double* xy, int n, int metric, int32_t* D, int ld, int32_t* max_out, void* stream);
+
+/*
  * k-nearest-neighbour candidate table: out[u, r] = r-th nearest v != u in (D[u,v], v)
  * order, k_eff = min(k, n-1) columns.  `ties_out` (device int32, may be NULL) receives the
  * number of rows whose k-th and (k+1)-th distances tie (where NumPy's argpartition choice

@@ -171,3 +171,41 @@ def test_tsplib_reader(berlin52_coords, tmp_path):
   p = t.load_tsplib_file(str(path))
   assert p.dimension == 52 and p.name == "b52"
   assert np.array_equal(p.coordinates, berlin52_coords)
+
+
+def test_tsplib_reader_reference_fixtures(berlin52_coords):
+  """The reference's fixture instances: GEO, ATT, EUC_2D coordinates and two explicit formats."""
+  import tsp_gpu_clean_amd as t
+  from tests.conftest import GOLDEN
+
+  kinds = {"burma14": ("GEO", 14), "att48": ("ATT", 48), "berlin52": ("EUC_2D", 52), "gr17": ("EXPLICIT", 17),
+           "bayg29": ("EXPLICIT", 29)}
+  for name, (edge, n) in kinds.items():
+    p = t.load_tsplib_file(os.path.join(GOLDEN, f"{name}.tsp"))
+    assert (p.edge_type, p.dimension, p.name) == (edge, n, name)
+    if edge == "EXPLICIT":
+      d = p.distances
+      assert d.shape == (n, n) and np.array_equal(d, d.T) and (np.diag(d) == 0).all() and (d[~np.eye(n, dtype=bool)] > 0).all()
+  np.testing.assert_array_equal(t.load_tsplib_file(os.path.join(GOLDEN, "berlin52.tsp")).coordinates, berlin52_coords)
+  gr17 = t.load_tsplib_file(os.path.join(GOLDEN, "gr17.tsp")).distances  # LOWER_DIAG_ROW: "0 / 633 0 / 257 390 0 / 91 661 228 0"
+  assert gr17[1, 0] == 633 and gr17[2, :2].tolist() == [257, 390] and gr17[3, :3].tolist() == [91, 661, 228]
+  bayg = t.load_tsplib_file(os.path.join(GOLDEN, "bayg29.tsp")).distances  # UPPER_ROW: "97 205 139 ..." then "129 103 ..."
+  assert bayg[0, 1:4].tolist() == [97, 205, 139] and bayg[1, 2:4].tolist() == [129, 103]
+
+
+def test_explicit_matrix_formats_agree():
+  from tsp_gpu_clean_amd.problem import explicit_matrix
+
+  rng = np.random.default_rng(4)
+  n = 7
+  a = rng.integers(1, 100, (n, n))
+  full = np.triu(a, 1) + np.triu(a, 1).T
+  iu, il = np.triu_indices(n, 1), np.tril_indices(n, -1)
+  iud, ild = np.triu_indices(n, 0), np.tril_indices(n, 0)
+  cases = {"FULL_MATRIX": full.ravel(), "UPPER_ROW": full[iu], "LOWER_ROW": full[il], "UPPER_DIAG_ROW": full[iud],
+           "LOWER_DIAG_ROW": full[ild], "UPPER_COL": full.T[il], "LOWER_COL": full.T[iu],
+           "UPPER_DIAG_COL": full.T[ild], "LOWER_DIAG_COL": full.T[iud]}
+  for fmt, vals in cases.items():
+    np.testing.assert_array_equal(explicit_matrix(vals, n, fmt), full, err_msg=fmt)
+  with pytest.raises(ValueError):
+    explicit_matrix(full[iu][:-1], n, "UPPER_ROW")

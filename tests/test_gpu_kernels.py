@@ -524,3 +524,28 @@ def test_swap_matches_oracle(torch_mod, nat, P, n):
   t = cu(pop)
   nat.swap(t, cu(probs), cu(p1), cu(p2), 0.3)
   np.testing.assert_array_equal(t.cpu().numpy(), ops.swap_apply(pop, probs, p1, p2, 0.3))
+
+
+# ---- TSPLIB metrics ------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("metric", ["EUC_2D", "CEIL_2D", "MAN_2D", "MAX_2D", "ATT", "GEO"])
+def test_dist_metric_matches_oracle(torch_mod, nat, metric):
+  say(f"dist_metric {metric}")
+  rng = np.random.default_rng(11)
+  if metric == "GEO":  # DDD.MM latitudes / longitudes
+    deg = np.stack([rng.integers(-89, 89, 700), rng.integers(-179, 179, 700)], axis=1).astype(np.float64)
+    xy = deg + np.sign(deg + 0.5) * rng.integers(0, 60, (700, 2)) / 100.0
+  else:
+    xy = np.concatenate([rng.random((400, 2)) * 1000.0, rng.integers(-5000, 5000, (300, 2)).astype(np.float64)])
+  want = compute_distance_matrix(xy, metric)
+  D = torch_mod.empty((xy.shape[0], xy.shape[0]), dtype=torch_mod.int32, device="cuda")
+  mx = torch_mod.zeros(1, dtype=torch_mod.int32, device="cuda")
+  nat.dist_metric(cu(xy), metric, D, mx)
+  np.testing.assert_array_equal(D.cpu().numpy(), want)
+  assert int(mx.item()) == int(want.max())
+  pts = np.array([[16.47, 96.10], [16.47, 94.44], [20.09, 92.54]]) if metric == "GEO" else np.array(
+    [[0.0, 0.0], [3.1, 4.0], [1.5, 2.5]])
+  small = torch_mod.empty((3, 3), dtype=torch_mod.int32, device="cuda")
+  nat.dist_metric(cu(pts), metric, small)
+  np.testing.assert_array_equal(small.cpu().numpy(), compute_distance_matrix(pts, metric))

@@ -282,3 +282,38 @@ def test_chapter4_ga_end_to_end_matches_oracle(t):
   assert stats["convergence_history"] == res.stats["convergence_history"]
   assert stats["stop_reason"] == res.stats["stop_reason"]
   np.testing.assert_array_equal(best, res.best_tour)
+
+
+@pytest.mark.parametrize("name", ["burma14", "gr17", "bayg29", "att48", "berlin52"])
+def test_reference_fixture_instances(t, name):
+  """The reference's fixture instances (GEO, EXPLICIT lower/upper rows, ATT, EUC_2D) run through the
+  memetic pipeline: device distances equal the oracle's, and the GA reaches the published optimum
+  (problems.json) on the small ones and never goes below it."""
+  import json
+  import os
+
+  from oracle.distances import compute_distance_matrix
+  from tests.conftest import GOLDEN
+
+  say(f"fixture {name}")
+  with open(os.path.join(GOLDEN, "problems.json"), encoding="utf-8") as fh:
+    optimum = {p["name"]: p["optimal"] for p in json.load(fh)["problems"]}[name]
+  prob = t.load_tsplib_file(os.path.join(GOLDEN, f"{name}.tsp"))
+  ctx = t.ProblemContext(prob, seed=7)
+  want = prob.distances if prob.edge_type == "EXPLICIT" else compute_distance_matrix(prob.coordinates, prob.edge_type)
+  np.testing.assert_array_equal(ctx.get_cpu_distances(), want)
+  ls = t.LocalSearchGPUTourTwoOpt(mode="best", max_iterations=50)
+  ev = t.EvaluationGPUCost()
+  ga = t.GeneticAlgorithm(
+    population_size=128, mutation_rate=0.2, seed=7,
+    initialization=t.InitializationGPUPermRandom(seed=7), selection=t.SelectionGPUTournament(k=5, seed=7),
+    crossover=t.CrossoverGPUPermOrder(seed=7), mutation=t.MutationGPUPermInversion(seed=7), local_search=ls,
+    evaluation=ev, local_search_stage="none",
+    pipeline=[t.PostInitLocalSearchStep(ls, ev), t.EliteOnlyLocalSearchStep(ls, ev, elite_count=32)])
+  best, stats = ga.evolve(ctx, list(range(prob.dimension)), 150, patience=60)
+  cost = ops.tour_costs(best[None], want)[0]
+  assert cost == stats["best_fitness"] and cost >= optimum
+  if prob.dimension <= 29:
+    assert cost == optimum, (name, cost, optimum)
+  else:
+    assert cost <= 1.03 * optimum, (name, cost, optimum)

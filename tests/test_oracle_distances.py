@@ -2,9 +2,13 @@
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
+import pytest
 
 from oracle.distances import candidate_boundary_ties, compute_candidate_list, compute_distance_matrix
+from tests.conftest import GOLDEN
 
 
 def test_three_point_euc2d_known_answer():
@@ -59,3 +63,45 @@ def test_candidate_list_clamps_and_empty():
   d = compute_distance_matrix(np.array([[0.0, 0.0], [1.0, 0.0], [5.0, 0.0]]))
   assert compute_candidate_list(d, 20).tolist() == [[1, 2], [0, 2], [1, 0]]
   assert compute_candidate_list(d[:1, :1], 3).shape == (1, 0)
+
+
+# Known answers held by the reference's distance tests (test_unit_distance_matrix.py:93-125, 273-292):
+# three hand-checked points per metric, and burma14's published GEO distances.
+CARTESIAN_POINTS = np.array([[0.0, 0.0], [3.1, 4.0], [1.5, 2.5]])
+GEO_POINTS = np.array([[16.47, 96.10], [16.47, 94.44], [20.09, 92.54]])
+
+
+@pytest.mark.parametrize("metric,expected", [
+  ("EUC_2D", [[0, 5, 3], [5, 0, 2], [3, 2, 0]]),
+  ("CEIL_2D", [[0, 6, 3], [6, 0, 3], [3, 3, 0]]),
+  ("MAN_2D", [[0, 7, 4], [7, 0, 3], [4, 3, 0]]),
+  ("MAX_2D", [[0, 4, 3], [4, 0, 2], [3, 2, 0]]),
+  ("ATT", [[0, 2, 1], [2, 0, 1], [1, 1, 0]]),
+  ("GEO", [[0, 153, 510], [153, 0, 422], [510, 422, 0]]),
+])
+def test_metric_known_answers(metric, expected):
+  pts = GEO_POINTS if metric == "GEO" else CARTESIAN_POINTS
+  assert compute_distance_matrix(pts, metric).tolist() == expected
+
+
+def test_metric_edge_cases():
+  same = np.array([[1.5, 2.5], [1.5, 2.5]])
+  for m in ("EUC_2D", "CEIL_2D", "MAN_2D", "MAX_2D", "ATT"):
+    assert compute_distance_matrix(same, m).tolist() == [[0, 0], [0, 0]]
+  assert compute_distance_matrix(np.array([[16.47, 96.10], [16.47, 96.10]]), "GEO").tolist() == [[0, 1], [1, 0]]
+  for m in ("EUC_2D", "CEIL_2D", "MAN_2D", "MAX_2D", "ATT"):  # translation invariance
+    a = compute_distance_matrix(np.array([[0.0, 0.0], [3.0, 4.0]]), m)
+    b = compute_distance_matrix(np.array([[-7.0, -11.0], [-4.0, -7.0]]), m)
+    assert a.tolist() == b.tolist()
+  with pytest.raises(ValueError):
+    compute_distance_matrix(CARTESIAN_POINTS, "EXPLICIT")
+
+
+def test_burma14_geo_distances():
+  from tsp_gpu_clean_amd.problem import load_tsplib_file
+
+  prob = load_tsplib_file(os.path.join(GOLDEN, "burma14.tsp"))
+  assert prob.edge_type == "GEO" and prob.dimension == 14
+  d = compute_distance_matrix(prob.coordinates, "GEO")
+  assert d[0, :3].tolist() == [0, 153, 510] and d[1, 2] == 422
+  assert np.array_equal(d, d.T) and (np.diag(d) == 0).all()

@@ -56,6 +56,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
   "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_ox": [_P, _I, _I, _P, _P, _P, _P],
+  "tspmi_dist_metric": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_swap": [_P, _I, _I, _P, _P, _P, _D, _P],
   "tspmi_two_opt_euc2d": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
@@ -165,6 +166,25 @@ def dist_euc2d(xy, D, max_out=None) -> None:
   if max_out is not None:
     _need(max_out, torch.int32, "max_out")
   _check(_lib.tspmi_dist_euc2d(_ptr(xy), n, _ptr(D), D.shape[1], _ptr(max_out), _stream(torch)), "tspmi_dist_euc2d")
+
+
+METRICS = {"EUC_2D": 0, "CEIL_2D": 1, "MAN_2D": 2, "MAX_2D": 3, "ATT": 4, "GEO": 5}
+
+
+def dist_metric(xy, metric: str, D, max_out=None) -> None:
+  """Dense int32 matrix of a TSPLIB 2D metric (EUC_2D, CEIL_2D, MAN_2D, MAX_2D, ATT, GEO)."""
+  torch = require_device()
+  _need(xy, torch.float64, "xy", 2)
+  _need(D, torch.int32, "D", 2)
+  if metric not in METRICS:
+    raise ValueError(f"unsupported metric {metric!r}; expected one of {sorted(METRICS)}")
+  n = xy.shape[0]
+  if xy.shape[1] != 2 or D.shape[0] < n or D.shape[1] < n:
+    raise ValueError(f"dist_metric: xy {tuple(xy.shape)} vs D {tuple(D.shape)}")
+  if max_out is not None:
+    _need(max_out, torch.int32, "max_out")
+  _check(_lib.tspmi_dist_metric(_ptr(xy), n, METRICS[metric], _ptr(D), D.shape[1], _ptr(max_out), _stream(torch)),
+         "tspmi_dist_metric")
 
 
 def knn(D, k: int, out, ties_out=None) -> None:

@@ -103,7 +103,10 @@ class ProblemContext:
         xy = self.get_gpu_coordinates()
         d = torch.empty((n, n), dtype=torch.int32, device="cuda")
         mx_t = torch.zeros(1, dtype=torch.int32, device="cuda")
-        _native.dist_euc2d(xy, d, mx_t)
+        if self.problem.edge_type == "EUC_2D":
+          _native.dist_euc2d(xy, d, mx_t)
+        else:
+          _native.dist_metric(xy, self.problem.edge_type, d, mx_t)
         mx = int(mx_t.item())
       if mx > MAX_DISTANCE:
         raise ValueError(f"max distance {mx} exceeds {MAX_DISTANCE}: int32 2-opt deltas would not be exact")

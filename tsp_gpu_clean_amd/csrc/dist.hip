@@ -32,6 +32,73 @@ __global__ __launch_bounds__(kDistTile) void dist_euc2d_kernel(const double* __r
   }
 }
 
+// Other TSPLIB 2D metrics (src/distances/matrix.py:77-131, pairwise.py:62-102), each evaluated
+// with the operations and order of the NumPy expression (no FMA contraction).
+enum Metric : int { kEuc2d = 0, kCeil2d = 1, kMan2d = 2, kMax2d = 3, kAtt = 4, kGeo = 5 };
+
+__device__ __forceinline__ double geo_radians(double c) {
+#pragma clang fp contract(off)
+  const double deg = __builtin_trunc(c);
+  return 3.141592 * (deg + 5.0 * (c - deg) / 3.0) / 180.0;
+}
+
+template <int METRIC>
+__device__ __forceinline__ int metric_distance(double xu, double yu, double xv, double yv) {
+#pragma clang fp contract(off)
+  if constexpr (METRIC == kEuc2d) {
+    return euc2d_nint(xu, yu, xv, yv);
+  } else if constexpr (METRIC == kCeil2d) {
+    const double dx = xu - xv, dy = yu - yv;
+    return (int)__builtin_ceil(__builtin_sqrt(dx * dx + dy * dy));
+  } else if constexpr (METRIC == kMan2d) {
+    return (int)(__builtin_fabs(xu - xv) + __builtin_fabs(yu - yv) + 0.5);
+  } else if constexpr (METRIC == kMax2d) {
+    return (int)(__builtin_fmax(__builtin_fabs(xu - xv), __builtin_fabs(yu - yv)) + 0.5);
+  } else if constexpr (METRIC == kAtt) {
+    const double dx = xu - xv, dy = yu - yv;
+    const double r = __builtin_sqrt((dx * dx + dy * dy) / 10.0);
+    const int t = (int)(r + 0.5);
+    return (double)t < r ? t + 1 : t;
+  } else {
+    const double lat_i = geo_radians(xu), lon_i = geo_radians(yu);
+    const double lat_j = geo_radians(xv), lon_j = geo_radians(yv);
+    const double q1 = cos(lon_i - lon_j);
+    const double q2 = cos(lat_i - lat_j);
+    const double q3 = cos(lat_i + lat_j);
+    double arg = 0.5 * ((1.0 + q1) * q2 - (1.0 - q1) * q3);
+    arg = arg < -1.0 ? -1.0 : (arg > 1.0 ? 1.0 : arg);
+    return (int)(6378.388 * acos(arg) + 1.0);
+  }
+}
+
+template <int METRIC>
+__global__ __launch_bounds__(kDistTile) void dist_metric_kernel(const double* __restrict__ xy, int n,
+                                                                int32_t* __restrict__ D, int ld,
+                                                                int32_t* __restrict__ max_out) {
+  __shared__ int s_max[kDistTile / kWave];
+  const int u = blockIdx.y;
+  const int v = blockIdx.x * kDistTile + threadIdx.x;
+  int d = 0;
+  if (v < n) {
+    d = (u == v) ? 0 : metric_distance<METRIC>(xy[2 * u], xy[2 * u + 1], xy[2 * v], xy[2 * v + 1]);
+    D[(size_t)u * ld + v] = d;
+  }
+  if (max_out != nullptr) {
+    int m = d;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+      const int o = __shfl_xor(m, s, kWave);
+      m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_max[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kDistTile / kWave; ++w) m = s_max[w] > m ? s_max[w] : m;
+      atomicMax(max_out, m);
+    }
+  }
+}
+
 // One workgroup per row u: k rounds of a block argmin over 64-bit keys (D[u,v] << 32 | v),
 // each round taking the smallest key above the previous one.  Keys are unique, so the result
 // is the (distance, node index) order of src/distances/matrix.py:205-209.
@@ -117,5 +184,26 @@ extern "C" int tspmi_tour_cost(const int32_t* tours, int P, int n, const int32_t
   hipLaunchKernelGGL(tour_cost_kernel, dim3((P + per_block - 1) / per_block), dim3(kCostBlock), 0,
                      (hipStream_t)stream, tours, P, n, D, ld, closed, out);
   TSPMI_CHECK_LAUNCH("tspmi_tour_cost launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_dist_metric(const double* xy, int n, int metric, int32_t* D, int ld, int32_t* max_out,
+                                 void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(n >= 0 && ld >= n && n <= 65535, "tspmi_dist_metric: bad shape n=%d ld=%d (n <= 65535)", n, ld);
+  TSPMI_CHECK_ARG(metric >= kEuc2d && metric <= kGeo, "tspmi_dist_metric: unknown metric %d", metric);
+  if (n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(xy && D, "tspmi_dist_metric: null pointer");
+  const dim3 grid((n + kDistTile - 1) / kDistTile, n);
+  hipStream_t s = (hipStream_t)stream;
+  switch (metric) {
+    case kEuc2d: hipLaunchKernelGGL(dist_metric_kernel<kEuc2d>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+    case kCeil2d: hipLaunchKernelGGL(dist_metric_kernel<kCeil2d>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+    case kMan2d: hipLaunchKernelGGL(dist_metric_kernel<kMan2d>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+    case kMax2d: hipLaunchKernelGGL(dist_metric_kernel<kMax2d>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+    case kAtt: hipLaunchKernelGGL(dist_metric_kernel<kAtt>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+    default: hipLaunchKernelGGL(dist_metric_kernel<kGeo>, grid, dim3(kDistTile), 0, s, xy, n, D, ld, max_out); break;
+  }
+  TSPMI_CHECK_LAUNCH("tspmi_dist_metric launch");
   return TSPMI_OK;
 }
