@@ -1,0 +1,75 @@
+"""The summary record (restating src/benchmarking/runner/algorithm_runner.py:75-442 for TSP) on
+hand-made run results, and config parsing without a device."""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN
+
+REFERENCE_KEYS = {
+  "algorithm", "repetitions", "successful_runs", "mean_time", "std_time", "min_time", "max_time", "mean_cost",
+  "std_cost", "best_cost", "worst_cost", "mean_initial_cost", "std_initial_cost", "mean_gap", "std_gap", "best_gap",
+  "mean_improvement", "mean_generations", "std_generations", "min_generations", "max_generations", "mean_h2d_mb",
+  "mean_d2h_mb", "total_transfer_mb", "mean_kernels", "best_tour", "raw_best_tours", "problem_type", "edge_type",
+  "raw_times", "raw_initial_costs", "raw_costs", "raw_gaps", "raw_generations", "raw_stop_reasons", "raw_seeds",
+  "raw_phase_timings", "mean_phase_timings", "phase_share_pct", "phase_share_cv_pct",
+}
+
+
+def _results():
+  return {
+    "times": [1.0, 2.0, float("nan"), 3.0], "initial_costs": [200.0, 100.0, float("nan"), 400.0],
+    "final_costs": [110.0, 100.0, float("nan"), 120.0], "gaps": [10.0, 0.0, float("nan"), 20.0],
+    "generations": [5, 7, 0, 9], "stop_reasons": ["a", "b", "failed", "c"], "seeds": [42, 43, 44, 45],
+    "h2d_bytes": [1e6, 3e6, 0, 2e6], "d2h_bytes": [0, 0, 0, 4e6], "kernel_launches": [10, 20, 0, 30],
+    "phase_timings": [{"ls": {"total_ms": 500.0, "calls": 5}}, {"ls": {"total_ms": 1000.0, "calls": 5}}, {},
+                      {"ls": {"total_ms": 600.0, "calls": 2}}],
+    "best_tours": [[0, 1, 2], [0, 2, 1], None, [1, 0, 2]],
+  }
+
+
+def test_summary_record_values():
+  from tsp_gpu_clean_amd import Problem
+  from tsp_gpu_clean_amd.runner import compute_series_stats, summary_payload
+
+  prob = Problem.from_coordinates("p", np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]))
+  rec = summary_payload(_results(), prob, "algo", has_known_optimal=True)
+  assert set(rec) == REFERENCE_KEYS
+  assert rec["repetitions"] == 4 and rec["successful_runs"] == 3
+  assert rec["mean_time"] == pytest.approx(2.0) and rec["std_time"] == pytest.approx(1.0)  # ddof = 1
+  assert (rec["best_cost"], rec["worst_cost"]) == (100.0, 120.0)
+  assert rec["best_tour"] == [0, 2, 1]  # the run with the lowest final cost
+  assert rec["mean_gap"] == pytest.approx(10.0) and rec["best_gap"] == 0.0
+  assert rec["mean_improvement"] == pytest.approx((45.0 + 0.0 + 70.0) / 3)
+  assert rec["mean_h2d_mb"] == pytest.approx(1.5) and rec["mean_d2h_mb"] == pytest.approx(1.0)  # over 4 runs
+  assert rec["mean_kernels"] == pytest.approx(15.0)
+  assert rec["raw_seeds"] == [42, 43, 45] and rec["raw_stop_reasons"] == ["a", "b", "c"]
+  assert rec["mean_phase_timings"]["ls"]["total_ms"] == pytest.approx(700.0)
+  assert rec["phase_share_pct"]["ls"] == pytest.approx((50.0 + 50.0 + 20.0) / 3)
+  assert compute_series_stats(np.array([np.nan, np.nan]), 2) == (0.0, 0.0, 0.0, 0.0)
+  assert compute_series_stats(np.array([3.0]), 1) == (3.0, 0.0, 3.0, 3.0)
+  rec2 = summary_payload(_results(), prob, "algo", has_known_optimal=False)
+  assert rec2["mean_gap"] is None and rec2["raw_gaps"] is None
+
+
+def test_record_json_and_configs(tmp_path):
+  from tsp_gpu_clean_amd import Problem
+  from tsp_gpu_clean_amd.runner import build_algorithm, save_record, summary_payload
+
+  prob = Problem.from_coordinates("p", np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]))
+  path = tmp_path / "rec.json"
+  save_record(str(path), summary_payload(_results(), prob, "algo"))
+  loaded = json.loads(path.read_text())
+  assert set(loaded) == REFERENCE_KEYS and loaded["raw_gaps"] == [10.0, 0.0, 20.0]
+  with open(os.path.join(GOLDEN, "algorithms.json"), encoding="utf-8") as fh:
+    cfg = json.load(fh)
+  ga = build_algorithm(cfg["algorithm_configs"]["FullGPUModular"], cfg["default_params"])
+  assert type(ga.crossover).__name__ == "CrossoverGPUPermOrder" and ga.population_size == 32
+  assert ga.selection.k == 3 and ga.local_search.max_iterations == 5 and ga.local_search.candidate_k == 20
+  with pytest.raises(KeyError):  # host (CPU) strategy classes are the reference's, not this package's
+    build_algorithm(cfg["algorithm_configs"]["CPU"], cfg["default_params"])
