@@ -80,7 +80,9 @@ int tspmi_tour_cost(const int32_t* tours, int P, int n, const int32_t* D, int ld
  *         order, applied unless its window [i, j] meets an already accepted one; two_opt.py:491-563;
  *         needs about 30 n bytes of LDS, so n <= ~5000).
  *   cand: (n, k) candidate table or NULL/k=0 for the full neighbourhood.
- *   max_iter: passes per tour; < 0 runs each tour to its local optimum.
+ *   max_iter: passes per tour; < 0 runs each tour to its local optimum, which requires a
+ *             symmetric matrix (on an asymmetric one the delta formula does not make the cost
+ *             decrease and the moves can cycle forever: pass a bound there).
  *   stats: device int64[3] accumulated with {deltas evaluated, moves applied, passes}
  *          (may be NULL).
  * Replaces: src/algorithms/strategies/local_search/two_opt.py:831-899 (_improve_grid) with

@@ -212,6 +212,8 @@ def tour_cost(tours, D, out, closed: bool = True) -> None:
 
 
 def two_opt(tours, D, cand=None, mode: int = 0, max_iter: int = 10, stats=None) -> None:
+  """2-opt passes in place; max_iter < 0 runs to the local optimum (symmetric D only: on an
+  asymmetric matrix the moves can cycle, so pass a bound)."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   _need(D, torch.int32, "D", 2)

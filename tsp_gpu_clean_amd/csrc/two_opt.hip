@@ -790,6 +790,8 @@ extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t*
   hipLaunchKernelGGL((two_opt_cand_kernel<BL, KPV, U>), dim3(P), dim3(BL), lds, s, tours, n, D, ld, ct, k, max_iter, \
                      stats)
   // (A 16-way unroll at 1024 threads was measured slower on 16-tour elite calls: 16 vs 14.5 us per pass.)
+  // (Holding the table in LDS for these launches saved 7 % per pass but cost 5.6 us per launch to
+  // fill, measured on 16-tour elite calls; not kept.)
   if (block == 1024) {
     if (kp == 8) TSPMI_LAUNCH_CAND(1024, 8, 4);
     else if (kp == 16) TSPMI_LAUNCH_CAND(1024, 16, 4);

@@ -160,4 +160,6 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
 }
 
 
+
+
 }  // namespace tspmi
