@@ -115,6 +115,7 @@ def time_generations(torch, native, generations=200, warmup=5):
   graph.replay()
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
+  loop.verify()
   best_cost = float(loop.fitness.min().item())
   elite_deltas = int(loop.stats[0].item()) / generations
   return generations / wall, wall, best_cost, elite_deltas

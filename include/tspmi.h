@@ -111,6 +111,31 @@ int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld,
                          int max_iter, int64_t* stats, void* stream);
 
 /*
+ * tspmi_two_opt_packed that also writes cost[p] = closed cost of tour p after the search (P
+ * float64; NULL for none), the value tspmi_tour_cost(closed = 1) gives, without a second launch.
+ * Replaces: the local-search-then-evaluate pair of the elite step, pipeline_steps.py:166-183
+ *           (EliteOnlyLocalSearchStep: ls.improve_batch, then evaluation.evaluate_batch).
+ */
+int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                              int max_iter, int64_t* stats, double* cost, void* stream);
+
+/*
+ * tspmi_two_opt_packed_cost for a few tours (the GA's elite set): when P tours leave CUs idle,
+ * each tour is searched by a cluster of up to 16 workgroups that share its candidate pairs and
+ * meet once per pass at a bounded barrier in `sync`; otherwise this is tspmi_two_opt_packed_cost.
+ * Same tours, moves, counters and costs either way.
+ *   sync: device int64[528 * P], zeroed before the first call; every call leaves it ready for the
+ *         next (its flags and counters cleared).  Word 528 p + 512 is set when tour p's cluster
+ *         was not co-resident and its barrier timed out (that tour and its cost are then left
+ *         unchanged): check it after the work completes and clear it before the next call.
+ *         Calls that overlap on one `sync` race.
+ * Replaces: the elite local search of pipeline_steps.py:166-183 (EliteOnlyLocalSearchStep) with
+ *           two_opt.py:831-899 in candidate mode on the elite rows.
+ */
+int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                        int max_iter, int64_t* stats, double* cost, uint64_t* sync, void* stream);
+
+/*
  * EUC_2D variants that compute every distance from the coordinates xy (n, 2) float64 (16-byte
  * aligned) with the FP64 nint of tspmi_dist_euc2d instead of reading a matrix: same tours, moves
  * and counters as tspmi_two_opt(mode = BEST, full neighbourhood) and tspmi_two_opt_packed on the
@@ -121,6 +146,10 @@ int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld,
 int tspmi_two_opt_euc2d(int32_t* tours, int P, int n, const double* xy, int max_iter, int64_t* stats, void* stream);
 int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
                                int max_iter, int64_t* stats, void* stream);
+/* tspmi_two_opt_packed_euc2d with the final closed costs (as tspmi_two_opt_packed_cost);
+ * costs need n >= 4 and max_iter != 0. */
+int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
+                                    int max_iter, int64_t* stats, double* cost, void* stream);
 
 /*
  * Tournament selection: out[r] = pop[w_r] with w_r = cand[r, argmin_c fitness[cand[r, c]]]
@@ -183,7 +212,8 @@ int tspmi_generation(const int32_t* pop, const double* fitness, int P, int n, co
  * (key[j] == key[i] and j < i)}; writes order[rank[i]] = i for rank[i] < keep.
  * With keys = concat(old_fit, new_fit) this is the (mu + lambda) survivor order of
  * genetic_algorithm.py:618-706 (lexsort :689); with keys = fitness and keep = 16 it is the
- * elite set of pipeline_steps.py:153-163.
+ * elite set of pipeline_steps.py:153-163.  Keys compare as IEEE totally-ordered bits with -0.0
+ * equal to +0.0 (NaN sorts after +inf), so the order is defined for every input.  M <= 2^24.
  */
 int tspmi_rank_select(const double* keys, int M, int keep, int32_t* order, void* stream);
 

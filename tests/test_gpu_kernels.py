@@ -307,6 +307,32 @@ def test_rank_select_is_stable_survival(torch_mod, nat):
   np.testing.assert_array_equal(order16.cpu().numpy(), np.argsort(old, kind="stable")[:16])
 
 
+@pytest.mark.parametrize("M,keep", [(1, 1), (7, 7), (64, 10), (65, 65), (2048, 1024), (5000, 4999), (20000, 300)])
+def test_rank_select_matches_stable_argsort(torch_mod, nat, M, keep):
+  """Ties, signed zeros, infinities and NaN: the order is numpy's stable argsort (NaN last)."""
+  say(f"rank_select M={M} keep={keep}")
+  rng = np.random.default_rng(M)
+  keys = rng.integers(-20, 20, M).astype(np.float64)
+  special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e300, -1e-300])
+  hit = rng.random(M) < 0.2
+  keys[hit] = special[rng.integers(0, len(special), int(hit.sum()))]
+  order = torch_mod.full((keep,), -1, dtype=torch_mod.int32, device="cuda")
+  nat.rank_select(cu(keys), keep, order)
+  np.testing.assert_array_equal(order.cpu().numpy(), np.argsort(keys, kind="stable")[:keep])
+
+
+@pytest.mark.parametrize("n", [1, 21, 1000, 1003])
+def test_gather_rows_widths(torch_mod, nat, n):
+  """Rows of every width, 16-byte copies (n % 4 == 0) and element copies alike."""
+  say(f"gather_rows n={n}")
+  A = _population(n, 9, 3)
+  B = _population(n, 7, 4)
+  src = np.array([15, 2, 8, 0, 9, 3], dtype=np.int32)
+  out = torch_mod.empty((6, n), dtype=torch_mod.int32, device="cuda")
+  nat.gather_rows(cu(A), cu(np.zeros(9)), cu(B), cu(np.zeros(7)), cu(src), out, None)
+  np.testing.assert_array_equal(out.cpu().numpy(), np.vstack([A, B])[src])
+
+
 def test_gather_and_scatter_rows(torch_mod, nat):
   say("gather/scatter rows")
   A = _population(20, 6, 1)
@@ -415,6 +441,79 @@ def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
   np.testing.assert_array_equal(got[rows], want)
   same, _ = c_two_opt.improve_batch(got, d, cand, "best", 1)
   np.testing.assert_array_equal(same, got)
+
+
+@pytest.mark.parametrize("n,P,k,iters", [(1000, 16, 20, 10), (1024, 16, 20, -1), (1025, 16, 20, 10), (300, 300, 8, 5),
+                                         (6, 3, 5, 4)])
+def test_packed_two_opt_fused_cost(torch_mod, nat, n, P, k, iters):
+  """The cost written by the search launch equals the tour cost of its result, for the
+  register-resident form (n * k <= 20480, P <= CUs) and the general one alike, matrix and
+  coordinates."""
+  say(f"two_opt_packed cost n={n} P={P} k={k}")
+  xy = uniform_coords(n, seed=n + P)
+  D, _ = device_dist(nat, torch_mod, xy)
+  cand = torch_mod.empty((n, min(k, n - 1)), dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, k, cand)
+  kk = cand.shape[1]
+  table = torch_mod.empty((n, nat.candidate_lanes(kk), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cand, D, table)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n), P))
+  plain, fused, coord = cu(tours), cu(tours), cu(tours)
+  s0, s1, s2 = (torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda") for _ in range(3))
+  cost = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
+  cost_xy = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
+  nat.two_opt_packed(plain, D, table, kk, iters, s0)
+  nat.two_opt_packed(fused, D, table, kk, iters, s1, cost)
+  nat.two_opt_packed_euc2d(coord, cu(xy), table, kk, iters, s2, cost_xy)
+  np.testing.assert_array_equal(fused.cpu().numpy(), plain.cpu().numpy())
+  np.testing.assert_array_equal(coord.cpu().numpy(), plain.cpu().numpy())
+  assert s1.cpu().tolist() == s0.cpu().tolist() == s2.cpu().tolist()
+  want = torch_mod.empty(P, dtype=torch_mod.float64, device="cuda")
+  nat.tour_cost(plain, D, want)
+  np.testing.assert_array_equal(cost.cpu().numpy(), want.cpu().numpy())
+  np.testing.assert_array_equal(cost_xy.cpu().numpy(), want.cpu().numpy())
+  rows = [0, P - 1]
+  ref, _ = c_two_opt.improve_batch(tours[rows], compute_distance_matrix(xy), cand.cpu().numpy(), "best", iters)
+  np.testing.assert_array_equal(plain.cpu().numpy()[rows], ref)
+
+
+@pytest.mark.parametrize("n,P,k,iters", [(1000, 16, 20, 10), (1000, 16, 20, -1), (300, 4, 8, -1), (2000, 8, 16, 20),
+                                         (6000, 2, 5, 8), (1000, 200, 20, 5), (7, 3, 6, 3)])
+def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters):
+  """Workgroup clusters per tour (few tours) or the one-workgroup fallback: the tours, counters and
+  costs of tspmi_two_opt_packed, the C oracle's tours, and a workspace left cleared for the next
+  call (run twice in a row)."""
+  say(f"two_opt_elite n={n} P={P} k={k} iters={iters}")
+  xy = uniform_coords(n, seed=n * 3 + P)
+  D, _ = device_dist(nat, torch_mod, xy)
+  kk = min(k, n - 1)
+  cand = torch_mod.empty((n, kk), dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, kk, cand)
+  table = torch_mod.empty((n, nat.candidate_lanes(kk), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cand, D, table)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + P), P))
+  sync = torch_mod.zeros(nat.ELITE_SYNC_WORDS * P, dtype=torch_mod.int64, device="cuda")
+  want, got = cu(tours), cu(tours)
+  for call in range(2):
+    sw = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+    sg = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+    cost = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
+    nat.two_opt_packed(want, D, table, kk, iters, sw)
+    nat.two_opt_elite(got, D, table, kk, iters, sg, cost, sync)
+    torch_mod.cuda.synchronize()
+    assert not nat.elite_failed(sync)
+    words = sync.view(P, nat.ELITE_SYNC_WORDS)
+    assert not words[:, 0:256:16].any().item(), f"call {call}: pass flags not cleared"
+    assert not words[:, 512:514].any().item(), f"call {call}: error / departure words not cleared"
+    np.testing.assert_array_equal(got.cpu().numpy(), want.cpu().numpy(), err_msg=f"call {call}")
+    assert sg.cpu().tolist() == sw.cpu().tolist()
+    ref = torch_mod.empty(P, dtype=torch_mod.float64, device="cuda")
+    nat.tour_cost(want, D, ref)
+    np.testing.assert_array_equal(cost.cpu().numpy(), ref.cpu().numpy())
+  rows = [0, P - 1]
+  first, _ = c_two_opt.improve_batch(tours[rows], compute_distance_matrix(xy), cand.cpu().numpy(), "best", iters)
+  again, _ = c_two_opt.improve_batch(first, compute_distance_matrix(xy), cand.cpu().numpy(), "best", iters)
+  np.testing.assert_array_equal(got.cpu().numpy()[rows], again)
 
 
 @pytest.mark.parametrize("n,iters,rows", [(2100, 60, 3), (3000, 40, 2), (6147, 10, 2), (10000, 6, 2), (10243, 4, 1),

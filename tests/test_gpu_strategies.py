@@ -185,6 +185,8 @@ def test_device_loop_matches_step_pipeline_and_oracle(t):
     loop.step(draws, g)
     np.testing.assert_array_equal(loop.population.cpu().numpy(), res.populations[g], err_msg=f"generation {g + 1}")
     np.testing.assert_array_equal(loop.fitness.cpu().numpy(), res.fitnesses[g])
+  loop.verify()
+  assert not loop.sync.view(-1, 528)[:, list(range(0, 256, 16)) + [512, 513]].any().item()  # barrier words cleared
   assert ctx.rng.bit_generator.state == rng.bit_generator.state
 
 
@@ -221,6 +223,7 @@ def test_device_loop_graph_replays_match_eager(t):
     np.testing.assert_array_equal(loop.fitness.cpu().numpy(), want_fit)
     np.testing.assert_array_equal(best.cpu().numpy(), want_best)
     _ = (loop.population.float() * 2).sum().item()  # unrelated eager work between replays
+  loop.verify()
 
 
 def test_local_search_strategy_large_n_computes_distances_from_coordinates(t):

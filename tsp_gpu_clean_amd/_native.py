@@ -35,6 +35,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
   "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
+  "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
+  "tspmi_two_opt_elite": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
@@ -60,6 +62,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_swap": [_P, _I, _I, _P, _P, _P, _D, _P],
   "tspmi_two_opt_euc2d": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
+  "tspmi_two_opt_packed_euc2d_cost": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_scorer_param_count": [],
   "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
   "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
@@ -247,7 +250,9 @@ def candidate_table(cand, D, out) -> None:
          "tspmi_candidate_table")
 
 
-def two_opt_packed(tours, D, table, k: int, max_iter: int = 10, stats=None) -> None:
+def two_opt_packed(tours, D, table, k: int, max_iter: int = 10, stats=None, cost=None) -> None:
+  """Candidate-list best-improvement 2-opt on a packed table; ``cost`` (float64 (P,), optional)
+  receives each final tour's closed cost from the same launch."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   _need(D, torch.int32, "D", 2)
@@ -257,8 +262,46 @@ def two_opt_packed(tours, D, table, k: int, max_iter: int = 10, stats=None) -> N
   P, n = tours.shape
   if table.shape[0] != n or table.shape[1] != candidate_lanes(k):
     raise ValueError("two_opt_packed: table does not match n / k")
-  _check(_lib.tspmi_two_opt_packed(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
-                                   _ptr(stats), _stream(torch)), "tspmi_two_opt_packed")
+  if cost is not None:
+    _need(cost, torch.float64, "cost", 1)
+    if cost.shape[0] < P:
+      raise ValueError("two_opt_packed: cost shorter than the number of tours")
+  _check(_lib.tspmi_two_opt_packed_cost(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
+                                        _ptr(stats), _ptr(cost), _stream(torch)), "tspmi_two_opt_packed_cost")
+
+
+ELITE_SYNC_WORDS = 528  # int64 words of the elite 2-opt workspace per tour
+ELITE_ERROR_WORD = 512  # its barrier-timeout word
+
+
+def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync) -> None:
+  """Elite-set candidate 2-opt with final costs: :func:`two_opt_packed` semantics, one cluster of
+  workgroups per tour when the tours leave CUs idle.  ``sync`` is an int64 device workspace of
+  ``ELITE_SYNC_WORDS * P`` words, zeroed before first use; :func:`elite_failed` reports a cluster
+  whose barrier timed out (its tour and cost are then left unchanged)."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(table, torch.int32, "table", 3)
+  _need(sync, torch.int64, "sync", 1)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  P, n = tours.shape
+  if table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+    raise ValueError("two_opt_elite: table does not match n / k")
+  if cost is not None:
+    _need(cost, torch.float64, "cost", 1)
+    if cost.shape[0] < P:
+      raise ValueError("two_opt_elite: cost shorter than the number of tours")
+  if sync.shape[0] < ELITE_SYNC_WORDS * P:
+    raise ValueError(f"two_opt_elite: sync needs {ELITE_SYNC_WORDS * P} words")
+  _check(_lib.tspmi_two_opt_elite(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
+                                  _ptr(stats), _ptr(cost), _ptr(sync), _stream(torch)), "tspmi_two_opt_elite")
+
+
+def elite_failed(sync) -> bool:
+  """True when a cluster barrier of :func:`two_opt_elite` timed out (synchronises the device)."""
+  return bool((sync.view(-1, ELITE_SYNC_WORDS)[:, ELITE_ERROR_WORD] != 0).any().item())
 
 
 def tournament(pop, fitness, cand, out, winners_out=None) -> None:
@@ -409,8 +452,9 @@ def two_opt_euc2d(tours, xy, max_iter: int = 10, stats=None) -> None:
          "tspmi_two_opt_euc2d")
 
 
-def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None) -> None:
-  """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates."""
+def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None, cost=None) -> None:
+  """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates;
+  ``cost`` as for :func:`two_opt_packed` (needs n >= 4 and max_iter != 0)."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   _need(xy, torch.float64, "xy", 2)
@@ -420,8 +464,12 @@ def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=Non
   P, n = tours.shape
   if xy.shape != (n, 2) or table.shape[0] != n or table.shape[1] != candidate_lanes(k):
     raise ValueError("two_opt_packed_euc2d: xy / table do not match n / k")
-  _check(_lib.tspmi_two_opt_packed_euc2d(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter), _ptr(stats),
-                                         _stream(torch)), "tspmi_two_opt_packed_euc2d")
+  if cost is not None:
+    _need(cost, torch.float64, "cost", 1)
+    if cost.shape[0] < P:
+      raise ValueError("two_opt_packed_euc2d: cost shorter than the number of tours")
+  _check(_lib.tspmi_two_opt_packed_euc2d_cost(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
+                                              _ptr(stats), _ptr(cost), _stream(torch)), "tspmi_two_opt_packed_euc2d_cost")
 
 
 def nn_greedy_init(starts, D, out) -> None:

@@ -28,21 +28,70 @@ __device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ can
 
 // PMX child of (p1, p2) with cuts [c1, c2] into `child` (pmx_crossover.py:49-105, open
 // permutation).  inv1/inv2 map node -> position in p1/p2.  Caller synchronises afterwards.
+//
+// The reference places each value p2[idx] of the segment that p1's segment lacks by walking
+// pos -> inv2[p1[pos]] from idx until pos leaves [c1, c2].  Those walks are paths of the partial
+// map S(pos) = inv2[p1[pos]] on segment positions: a start idx has no preimage inside the
+// segment, so it never lies on a cycle.  With random parents a walk runs to ~150 dependent steps
+// at n = 1000, and the longest one sets the workgroup's time.  When `jump` (c2 - c1 + 1 ints of
+// LDS) is given, S is tabulated and squared in place, T <- T o T with exit positions fixed, until
+// every start's entry has left the segment: ceil(log2(longest walk)) rounds of one barrier each,
+// none at all when no value needs placing.  Each thread squares the entries of the start
+// positions it owns, so it knows when its own starts are resolved.  An entry read while its
+// owner rewrites it is either the old or the new value, both points of the same path no further
+// than its exit, so the race never changes a result.  The placement -- the exit position of each
+// walk -- is the reference's.
 __device__ void pmx_child(const int32_t* p1, const int32_t* p2, const int32_t* inv1, const int32_t* inv2, int n,
-                          int c1, int c2, int32_t* child) {
+                          int c1, int c2, int32_t* child, int32_t* jump) {
   for (int x = threadIdx.x; x < n; x += blockDim.x) child[x] = (x >= c1 && x <= c2) ? p1[x] : -1;
-  __syncthreads();
   const int seg_len = c2 - c1 + 1;
-  for (int idx = c1 + threadIdx.x; idx <= c2; idx += blockDim.x) {
-    const int val = p2[idx];
-    const int pv = inv1[val];
-    if (pv >= c1 && pv <= c2) continue;  // val already in the copied segment
-    int pos = idx, steps = 0;
-    while (pos >= c1 && pos <= c2 && steps <= seg_len) {
-      pos = inv2[p1[pos]];
-      ++steps;
+  if (jump != nullptr && seg_len <= 32 * (int)blockDim.x) {
+    uint32_t starts = 0;  // bit m: segment entry threadIdx.x + m * blockDim.x starts a walk
+    bool pending = false;
+    int m = 0;
+    for (int e = threadIdx.x; e < seg_len; e += blockDim.x, ++m) {
+      const int pv = inv1[p2[c1 + e]];
+      const int t = inv2[p1[c1 + e]];
+      jump[e] = t;
+      if (pv < c1 || pv > c2) {
+        starts |= 1u << m;
+        pending |= (t >= c1 && t <= c2);
+      }
     }
-    if ((unsigned)pos < (unsigned)n && (pos < c1 || pos > c2)) child[pos] = val;
+    pending = __syncthreads_or(pending);
+    const int max_rounds = 32 - __clz(seg_len);  // 2^rounds >= seg_len >= any walk
+    for (int r = 0; pending && r < max_rounds; ++r) {
+      bool mine = false;
+      m = 0;
+      for (int e = threadIdx.x; e < seg_len; e += blockDim.x, ++m) {
+        int t = jump[e];
+        if (t >= c1 && t <= c2) {
+          t = jump[t - c1];
+          jump[e] = t;
+        }
+        mine |= ((starts >> m) & 1u) && t >= c1 && t <= c2;
+      }
+      pending = __syncthreads_or(mine);
+    }
+    m = 0;
+    for (int e = threadIdx.x; e < seg_len; e += blockDim.x, ++m) {
+      if (!((starts >> m) & 1u)) continue;
+      const int pos = jump[e];
+      if ((unsigned)pos < (unsigned)n && (pos < c1 || pos > c2)) child[pos] = p2[c1 + e];
+    }
+  } else {
+    __syncthreads();
+    for (int idx = c1 + threadIdx.x; idx <= c2; idx += blockDim.x) {
+      const int val = p2[idx];
+      const int pv = inv1[val];
+      if (pv >= c1 && pv <= c2) continue;  // val already in the copied segment
+      int pos = idx, steps = 0;
+      while (pos >= c1 && pos <= c2 && steps <= seg_len) {
+        pos = inv2[p1[pos]];
+        ++steps;
+      }
+      if ((unsigned)pos < (unsigned)n && (pos < c1 || pos > c2)) child[pos] = val;
+    }
   }
   __syncthreads();
   for (int x = threadIdx.x; x < n; x += blockDim.x)
@@ -76,9 +125,33 @@ __device__ __forceinline__ double block_cost(const int32_t* t, int n, const int3
   return (double)block_sum_i64(s, red);
 }
 
+// Inversion (when drawn), cost and store of child row r held in LDS `ch`.
+__device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const int32_t* __restrict__ D, int ld,
+                                             const double* __restrict__ probs, const int32_t* __restrict__ pos1,
+                                             const int32_t* __restrict__ pos2, double rate,
+                                             int32_t* __restrict__ offspring, double* __restrict__ off_fit,
+                                             int64_t* red) {
+  if (probs[r] < rate) {
+    int lo = pos1[r], hi = pos2[r];
+    if (lo > hi) {
+      const int t = lo;
+      lo = hi;
+      hi = t;
+    }
+    lo = lo < 0 ? 0 : lo;
+    hi = hi >= n ? n - 1 : hi;
+    reverse_range(ch, lo, hi);
+    __syncthreads();
+  }
+  const double cost = block_cost(ch, n, D, ld, red);
+  for (int x = threadIdx.x; x < n; x += blockDim.x) offspring[(size_t)r * n + x] = ch[x];
+  if (threadIdx.x == 0) off_fit[r] = cost;
+}
+
 // Fused tournament -> PMX -> inversion -> evaluate of child pair q (rows 2q, 2q+1) by one
 // workgroup.  LDS: p1, p2, inv1, inv2, childA, childB (6n ints) in `lds`; `red` holds one int64
-// per wave and `s_w` two ints.
+// per wave and `s_w` two ints.  Child A is finished and stored before child B is built, so each
+// child's PMX borrows the other child's buffer for its jump table.
 __device__ void generation_pair(int q, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P, int n,
                                 const int32_t* __restrict__ D, int ld, const int32_t* __restrict__ tour_cand, int k,
                                 const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
@@ -104,40 +177,24 @@ __device__ void generation_pair(int q, const int32_t* __restrict__ pop, const do
   if (single) {
     for (int x = threadIdx.x; x < n; x += blockDim.x) cA[x] = p1[x];
     __syncthreads();
-  } else {
-    int c1 = cut_lo[q], c2 = cut_hi[q];
-    if (c1 > c2) {
-      const int t = c1;
-      c1 = c2;
-      c2 = t;
-    }
-    c1 = c1 < 0 ? 0 : c1;
-    c2 = c2 >= n ? n - 1 : c2;
-    pmx_child(p1, p2, inv1, inv2, n, c1, c2, cA);
-    pmx_child(p2, p1, inv2, inv1, n, c1, c2, cB);
-    __syncthreads();
+    finish_child(r0, cA, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+    return;
   }
-
-  const int nrows = single ? 1 : 2;
-  for (int c = 0; c < nrows; ++c) {
-    const int r = r0 + c;
-    int32_t* ch = c == 0 ? cA : cB;
-    if (probs[r] < rate) {
-      int lo = pos1[r], hi = pos2[r];
-      if (lo > hi) {
-        const int t = lo;
-        lo = hi;
-        hi = t;
-      }
-      lo = lo < 0 ? 0 : lo;
-      hi = hi >= n ? n - 1 : hi;
-      reverse_range(ch, lo, hi);
-      __syncthreads();
-    }
-    const double cost = block_cost(ch, n, D, ld, red);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) offspring[(size_t)r * n + x] = ch[x];
-    if (threadIdx.x == 0) off_fit[r] = cost;
+  int c1 = cut_lo[q], c2 = cut_hi[q];
+  if (c1 > c2) {
+    const int t = c1;
+    c1 = c2;
+    c2 = t;
   }
+  c1 = c1 < 0 ? 0 : c1;
+  c2 = c2 >= n ? n - 1 : c2;
+  pmx_child(p1, p2, inv1, inv2, n, c1, c2, cA, cB);
+  __syncthreads();
+  finish_child(r0, cA, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+  __syncthreads();  // child A's last reads are done before its buffer becomes B's jump table
+  pmx_child(p2, p1, inv2, inv1, n, c1, c2, cB, cA);
+  __syncthreads();
+  finish_child(r0 + 1, cB, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
 }
 
 

@@ -23,11 +23,12 @@ __global__ __launch_bounds__(kRowBlock) void tournament_kernel(const int32_t* __
   for (int x = lane; x < n; x += kWave) dst[x] = src[x];
 }
 
-// One workgroup per pair (2q, 2q+1); LDS: p1, p2, inv1, inv2 (4n ints).
+// One workgroup per pair (2q, 2q+1); LDS: p1, p2, inv1, inv2 (4n ints) and, when `with_jump`,
+// the PMX jump table (n more ints).
 __global__ __launch_bounds__(kRowBlock) void pmx_kernel(const int32_t* __restrict__ parents, int P, int n,
                                                         const int32_t* __restrict__ cut_lo,
                                                         const int32_t* __restrict__ cut_hi,
-                                                        int32_t* __restrict__ out) {
+                                                        int32_t* __restrict__ out, int with_jump) {
   extern __shared__ int32_t lds[];
   const int q = blockIdx.x;
   const int r0 = 2 * q;
@@ -50,8 +51,10 @@ __global__ __launch_bounds__(kRowBlock) void pmx_kernel(const int32_t* __restric
   c1 = c1 < 0 ? 0 : c1;
   c2 = c2 >= n ? n - 1 : c2;
   __syncthreads();
-  pmx_child(p1, p2, inv1, inv2, n, c1, c2, out + (size_t)r0 * n);
-  pmx_child(p2, p1, inv2, inv1, n, c1, c2, out + (size_t)(r0 + 1) * n);
+  int32_t* jump = with_jump ? lds + 4 * n : nullptr;
+  pmx_child(p1, p2, inv1, inv2, n, c1, c2, out + (size_t)r0 * n, jump);
+  __syncthreads();  // the jump table is reused by the second child
+  pmx_child(p2, p1, inv2, inv1, n, c1, c2, out + (size_t)(r0 + 1) * n, jump);
 }
 
 // One wave per row.
@@ -93,56 +96,71 @@ __global__ __launch_bounds__(kRowBlock) void generation_kernel(
                   off_fit, lds, red, s_w);
 }
 
-// Stable rank of M float64 keys; order[rank[i]] = i for rank[i] < keep.
-// Keys are staged through LDS in tiles so each thread compares its key against all M keys.
-constexpr int kRankBlock = 256;
-constexpr int kRankTile = 2048;
+// Stable ascending order of M float64 keys: order[rank(i)] = i for rank(i) < keep, with
+// rank(i) = #{j : (b_j, j) < (b_i, i)} over the orderable key bits b (the total order the bitonic
+// network of the batch survival kernel sorts by, so both select the same rows).  A workgroup owns
+// 64 keys, one per lane; the keys pass through LDS in tiles of kCountTile (already as orderable
+// bits), its 16 waves split each tile, every lane of a wave reading the same key j (a broadcast,
+// no bank conflict), and the per-wave counts are added at the end.  A 2048-key selection is one
+// short pass instead of 66 sorting-network stages.
+constexpr int kCountBlock = 1024;
+constexpr int kCountWaves = kCountBlock / kWave;
+constexpr int kCountTile = 4096;
 
-__global__ __launch_bounds__(kRankBlock) void rank_select_kernel(const double* __restrict__ keys, int M, int keep,
+__global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* __restrict__ keys, int M, int keep,
                                                                  int32_t* __restrict__ order) {
-  __shared__ double tile[kRankTile];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double ki = i < M ? keys[i] : 0.0;
-  int rank = 0;
-  for (int base = 0; base < M; base += kRankTile) {
-    const int len = (M - base) < kRankTile ? (M - base) : kRankTile;
+  __shared__ uint64_t tile[kCountTile];
+  __shared__ int part[kCountWaves][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int i = blockIdx.x * kWave + lane;
+  const uint64_t bi = i < M ? orderable_bits(keys[i]) : 0ull;
+  int cnt = 0;
+  for (int base = 0; base < M; base += kCountTile) {
+    const int len = (M - base) < kCountTile ? (M - base) : kCountTile;
     __syncthreads();
-    for (int x = threadIdx.x; x < len; x += blockDim.x) tile[x] = keys[base + x];
+    for (int x = threadIdx.x; x < len; x += kCountBlock) tile[x] = orderable_bits(keys[base + x]);
     __syncthreads();
-    if (i < M) {
-      for (int x = 0; x < len; ++x) {
-        const double kj = tile[x];
-        const int j = base + x;
-        rank += (kj < ki) || (kj == ki && j < i);
-      }
+    const int chunk = (len + kCountWaves - 1) / kCountWaves;
+    const int j0 = wid * chunk;
+    const int j1 = (j0 + chunk) < len ? (j0 + chunk) : len;
+    const int ib = i - base;  // j < i  <=>  x < ib
+#pragma unroll 4
+    for (int x = j0; x < j1; ++x) {
+      const uint64_t bj = tile[x];
+      cnt += (bj < bi) | ((bj == bi) & (x < ib));
     }
   }
-  if (i < M && rank < keep) order[rank] = i;
+  part[wid][lane] = cnt;
+  __syncthreads();
+  if (wid == 0 && i < M) {
+    int rank = 0;
+#pragma unroll
+    for (int w = 0; w < kCountWaves; ++w) rank += part[w][lane];
+    if (rank < keep) order[rank] = i;
+  }
 }
 
-// Stable order of up to 4096 float64 keys in one workgroup: bitonic sort of (orderable key bits,
-// index) pairs in LDS.  Indices are unique, so the network yields exactly the stable ascending
-// order that the rank kernel above computes, in O(M log^2 M) instead of O(M^2) work.
-__global__ __launch_bounds__(kSortBlock) void sort_select_kernel(const double* __restrict__ keys, int M, int M2,
-                                                                 int keep, int32_t* __restrict__ order) {
-  __shared__ uint64_t s_key[kSortMax];
-  __shared__ int32_t s_idx[kSortMax];
-  sort_select_body(keys, M, M2, keep, order, s_key, s_idx);
-}
-
-// One wave per destination row.
+// One wave per destination row; 16-byte copies when `vec` (n % 4 == 0 and 16-byte aligned rows).
 __global__ __launch_bounds__(kRowBlock) void gather_rows_kernel(const int32_t* __restrict__ A,
                                                                 const double* __restrict__ fitA, int P0,
                                                                 const int32_t* __restrict__ B,
                                                                 const double* __restrict__ fitB, int n,
                                                                 const int32_t* __restrict__ src, int rows,
-                                                                int32_t* __restrict__ out, double* __restrict__ out_fit) {
+                                                                int32_t* __restrict__ out, double* __restrict__ out_fit,
+                                                                int vec) {
   const int d = blockIdx.x * (kRowBlock / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (d >= rows) return;
   const int s = src[d];
   const int32_t* row = s < P0 ? A + (size_t)s * n : B + (size_t)(s - P0) * n;
-  for (int x = lane; x < n; x += kWave) out[(size_t)d * n + x] = row[x];
+  if (vec) {
+    const int4* r4 = reinterpret_cast<const int4*>(row);
+    int4* o4 = reinterpret_cast<int4*>(out + (size_t)d * n);
+    for (int x = lane; x < (n >> 2); x += kWave) o4[x] = r4[x];
+  } else {
+    for (int x = lane; x < n; x += kWave) out[(size_t)d * n + x] = row[x];
+  }
   if (out_fit != nullptr && lane == 0) out_fit[d] = s < P0 ? fitA[s] : fitB[s - P0];
 }
 
@@ -250,10 +268,13 @@ extern "C" int tspmi_pmx(const int32_t* parents, int P, int n, const int32_t* cu
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_pmx: negative shape");
   if (P == 0 || n == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(parents && out && (P < 2 || (cut_lo && cut_hi)), "tspmi_pmx: null pointer");
-  const size_t lds = (size_t)4 * n * sizeof(int32_t);
-  TSPMI_CHECK_ARG(lds <= (size_t)kMaxLds, "tspmi_pmx: n=%d exceeds the LDS limit", n);
+  const size_t lds4 = (size_t)4 * n * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds4 <= (size_t)kMaxLds, "tspmi_pmx: n=%d exceeds the LDS limit", n);
+  // The jump table (n more ints) when it fits; above that the chains are walked step by step.
+  const int with_jump = lds4 + (size_t)n * sizeof(int32_t) <= (size_t)kMaxLds ? 1 : 0;
+  const size_t lds = lds4 + (with_jump ? (size_t)n * sizeof(int32_t) : 0);
   hipLaunchKernelGGL(pmx_kernel, dim3((P + 1) / 2), dim3(kRowBlock), lds, (hipStream_t)stream, parents, P, n, cut_lo,
-                     cut_hi, out);
+                     cut_hi, out, with_jump);
   TSPMI_CHECK_LAUNCH("tspmi_pmx launch");
   return TSPMI_OK;
 }
@@ -289,15 +310,9 @@ extern "C" int tspmi_rank_select(const double* keys, int M, int keep, int32_t* o
   TSPMI_CHECK_ARG(M >= 0 && keep >= 0 && keep <= M, "tspmi_rank_select: bad sizes M=%d keep=%d", M, keep);
   if (M == 0 || keep == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(keys && order, "tspmi_rank_select: null pointer");
-  if (M <= kSortMax) {
-    int M2 = 1;
-    while (M2 < M) M2 <<= 1;
-    hipLaunchKernelGGL(sort_select_kernel, dim3(1), dim3(kSortBlock), 0, (hipStream_t)stream, keys, M, M2, keep, order);
-    TSPMI_CHECK_LAUNCH("tspmi_rank_select launch");
-    return TSPMI_OK;
-  }
-  hipLaunchKernelGGL(rank_select_kernel, dim3((M + kRankBlock - 1) / kRankBlock), dim3(kRankBlock), 0,
-                     (hipStream_t)stream, keys, M, keep, order);
+  TSPMI_CHECK_ARG(M <= (1 << 24), "tspmi_rank_select: M=%d above 2^24", M);
+  hipLaunchKernelGGL(rank_count_kernel, dim3((M + kWave - 1) / kWave), dim3(kCountBlock), 0, (hipStream_t)stream, keys,
+                     M, keep, order);
   TSPMI_CHECK_LAUNCH("tspmi_rank_select launch");
   return TSPMI_OK;
 }
@@ -308,8 +323,9 @@ extern "C" int tspmi_gather_rows(const int32_t* A, const double* fitA, int P0, c
   if (rows == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(src && out && (A || P0 == 0), "tspmi_gather_rows: null pointer");
   TSPMI_CHECK_ARG(out_fit == nullptr || (fitA || P0 == 0), "tspmi_gather_rows: out_fit needs source fitness");
+  const int vec = (n % 4 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && ((uintptr_t)out % 16 == 0);
   hipLaunchKernelGGL(gather_rows_kernel, dim3(row_blocks(rows)), dim3(kRowBlock), 0, (hipStream_t)stream, A, fitA, P0,
-                     B, fitB, n, src, rows, out, out_fit);
+                     B, fitB, n, src, rows, out, out_fit, vec);
   TSPMI_CHECK_LAUNCH("tspmi_gather_rows launch");
   return TSPMI_OK;
 }

@@ -489,12 +489,47 @@ template <int BLOCK, int KP, int UNROLL>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict__ tours, int n,
                                                              const int32_t* __restrict__ D, int ld,
                                                              const int2* __restrict__ ct, int k, int max_iter,
-                                                             int64_t* __restrict__ stats) {
+                                                             int64_t* __restrict__ stats, double* __restrict__ cost) {
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, MatrixDist{D, ld}, ct, k, max_iter, stats,
-                                       lds, red, red64);
+                                       lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
+}
+
+// One tour per workgroup of 1024 threads, pairs in registers (reg_two_opt_tour).
+template <int PP>
+__global__ __launch_bounds__(1024) void two_opt_reg_kernel(int32_t* __restrict__ tours, int n,
+                                                           const int32_t* __restrict__ D, int ld,
+                                                           const int2* __restrict__ ct, int kp, int k, int max_iter,
+                                                           int64_t* __restrict__ stats, double* __restrict__ cost) {
+  extern __shared__ __align__(16) int32_t lds[];
+  __shared__ uint64_t red[1024 / kWave];
+  __shared__ int64_t red64[1024 / kWave];
+  reg_two_opt_tour<1024, PP>(tours + (size_t)blockIdx.x * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, stats,
+                             cost != nullptr ? cost + blockIdx.x : nullptr, lds, red, red64);
+}
+
+// G workgroups of kClusterBlock threads per tour (cluster_two_opt_tour); tour t is blocks t*G ..
+// t*G + G - 1.
+constexpr int kClusterBlock = 256;
+
+template <int PP>
+__global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t* __restrict__ tours, int n,
+                                                                        const int32_t* __restrict__ D, int ld,
+                                                                        const int2* __restrict__ ct, int kp, int k,
+                                                                        int max_iter, int G, uint64_t* __restrict__ sync,
+                                                                        int64_t* __restrict__ stats,
+                                                                        double* __restrict__ cost) {
+  extern __shared__ __align__(16) int32_t lds[];
+  __shared__ uint64_t red[kClusterBlock / kWave];
+  __shared__ int64_t red64[kClusterBlock / kWave];
+  __shared__ uint64_t s_best[1];
+  const int t = blockIdx.x / G;
+  const int g = blockIdx.x - t * G;
+  cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, G, g,
+                                          sync + (size_t)t * kClusterWords, stats,
+                                          cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best);
 }
 
 // Packed candidate table: out[u, r] = {N[u, r], D[u, N[u, r]]} for r < k, {-1, 0} for k <= r < KP.
@@ -768,12 +803,17 @@ extern "C" int tspmi_candidate_table(const int32_t* cand, int n, int k, const in
   return TSPMI_OK;
 }
 
-extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table,
-                                    int k, int max_iter, int64_t* stats, void* stream) {
+extern "C" int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int32_t* D, int ld,
+                                         const int32_t* table, int k, int max_iter, int64_t* stats, double* cost,
+                                         void* stream) {
   using namespace tspmi;
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_packed: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_packed: k=%d outside [1, 32]", k);
-  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;
+  if (P == 0) return TSPMI_OK;
+  if (n < 4 || max_iter == 0) {
+    // Nothing moves; the costs are still asked for.
+    return cost != nullptr ? tspmi_tour_cost(tours, P, n, D, ld, 1, cost, stream) : TSPMI_OK;
+  }
   TSPMI_CHECK_ARG(tours && D && table, "tspmi_two_opt_packed: null pointer");
   TSPMI_CHECK_ARG(ld >= n && (long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed: bad ld/n/k");
   TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0, "tspmi_two_opt_packed: table must be 8-byte aligned");
@@ -786,9 +826,25 @@ extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t*
   // up to two per CU get 8 waves each, more get 4.
   const int cus = device_cu_count();
   const int block = P <= cus ? 1024 : (P <= 2 * cus ? 512 : 256);
+  // One tour per CU and at most 20 pairs per thread: the register-resident form.
+  const int pp = (n * k + 1023) / 1024;
+  const size_t reg_lds = (size_t)(5 * n + 1) * sizeof(int32_t);
+  if (block == 1024 && pp <= 20 && n <= (1 << kRegJBits) && reg_lds + 1024 <= (size_t)kMaxLds) {
+#define TSPMI_LAUNCH_REG(PPV)                                                                                       \
+  hipLaunchKernelGGL((two_opt_reg_kernel<PPV>), dim3(P), dim3(1024), reg_lds, s, tours, n, D, ld, ct, kp, k, max_iter, \
+                     stats, cost)
+    if (pp <= 4) TSPMI_LAUNCH_REG(4);
+    else if (pp <= 8) TSPMI_LAUNCH_REG(8);
+    else if (pp <= 12) TSPMI_LAUNCH_REG(12);
+    else if (pp <= 16) TSPMI_LAUNCH_REG(16);
+    else TSPMI_LAUNCH_REG(20);
+#undef TSPMI_LAUNCH_REG
+    TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed launch");
+    return TSPMI_OK;
+  }
 #define TSPMI_LAUNCH_CAND(BL, KPV, U)                                                                           \
   hipLaunchKernelGGL((two_opt_cand_kernel<BL, KPV, U>), dim3(P), dim3(BL), lds, s, tours, n, D, ld, ct, k, max_iter, \
-                     stats)
+                     stats, cost)
   // (A 16-way unroll at 1024 threads was measured slower on 16-tour elite calls: 16 vs 14.5 us per pass.)
   // (Holding the table in LDS for these launches saved 7 % per pass but cost 5.6 us per launch to
   // fill, measured on 16-tour elite calls; not kept.)
@@ -807,5 +863,44 @@ extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t*
   }
 #undef TSPMI_LAUNCH_CAND
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table,
+                                    int k, int max_iter, int64_t* stats, void* stream) {
+  return tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, nullptr, stream);
+}
+
+extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                                   int max_iter, int64_t* stats, double* cost, uint64_t* sync, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_elite: negative shape");
+  TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_elite: k=%d outside [1, 32]", k);
+  const int cus = device_cu_count();
+  // Largest cluster (up to 16 workgroups) that keeps one workgroup per CU over all tours, if its
+  // share of pairs fits 8 registers per thread.
+  int G = P > 0 ? cus / P : 0;
+  G = G > 16 ? 16 : G;
+  const long pairs = (long)n * k;
+  const long per = G > 0 ? (pairs + (long)G * kClusterBlock - 1) / ((long)G * kClusterBlock) : 1L << 30;
+  const size_t lds = (size_t)(5 * n + 1) * sizeof(int32_t);
+  const bool cluster = P > 0 && n >= 4 && max_iter != 0 && G >= 2 && per <= 8 && n <= (1 << kRegJBits) &&
+                       pairs <= (1L << (32 - kRegJBits)) && lds + 1024 <= (size_t)kMaxLds;
+  if (!cluster) return tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, stream);
+  TSPMI_CHECK_ARG(tours && D && table && sync, "tspmi_two_opt_elite: null pointer");
+  TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt_elite: ld=%d < n=%d", ld, n);
+  TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0,
+                  "tspmi_two_opt_elite: table and sync must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const int2* ct = reinterpret_cast<const int2*>(table);
+  const int kp = cand_lanes(k);
+#define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                 \
+  hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, ct, kp, \
+                     k, max_iter, G, sync, stats, cost)
+  if (per <= 2) TSPMI_LAUNCH_CLUSTER(2);
+  else if (per <= 4) TSPMI_LAUNCH_CLUSTER(4);
+  else TSPMI_LAUNCH_CLUSTER(8);
+#undef TSPMI_LAUNCH_CLUSTER
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite launch");
   return TSPMI_OK;
 }

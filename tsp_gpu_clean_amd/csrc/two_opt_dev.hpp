@@ -69,10 +69,12 @@ __device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node)
 // division; UNROLL steps are issued together so their loads overlap.  Within a lane the pairs
 // are visited in increasing (i, r) scan order, so a strict `<` keeps the lane's first minimum and
 // the packed (delta, i*k + r) key restores the reference tie-break across lanes.
+//
+// `cost` (nullable) receives the closed cost of the final tour, the sum of its edge costs.
 template <int BLOCK, int KP, int UNROLL, class Dist>
 __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                   int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
-                                  int64_t* red64) {
+                                  int64_t* red64, double* __restrict__ cost = nullptr) {
   constexpr int NW = BLOCK / kWave;
   constexpr int RPW = kWave / KP;  // rows per wave step
   int32_t* s_tour = lds;           // n + 1 (s_tour[n] mirrors s_tour[0])
@@ -157,9 +159,331 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
       atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
     }
   }
+  if (cost != nullptr) {
+    int64_t part = 0;
+    for (int x = tid; x < n; x += BLOCK) part += s_edge[x];
+    const int64_t total = block_sum_i64(part, red64);
+    if (tid == 0) *cost = (double)total;
+  }
+}
+
+// Candidate-list steepest 2-opt for a few tours at a time (the GA's elite set), with the pairs in
+// registers and the tour in node-indexed LDS.
+//
+// Same pairs, deltas, tie-break and move as cand_two_opt_tour.  With one tour per workgroup and
+// pass after pass on it, that kernel's scan is a chain of two dependent global loads (candidate
+// row, then D[b][d]) per UNROLL rows and sets the pass time.  Here the (node, candidate) pairs
+// never change, so each thread loads its PP pairs once: pair p = tid + m * BLOCK is node a = p / k,
+// rank r = p % k, kept as (a | c << 16) and D[a][c].  LDS holds s_tour (position -> node, n + 1
+// entries) and one int4 per node {position, successor, D[node][successor], 0}, so a pair costs two
+// 16-byte LDS reads and one gather D[b][d] per pass, all issued together.  Pairs are not visited
+// in scan order, so a thread keeps the minimum of the whole key (delta, i * k + r); the move's j
+// rides in the key's low 13 bits under the scan index (n <= 8192, n * k <= 2^19), which leaves the
+// order unchanged and gives every thread the move straight from the reduction.
+//
+// LDS: 20 n + 4 bytes.  Requires n * k <= PP * BLOCK; at BLOCK = 1024, PP = 20 is the most that
+// stays within 128 VGPRs without spilling.
+constexpr int kRegJBits = 13;
+
+template <int BLOCK, int PP, class Dist>
+__device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+                                 int kp, int k, int max_iter, int64_t* __restrict__ stats, double* __restrict__ cost,
+                                 int32_t* lds, uint64_t* red, int64_t* red64) {
+  int4* s_nd = reinterpret_cast<int4*>(lds);  // n
+  int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
+  const int tid = threadIdx.x;
+  for (int x = tid; x < n; x += BLOCK) {
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  uint32_t ac[PP];
+  int dac[PP];
+  const int total = n * k;
+  // (a, r) of pair tid + m * BLOCK by stepping quotient and remainder (one division, not PP: a
+  // runtime integer division is a long instruction sequence); all PP table loads are issued
+  // before any is tested (a branch per load would serialise them).
+  int2 e[PP];
+  int pa[PP];
+  {
+    const int qb = BLOCK / k, rb = BLOCK - qb * k;
+    int a = tid / k, r = tid - a * k;
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      pa[m] = a;
+      const int p = tid + m * BLOCK;
+      e[m] = ct[(size_t)(p < total ? a : 0) * kp + (p < total ? r : 0)];
+      a += qb;
+      r += rb;
+      if (r >= k) {
+        r -= k;
+        ++a;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PP; ++m) {
+    const int p = tid + m * BLOCK;
+    const bool ok = p < total && e[m].x >= 0 && e[m].x < n;
+    ac[m] = ok ? ((uint32_t)pa[m] | ((uint32_t)e[m].x << 16)) : 0xffffffffu;
+    dac[m] = ok ? e[m].y : 0;
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += BLOCK) {
+    const int a = s_tour[x], b = s_tour[x + 1];
+    s_nd[a] = make_int4(x, b, dist(a, b), 0);
+  }
+  __syncthreads();
+
+  const int rows = n - 2;
+  int64_t my_deltas = 0;
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = kNoMove;
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      const uint32_t w = ac[m];
+      const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
+      const int4 na = s_nd[w == 0xffffffffu ? 0 : a];
+      const int4 nc = s_nd[w == 0xffffffffu ? 0 : c];
+      const int i = na.x, j = nc.x;
+      const int lim = (i == 0) ? n - 1 : n;
+      const bool ok = (w != 0xffffffffu) & ((unsigned)i < (unsigned)rows) & (j > i + 1) & (j < lim);
+      const int delta = dac[m] + dist(na.y, nc.y) - na.z - nc.z;
+      my_deltas += ok;
+      if (ok & (delta < 0)) {
+        const uint32_t r = (uint32_t)(tid + m * BLOCK - a * k);
+        const uint64_t key = pack_key(delta, (((uint32_t)i * (uint32_t)k + r) << kRegJBits) | (uint32_t)j);
+        best = key < best ? key : best;
+      }
+    }
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)((idx >> kRegJBits) / (uint32_t)k);
+    const int j = (int)(idx & ((1u << kRegJBits) - 1u));
+    // Reverse s_tour[i+1 .. j] (s_tour[0] and its mirror never move), then refresh the records of
+    // the nodes now at positions i .. j.
+    const int half = (j - i) >> 1;
+    for (int t = tid; t < half; t += BLOCK) {
+      const int lo = i + 1 + t, hi = j - t;
+      const int32_t tmp = s_tour[lo];
+      s_tour[lo] = s_tour[hi];
+      s_tour[hi] = tmp;
+    }
+    __syncthreads();
+    for (int x = i + tid; x <= j; x += BLOCK) {
+      const int u = s_tour[x], v = s_tour[x + 1];
+      s_nd[u] = make_int4(x, v, dist(u, v), 0);
+    }
+    __syncthreads();
+    ++moves;
+  }
+  for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
+  if (stats != nullptr) {
+    const int64_t tot = block_sum_i64(my_deltas, red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)tot);
+      atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+      atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+    }
+  }
+  if (cost != nullptr) {
+    int64_t part = 0;
+    for (int x = tid; x < n; x += BLOCK) part += s_nd[x].z;
+    const int64_t tot = block_sum_i64(part, red64);
+    if (tid == 0) *cost = (double)tot;
+  }
 }
 
 
 
+
+// Candidate-list steepest 2-opt of one tour by a cluster of G workgroups.
+//
+// Same pairs, deltas, tie-break and moves as reg_two_opt_tour.  With one workgroup per tour a
+// pass runs on one CU, which is issue- and gather-bound (20000 pairs at n = 1000, k = 20) while
+// the other CUs idle; here G workgroups share the tour's pairs (pair p = g * BLOCK + tid + m * G *
+// BLOCK, kept in registers as before) and each holds its own copy of the tour in LDS.  Per pass
+// every workgroup publishes its best key in its own line of the tour's workspace and then raises
+// its own pass flag (the key store has completed first); wave 0 of every workgroup polls the G
+// flags, one lane per flag, until all show this pass, reads the G keys and takes their minimum,
+// and every workgroup applies that move to its own copy.  Nothing is contended: each word has one
+// writer, on its own 128-byte line.  A workgroup's key of pass t goes to slot t & 1: it can only
+// be in pass t once every other workgroup has flagged pass t - 1, which each does after reading
+// the keys of pass t - 2.  The wait is bounded: a workgroup that spins kClusterSpinLimit times
+// (its cluster was not co-resident) raises the tour's error word, the others see it and leave,
+// and the tour is not written; the caller reads the word.  The last workgroup to leave (a
+// departure counter) clears the flags and the counter for the next launch.
+//
+// Workspace per tour (int64 words, lines of 16): line g < G = flag of workgroup g, line G + g =
+// keys of workgroup g (2 words), line 2 * kClusterMaxG = {error, departures}.
+constexpr int kClusterMaxG = 16;
+constexpr int kClusterLine = 16;
+constexpr int kClusterWords = (2 * kClusterMaxG + 1) * kClusterLine;
+constexpr unsigned kClusterSpinLimit = 1u << 22;
+constexpr uint64_t kClusterFail = ~1ull;  // never a key: a key's delta is negative
+
+__device__ __forceinline__ uint64_t load_relaxed(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_relaxed(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int BLOCK, int PP, class Dist>
+__device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+                                     int kp, int k, int max_iter, int G, int g, uint64_t* __restrict__ ws,
+                                     int64_t* __restrict__ stats, double* __restrict__ cost, int32_t* lds,
+                                     uint64_t* red, int64_t* red64, uint64_t* s_best) {
+  int4* s_nd = reinterpret_cast<int4*>(lds);  // n
+  int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
+  const int tid = threadIdx.x;
+  for (int x = tid; x < n; x += BLOCK) {
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  const int total = n * k;
+  const int base = g * BLOCK + tid;
+  const int stride = G * BLOCK;
+  uint32_t ac[PP];
+  int dac[PP];
+  {
+    int2 e[PP];
+    int pa[PP];
+    const int qb = stride / k, rb = stride - qb * k;
+    int a = base / k, r = base - a * k;
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      pa[m] = a;
+      const int p = base + m * stride;
+      e[m] = ct[(size_t)(p < total ? a : 0) * kp + (p < total ? r : 0)];
+      a += qb;
+      r += rb;
+      if (r >= k) {
+        r -= k;
+        ++a;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      const int p = base + m * stride;
+      const bool ok = p < total && e[m].x >= 0 && e[m].x < n;
+      ac[m] = ok ? ((uint32_t)pa[m] | ((uint32_t)e[m].x << 16)) : 0xffffffffu;
+      dac[m] = ok ? e[m].y : 0;
+    }
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += BLOCK) {
+    const int a = s_tour[x], b = s_tour[x + 1];
+    s_nd[a] = make_int4(x, b, dist(a, b), 0);
+  }
+  __syncthreads();
+
+  const int rows = n - 2;
+  int64_t my_deltas = 0;
+  int moves = 0, passes = 0;
+  bool failed = false;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = kNoMove;
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      const uint32_t w = ac[m];
+      const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
+      const int4 na = s_nd[w == 0xffffffffu ? 0 : a];
+      const int4 nc = s_nd[w == 0xffffffffu ? 0 : c];
+      const int i = na.x, j = nc.x;
+      const int lim = (i == 0) ? n - 1 : n;
+      const bool ok = (w != 0xffffffffu) & ((unsigned)i < (unsigned)rows) & (j > i + 1) & (j < lim);
+      const int delta = dac[m] + dist(na.y, nc.y) - na.z - nc.z;
+      my_deltas += ok;
+      if (ok & (delta < 0)) {
+        const uint32_t r = (uint32_t)(base + m * stride - a * k);
+        const uint64_t key = pack_key(delta, (((uint32_t)i * (uint32_t)k + r) << kRegJBits) | (uint32_t)j);
+        best = key < best ? key : best;
+      }
+    }
+    best = block_min_u64(best, red);
+    uint64_t* err = ws + 2 * kClusterMaxG * kClusterLine;
+    if (tid == 0) {
+      store_relaxed(ws + (G + g) * kClusterLine + (it & 1), best);
+      __builtin_amdgcn_s_waitcnt(0);  // the key is stored before the flag that announces it
+      store_relaxed(ws + g * kClusterLine, (uint64_t)(it + 1));
+    }
+    if (tid < kWave) {
+      const int lane = tid;
+      bool fail = false;
+      for (unsigned spin = 0;; ++spin) {
+        const uint64_t f = lane < G ? load_relaxed(ws + lane * kClusterLine) : ~0ull;
+        if (__all(f >= (uint64_t)(it + 1))) break;
+        if ((spin & 63) == 63 && (load_relaxed(err) != 0 || spin >= kClusterSpinLimit)) {
+          fail = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      uint64_t key = (!fail && lane < G) ? load_relaxed(ws + (G + lane) * kClusterLine + (it & 1)) : kNoMove;
+      key = wave_min_u64(key);
+      if (lane == 0) {
+        if (fail) store_relaxed(err, 1ull);
+        s_best[0] = fail ? kClusterFail : key;
+      }
+    }
+    __syncthreads();
+    const uint64_t all = s_best[0];
+    if (all == kClusterFail) {
+      failed = true;
+      break;
+    }
+    if (all == kNoMove) break;
+    const uint32_t idx = key_index(all);
+    const int i = (int)((idx >> kRegJBits) / (uint32_t)k);
+    const int j = (int)(idx & ((1u << kRegJBits) - 1u));
+    const int half = (j - i) >> 1;
+    for (int t = tid; t < half; t += BLOCK) {
+      const int lo = i + 1 + t, hi = j - t;
+      const int32_t tmp = s_tour[lo];
+      s_tour[lo] = s_tour[hi];
+      s_tour[hi] = tmp;
+    }
+    __syncthreads();
+    for (int x = i + tid; x <= j; x += BLOCK) {
+      const int u = s_tour[x], v = s_tour[x + 1];
+      s_nd[u] = make_int4(x, v, dist(u, v), 0);
+    }
+    __syncthreads();
+    ++moves;
+  }
+  if (!failed && g == 0) {
+    for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
+    if (cost != nullptr) {
+      int64_t part = 0;
+      for (int x = tid; x < n; x += BLOCK) part += s_nd[x].z;
+      const int64_t tot = block_sum_i64(part, red64);
+      if (tid == 0) *cost = (double)tot;
+    }
+  }
+  if (stats != nullptr) {
+    const int64_t tot = block_sum_i64(my_deltas, red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)tot);
+      if (g == 0) {
+        atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+        atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+      }
+    }
+  }
+  if (tid == 0) {
+    uint64_t* dep = ws + 2 * kClusterMaxG * kClusterLine + 1;
+    const uint64_t done = __hip_atomic_fetch_add(dep, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (done == (uint64_t)(G - 1)) {
+      for (int w = 0; w < G; ++w) store_relaxed(ws + w * kClusterLine, 0ull);
+      store_relaxed(dep, 0ull);
+    }
+  }
+}
 
 }  // namespace tspmi

@@ -118,12 +118,13 @@ template <int BLOCK, int KP, int UNROLL>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n,
                                                                 const double2* __restrict__ xy,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
-                                                                int64_t* __restrict__ stats) {
+                                                                int64_t* __restrict__ stats,
+                                                                double* __restrict__ cost) {
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, CoordDist{xy}, ct, k, max_iter, stats, lds,
-                                       red, red64);
+                                       red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
 }
 
 }  // namespace tspmi
@@ -157,11 +158,14 @@ extern "C" int tspmi_two_opt_euc2d(int32_t* tours, int P, int n, const double* x
   return TSPMI_OK;
 }
 
-extern "C" int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
-                                          int max_iter, int64_t* stats, void* stream) {
+extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
+                                               int k, int max_iter, int64_t* stats, double* cost, void* stream) {
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_packed_euc2d: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_packed_euc2d: k=%d outside [1, 32]", k);
-  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;
+  if (P == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(cost == nullptr || (n >= 4 && max_iter != 0),
+                  "tspmi_two_opt_packed_euc2d: costs need n >= 4 and max_iter != 0");
+  if (n < 4 || max_iter == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(tours && xy && table, "tspmi_two_opt_packed_euc2d: null pointer");
   TSPMI_CHECK_ARG(((uintptr_t)xy & 15) == 0 && ((uintptr_t)table & 7) == 0,
                   "tspmi_two_opt_packed_euc2d: xy must be 16-byte and table 8-byte aligned");
@@ -176,7 +180,7 @@ extern "C" int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const do
   const int block = P <= cus ? 1024 : (P <= 2 * cus ? 512 : 256);
 #define TSPMI_CXY(BL, KPV)                                                                                         \
   hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
-                     stats)
+                     stats, cost)
   if (block == 1024) {
     if (kp == 8) TSPMI_CXY(1024, 8);
     else if (kp == 16) TSPMI_CXY(1024, 16);
@@ -193,4 +197,9 @@ extern "C" int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const do
 #undef TSPMI_CXY
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d launch");
   return TSPMI_OK;
+}
+
+extern "C" int tspmi_two_opt_packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
+                                          int max_iter, int64_t* stats, void* stream) {
+  return tspmi_two_opt_packed_euc2d_cost(tours, P, n, xy, table, k, max_iter, stats, nullptr, stream);
 }

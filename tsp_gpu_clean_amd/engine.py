@@ -11,7 +11,8 @@ step-by-step ``GeneticAlgorithm`` pipeline.  Per generation the device runs:
   1. tspmi_generation  : fused tournament -> PMX -> inversion -> cost into the offspring buffer
   2. tspmi_rank_select : stable (mu + lambda) order of concat(fit, off_fit)
   3. tspmi_gather_rows : survivors into the other population buffer (ping-pong)
-  4. elite 2-opt       : tspmi_two_opt_packed + tspmi_tour_cost on the sorted survivors' prefix
+  4. elite 2-opt       : tspmi_two_opt_elite (search and re-score in one launch, one cluster of
+                         workgroups per tour) on the sorted survivors' prefix
   5. rank_select(1)    : index of the generation's best row, kept on device for the history
 Whole runs of generations can be captured into one HIP graph (``capture``).
 """
@@ -99,6 +100,8 @@ class DeviceMemeticLoop:
     self.off = torch.empty((P, n), dtype=torch.int32, device=dev)
     self.order = torch.empty(P, dtype=torch.int32, device=dev)
     self.stats = torch.zeros(3, dtype=torch.int64, device=dev)
+    # Barrier words of the elite search's workgroup clusters (left zeroed by every launch).
+    self.sync = torch.zeros(_native.ELITE_SYNC_WORDS * max(1, self.elite or 1), dtype=torch.int64, device=dev)
     self.cur = 0
 
   def load(self, population, fitness=None) -> None:
@@ -136,16 +139,26 @@ class DeviceMemeticLoop:
     if self.elite:
       E = self.elite
       elites = self.pop[b][:E]
-      if self.table is not None and self.xy is not None:
-        _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats)
-      elif self.table is not None:
-        _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
+      fused = self.table is not None and self.n >= 4 and self.ls_iters != 0
+      if fused and self.xy is not None:
+        _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
+      elif fused:
+        _native.two_opt_elite(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E], self.sync)
       else:
-        _native.two_opt(elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
-      _native.tour_cost(elites, self.D, kb[:E])
+        if self.table is not None:
+          _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
+        else:
+          _native.two_opt(elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
+        _native.tour_cost(elites, self.D, kb[:E])
     if best_out is not None:
       _native.rank_select(kb[: (self.elite or 1)], 1, best_out)
     self.cur = b
+
+  def verify(self) -> None:
+    """Raise if an elite search's workgroup cluster could not meet (its result was not written);
+    call after the work has been issued, outside graph capture (reads the device)."""
+    if _native.elite_failed(self.sync):
+      raise RuntimeError("elite 2-opt: a workgroup cluster was not co-resident; results are invalid")
 
   def run(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0) -> None:
     """Run ``generations`` steps from draw slot ``start``; ``best_idx`` collects each best row."""
