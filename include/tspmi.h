@@ -123,7 +123,8 @@ int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int32_t* D, in
  * tspmi_two_opt_packed_cost for a few tours (the GA's elite set): when P tours leave CUs idle,
  * each tour is searched by a cluster of up to 16 workgroups that share its candidate pairs and
  * meet once per pass at a bounded barrier in `sync`; otherwise this is tspmi_two_opt_packed_cost.
- * Same tours, moves, counters and costs either way.
+ * Same tours, moves, counters and costs either way.  best (device int32[1], may be NULL; needs
+ * cost) receives the index of the first minimum of the P costs, as tspmi_rank_select(cost, P, 1).
  *   sync: device int64[528 * P], zeroed before the first call; every call leaves it ready for the
  *         next (its flags and counters cleared).  Word 528 p + 512 is set when tour p's cluster
  *         was not co-resident and its barrier timed out (that tour and its cost are then left
@@ -133,7 +134,7 @@ int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int32_t* D, in
  *           two_opt.py:831-899 in candidate mode on the elite rows.
  */
 int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
-                        int max_iter, int64_t* stats, double* cost, uint64_t* sync, void* stream);
+                        int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync, void* stream);
 
 /*
  * EUC_2D variants that compute every distance from the coordinates xy (n, 2) float64 (16-byte
@@ -196,7 +197,7 @@ int tspmi_swap(int32_t* routes, int P, int n, const double* probs, const int32_t
                double rate, void* stream);
 
 /*
- * Fused generation: tournament -> PMX -> inversion -> evaluate, one workgroup per child pair.
+ * Fused generation: tournament -> PMX -> inversion -> evaluate, one workgroup per child row.
  * Inputs are the host-drawn buffers of one generation: tour_cand (P, k), cut_lo/cut_hi (P/2),
  * probs/pos1/pos2 (P).  Writes offspring (P, n) and their costs off_fit (P).
  * Replaces: the SelectionStep -> CrossoverStep -> MutationStep -> EvaluationStep sequence of
@@ -294,7 +295,7 @@ int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_inst, const
 int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
                      const int32_t* pop, const int32_t* D_all, double* keys, void* stream);
 
-/* Fused generation of every instance: items are (instance, child pair q); block in
+/* Fused generation of every instance: items are (instance, child row r); block in
  * {128, 256, 512, 1024}; grid <= 0 means one workgroup per item, else a persistent grid.
  * Replaces genetic_algorithm.py:120-213 per instance. */
 int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,

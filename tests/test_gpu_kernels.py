@@ -498,13 +498,15 @@ def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters)
     sw = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
     sg = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
     cost = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
+    best = torch_mod.full((1,), -1, dtype=torch_mod.int32, device="cuda")
     nat.two_opt_packed(want, D, table, kk, iters, sw)
-    nat.two_opt_elite(got, D, table, kk, iters, sg, cost, sync)
+    nat.two_opt_elite(got, D, table, kk, iters, sg, cost, sync, best)
     torch_mod.cuda.synchronize()
+    assert int(best.item()) == int(np.argmin(cost.cpu().numpy()))
     assert not nat.elite_failed(sync)
     words = sync.view(P, nat.ELITE_SYNC_WORDS)
-    assert not words[:, 0:256:16].any().item(), f"call {call}: pass flags not cleared"
-    assert not words[:, 512:514].any().item(), f"call {call}: error / departure words not cleared"
+    assert not words[:, 0:256:16].any().item() and not words[:, 1:256:16].any().item(), f"call {call}: slots not cleared"
+    assert not words[:, 512:515].any().item(), f"call {call}: error / departure / count words not cleared"
     np.testing.assert_array_equal(got.cpu().numpy(), want.cpu().numpy(), err_msg=f"call {call}")
     assert sg.cpu().tolist() == sw.cpu().tolist()
     ref = torch_mod.empty(P, dtype=torch_mod.float64, device="cuda")

@@ -186,7 +186,7 @@ def test_device_loop_matches_step_pipeline_and_oracle(t):
     np.testing.assert_array_equal(loop.population.cpu().numpy(), res.populations[g], err_msg=f"generation {g + 1}")
     np.testing.assert_array_equal(loop.fitness.cpu().numpy(), res.fitnesses[g])
   loop.verify()
-  assert not loop.sync.view(-1, 528)[:, list(range(0, 256, 16)) + [512, 513]].any().item()  # barrier words cleared
+  assert not loop.sync.view(-1, 528)[:, list(range(0, 256, 16)) + list(range(1, 256, 16)) + [512, 513, 514]].any().item()
   assert ctx.rng.bit_generator.state == rng.bit_generator.state
 
 

@@ -36,7 +36,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
-  "tspmi_two_opt_elite": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P],
+  "tspmi_two_opt_elite": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P],
   "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
@@ -274,11 +274,12 @@ ELITE_SYNC_WORDS = 528  # int64 words of the elite 2-opt workspace per tour
 ELITE_ERROR_WORD = 512  # its barrier-timeout word
 
 
-def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync) -> None:
+def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync, best=None) -> None:
   """Elite-set candidate 2-opt with final costs: :func:`two_opt_packed` semantics, one cluster of
   workgroups per tour when the tours leave CUs idle.  ``sync`` is an int64 device workspace of
   ``ELITE_SYNC_WORDS * P`` words, zeroed before first use; :func:`elite_failed` reports a cluster
-  whose barrier timed out (its tour and cost are then left unchanged)."""
+  whose barrier timed out (its tour and cost are then left unchanged).  ``best`` (int32 (1,),
+  optional) receives the index of the first minimum cost."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   _need(D, torch.int32, "D", 2)
@@ -295,8 +296,13 @@ def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync) -> 
       raise ValueError("two_opt_elite: cost shorter than the number of tours")
   if sync.shape[0] < ELITE_SYNC_WORDS * P:
     raise ValueError(f"two_opt_elite: sync needs {ELITE_SYNC_WORDS * P} words")
+  if best is not None:
+    _need(best, torch.int32, "best", 1)
+    if cost is None:
+      raise ValueError("two_opt_elite: best needs cost")
   _check(_lib.tspmi_two_opt_elite(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
-                                  _ptr(stats), _ptr(cost), _ptr(sync), _stream(torch)), "tspmi_two_opt_elite")
+                                  _ptr(stats), _ptr(cost), _ptr(best), _ptr(sync), _stream(torch)),
+         "tspmi_two_opt_elite")
 
 
 def elite_failed(sync) -> bool:

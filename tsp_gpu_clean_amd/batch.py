@@ -94,7 +94,6 @@ class BatchedMemeticGA:
       return _cu(inst), _cu(idx), int(inst.size)
 
     self.w_dist = items(ns)
-    self.w_pair = items(np.full(B, (P + 1) // 2))
     self.w_row = items(np.full(B, P))
     self.w_elite = items(np.full(B, self.elite))
     xy = np.concatenate([np.ascontiguousarray(c, dtype=np.float64).reshape(-1) for c in coords])
@@ -168,7 +167,7 @@ class BatchedMemeticGA:
 
   # ---- one generation ---------------------------------------------------------------------------
   def _generation(self, a, d: BatchDraws, g: int, block: int, grid: int) -> None:
-    _native.batch_call("tspmi_batch_generation", self.desc, *self.w_pair, self.max_n, self.pop[a], self.keys[a], self.D,
+    _native.batch_call("tspmi_batch_generation", self.desc, *self.w_row, self.max_n, self.pop[a], self.keys[a], self.D,
                        d.tour_cand[g], self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g],
                        self.rate, self.off, block, grid)
 

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restri
   if (lane == 0) keys[2 * I.row + r] = (double)s;
 }
 
-// Fused generation of every instance: one workgroup per (instance, child pair), persistent grid.
+// Fused generation of every instance: one workgroup per (instance, child row), persistent grid.
 __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst,
                                         const int32_t* __restrict__ item_idx, int nitems,
                                         const int32_t* __restrict__ popA, double* __restrict__ keysA,
@@ -154,9 +154,9 @@ __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Inst I = load_inst(desc, item_inst[it]);
     double* keys = keysA + 2 * I.row;
-    generation_pair(item_idx[it], popA + I.pop, keys, I.P, I.n, D_all + I.d, I.n, tour_cand + (size_t)I.row * k, k,
-                    cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
-                    keys + I.P, lds, red, s_w);
+    generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n, D_all + I.d, I.n, tour_cand + (size_t)I.row * k, k,
+                     cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
+                     keys + I.P, lds, red, s_w);
     __syncthreads();
   }
 }

@@ -148,38 +148,41 @@ __device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const in
   if (threadIdx.x == 0) off_fit[r] = cost;
 }
 
-// Fused tournament -> PMX -> inversion -> evaluate of child pair q (rows 2q, 2q+1) by one
-// workgroup.  LDS: p1, p2, inv1, inv2, childA, childB (6n ints) in `lds`; `red` holds one int64
-// per wave and `s_w` two ints.  Child A is finished and stored before child B is built, so each
-// child's PMX borrows the other child's buffer for its jump table.
-__device__ void generation_pair(int q, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P, int n,
-                                const int32_t* __restrict__ D, int ld, const int32_t* __restrict__ tour_cand, int k,
-                                const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
-                                const double* __restrict__ probs, const int32_t* __restrict__ pos1,
-                                const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ offspring,
-                                double* __restrict__ off_fit, int32_t* lds, int64_t* red, int* s_w) {
+// Fused tournament -> PMX -> inversion -> evaluate of child row r (pair q = r / 2, which shares
+// its parents and cut with row r ^ 1) by one workgroup.  LDS: p1, p2, inv1, inv2, child, jump
+// table (6n ints) in `lds`; `red` holds one int64 per wave and `s_w` two ints.  Both rows of a
+// pair load both tournament winners; row 2q is PMX(p1, p2), row 2q + 1 is PMX(p2, p1), and an
+// odd last row copies its own winner (pmx_crossover.py:128-171).  One workgroup per child halves
+// the serial chain of a pair's workgroup (two PMX builds, costs and stores).
+__device__ void generation_child(int r, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P,
+                                 int n, const int32_t* __restrict__ D, int ld, const int32_t* __restrict__ tour_cand,
+                                 int k, const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
+                                 const double* __restrict__ probs, const int32_t* __restrict__ pos1,
+                                 const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ offspring,
+                                 double* __restrict__ off_fit, int32_t* lds, int64_t* red, int* s_w) {
+  const int q = r >> 1;
   const int r0 = 2 * q;
   const bool single = (r0 + 1 >= P);
   int32_t* p1 = lds;
   int32_t* p2 = lds + n;
   int32_t* inv1 = lds + 2 * n;
   int32_t* inv2 = lds + 3 * n;
-  int32_t* cA = lds + 4 * n;
-  int32_t* cB = lds + 5 * n;
+  int32_t* ch = lds + 4 * n;
+  int32_t* jump = lds + 5 * n;
 
   if (threadIdx.x < 2 && (threadIdx.x == 0 || !single))
     s_w[threadIdx.x] = tournament_winner(tour_cand + (size_t)(r0 + threadIdx.x) * k, k, fitness, P);
   __syncthreads();
-  load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
-  if (!single) load_parent(pop + (size_t)s_w[1] * n, n, p2, inv2);
-  __syncthreads();
-
   if (single) {
-    for (int x = threadIdx.x; x < n; x += blockDim.x) cA[x] = p1[x];
+    const int32_t* src = pop + (size_t)s_w[0] * n;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) ch[x] = clamp_id(src[x], n);
     __syncthreads();
-    finish_child(r0, cA, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+    finish_child(r, ch, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
     return;
   }
+  load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
+  load_parent(pop + (size_t)s_w[1] * n, n, p2, inv2);
+  __syncthreads();
   int c1 = cut_lo[q], c2 = cut_hi[q];
   if (c1 > c2) {
     const int t = c1;
@@ -188,13 +191,10 @@ __device__ void generation_pair(int q, const int32_t* __restrict__ pop, const do
   }
   c1 = c1 < 0 ? 0 : c1;
   c2 = c2 >= n ? n - 1 : c2;
-  pmx_child(p1, p2, inv1, inv2, n, c1, c2, cA, cB);
+  if (r == r0) pmx_child(p1, p2, inv1, inv2, n, c1, c2, ch, jump);
+  else pmx_child(p2, p1, inv2, inv1, n, c1, c2, ch, jump);
   __syncthreads();
-  finish_child(r0, cA, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
-  __syncthreads();  // child A's last reads are done before its buffer becomes B's jump table
-  pmx_child(p2, p1, inv2, inv1, n, c1, c2, cB, cA);
-  __syncthreads();
-  finish_child(r0 + 1, cB, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+  finish_child(r, ch, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
 }
 
 

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kRowBlock) void inversion_kernel(int32_t* __restric
   }
 }
 
-// Fused tournament -> PMX -> inversion -> evaluate; one workgroup per child pair.
+// Fused tournament -> PMX -> inversion -> evaluate; one workgroup per child row.
 __global__ __launch_bounds__(kRowBlock) void generation_kernel(
     const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P, int n, const int32_t* __restrict__ D,
     int ld, const int32_t* __restrict__ tour_cand, int k, const int32_t* __restrict__ cut_lo,
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(kRowBlock) void generation_kernel(
   extern __shared__ int32_t lds[];
   __shared__ int64_t red[kRowBlock / kWave];
   __shared__ int s_w[2];
-  generation_pair(blockIdx.x, pop, fitness, P, n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring,
-                  off_fit, lds, red, s_w);
+  generation_child(blockIdx.x, pop, fitness, P, n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate,
+                   offspring, off_fit, lds, red, s_w);
 }
 
 // Stable ascending order of M float64 keys: order[rank(i)] = i for rank(i) < keep, with
@@ -300,7 +300,7 @@ extern "C" int tspmi_generation(const int32_t* pop, const double* fitness, int P
                   "tspmi_generation: null pointer");
   const size_t lds = (size_t)6 * n * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 64 <= (size_t)kMaxLds, "tspmi_generation: n=%d exceeds the fused LDS limit", n);
-  hipLaunchKernelGGL(generation_kernel, dim3((P + 1) / 2), dim3(kRowBlock), lds, (hipStream_t)stream, pop, fitness, P,
+  hipLaunchKernelGGL(generation_kernel, dim3(P), dim3(kRowBlock), lds, (hipStream_t)stream, pop, fitness, P,
                      n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, offspring, off_fit);
   TSPMI_CHECK_LAUNCH("tspmi_generation launch");
   return TSPMI_OK;

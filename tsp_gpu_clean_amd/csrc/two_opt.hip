@@ -520,7 +520,8 @@ __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t*
                                                                         const int2* __restrict__ ct, int kp, int k,
                                                                         int max_iter, int G, uint64_t* __restrict__ sync,
                                                                         int64_t* __restrict__ stats,
-                                                                        double* __restrict__ cost) {
+                                                                        double* __restrict__ cost, int P,
+                                                                        int32_t* __restrict__ best) {
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[kClusterBlock / kWave];
   __shared__ int64_t red64[kClusterBlock / kWave];
@@ -529,7 +530,8 @@ __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t*
   const int g = blockIdx.x - t * G;
   cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, G, g,
                                           sync + (size_t)t * kClusterWords, stats,
-                                          cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best);
+                                          cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best, P, sync, cost,
+                                          best);
 }
 
 // Packed candidate table: out[u, r] = {N[u, r], D[u, N[u, r]]} for r < k, {-1, 0} for k <= r < KP.
@@ -872,7 +874,8 @@ extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t*
 }
 
 extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
-                                   int max_iter, int64_t* stats, double* cost, uint64_t* sync, void* stream) {
+                                   int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync,
+                                   void* stream) {
   using namespace tspmi;
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_elite: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_elite: k=%d outside [1, 32]", k);
@@ -885,8 +888,13 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
   const long per = G > 0 ? (pairs + (long)G * kClusterBlock - 1) / ((long)G * kClusterBlock) : 1L << 30;
   const size_t lds = (size_t)(5 * n + 1) * sizeof(int32_t);
   const bool cluster = P > 0 && n >= 4 && max_iter != 0 && G >= 2 && per <= 8 && n <= (1 << kRegJBits) &&
-                       pairs <= (1L << (32 - kRegJBits)) && lds + 1024 <= (size_t)kMaxLds;
-  if (!cluster) return tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, stream);
+                       pairs < (1L << (32 - kRegJBits)) && lds + 1024 <= (size_t)kMaxLds;
+  TSPMI_CHECK_ARG(best == nullptr || cost != nullptr, "tspmi_two_opt_elite: best needs cost");
+  if (!cluster) {
+    const int rc = tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, stream);
+    if (rc != TSPMI_OK || best == nullptr || P == 0) return rc;
+    return tspmi_rank_select(cost, P, 1, best, stream);
+  }
   TSPMI_CHECK_ARG(tours && D && table && sync, "tspmi_two_opt_elite: null pointer");
   TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt_elite: ld=%d < n=%d", ld, n);
   TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0,
@@ -896,7 +904,7 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
   const int kp = cand_lanes(k);
 #define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                 \
   hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, ct, kp, \
-                     k, max_iter, G, sync, stats, cost)
+                     k, max_iter, G, sync, stats, cost, P, best)
   if (per <= 2) TSPMI_LAUNCH_CLUSTER(2);
   else if (per <= 4) TSPMI_LAUNCH_CLUSTER(4);
   else TSPMI_LAUNCH_CLUSTER(8);

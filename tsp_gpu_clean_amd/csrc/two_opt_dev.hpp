@@ -306,24 +306,29 @@ __device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist d
 // pass runs on one CU, which is issue- and gather-bound (20000 pairs at n = 1000, k = 20) while
 // the other CUs idle; here G workgroups share the tour's pairs (pair p = g * BLOCK + tid + m * G *
 // BLOCK, kept in registers as before) and each holds its own copy of the tour in LDS.  Per pass
-// every workgroup publishes its best key in its own line of the tour's workspace and then raises
-// its own pass flag (the key store has completed first); wave 0 of every workgroup polls the G
-// flags, one lane per flag, until all show this pass, reads the G keys and takes their minimum,
-// and every workgroup applies that move to its own copy.  Nothing is contended: each word has one
-// writer, on its own 128-byte line.  A workgroup's key of pass t goes to slot t & 1: it can only
-// be in pass t once every other workgroup has flagged pass t - 1, which each does after reading
-// the keys of pass t - 2.  The wait is bounded: a workgroup that spins kClusterSpinLimit times
-// (its cluster was not co-resident) raises the tour's error word, the others see it and leave,
-// and the tour is not written; the caller reads the word.  The last workgroup to leave (a
-// departure counter) clears the flags and the counter for the next launch.
+// every workgroup publishes its best key as one 64-bit word in its own line of the tour's
+// workspace; wave 0 of every workgroup polls the G words, one lane each, until all carry this
+// pass's tag, takes their minimum, and every workgroup applies that move to its own copy.
+// Nothing is contended (each word has one writer, on its own 128-byte line) and the data is its
+// own flag: a real key has bit 63 clear (its delta is negative), so bit 63 holds a tag, and "no
+// move" is kClusterNone.  Pass t uses slot t & 1 with tag ((t >> 1) + 1) & 1, so consecutive uses
+// of a slot alternate tags and the zeroed workspace (tag 0) never reads as pass 0 or 1.  A slot
+// is rewritten two passes later, by which time every workgroup has read it: a workgroup only
+// reaches pass t after every other one has published pass t - 1, which each does after reading
+// pass t - 2.  The wait is bounded: a workgroup that spins kClusterSpinLimit times (its cluster
+// was not co-resident) raises the tour's error word, the others see it and leave, and the tour
+// is not written; the caller reads the word.  The last workgroup to leave (a departure counter)
+// clears the slots and the counter for the next launch.
 //
-// Workspace per tour (int64 words, lines of 16): line g < G = flag of workgroup g, line G + g =
-// keys of workgroup g (2 words), line 2 * kClusterMaxG = {error, departures}.
+// Workspace per tour (int64 words, lines of 16): line g < G = slots of workgroup g (2 words),
+// line 2 * kClusterMaxG = {error, departures}.
 constexpr int kClusterMaxG = 16;
 constexpr int kClusterLine = 16;
 constexpr int kClusterWords = (2 * kClusterMaxG + 1) * kClusterLine;
 constexpr unsigned kClusterSpinLimit = 1u << 22;
-constexpr uint64_t kClusterFail = ~1ull;  // never a key: a key's delta is negative
+constexpr uint64_t kClusterFail = ~1ull;           // never a key: a key's delta is negative
+constexpr uint64_t kClusterNone = ~0ull >> 1;      // above every key, bit 63 clear
+constexpr uint64_t kClusterTag = 1ull << 63;
 
 __device__ __forceinline__ uint64_t load_relaxed(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -332,11 +337,16 @@ __device__ __forceinline__ void store_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+//
+// `best` (nullable) receives the index of the first minimum of the P tours' costs (cost_all,
+// written with agent-scope stores): the last tour to finish, counted in word 2 of tour 0's
+// control line (ws0), takes the minimum and clears the count.
 template <int BLOCK, int PP, class Dist>
 __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                      int kp, int k, int max_iter, int G, int g, uint64_t* __restrict__ ws,
                                      int64_t* __restrict__ stats, double* __restrict__ cost, int32_t* lds,
-                                     uint64_t* red, int64_t* red64, uint64_t* s_best) {
+                                     uint64_t* red, int64_t* red64, uint64_t* s_best, int P, uint64_t* ws0,
+                                     double* __restrict__ cost_all, int32_t* __restrict__ best) {
   int4* s_nd = reinterpret_cast<int4*>(lds);  // n
   int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
   const int tid = threadIdx.x;
@@ -408,28 +418,25 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
     }
     best = block_min_u64(best, red);
     uint64_t* err = ws + 2 * kClusterMaxG * kClusterLine;
-    if (tid == 0) {
-      store_relaxed(ws + (G + g) * kClusterLine + (it & 1), best);
-      __builtin_amdgcn_s_waitcnt(0);  // the key is stored before the flag that announces it
-      store_relaxed(ws + g * kClusterLine, (uint64_t)(it + 1));
-    }
+    const uint64_t tag = ((it >> 1) & 1) ? 0ull : kClusterTag;
+    if (tid == 0) store_relaxed(ws + g * kClusterLine + (it & 1), (best == kNoMove ? kClusterNone : best) | tag);
     if (tid < kWave) {
       const int lane = tid;
       bool fail = false;
+      uint64_t w = kClusterNone | tag;
       for (unsigned spin = 0;; ++spin) {
-        const uint64_t f = lane < G ? load_relaxed(ws + lane * kClusterLine) : ~0ull;
-        if (__all(f >= (uint64_t)(it + 1))) break;
+        if (lane < G) w = load_relaxed(ws + lane * kClusterLine + (it & 1));
+        if (__all((w & kClusterTag) == tag)) break;
         if ((spin & 63) == 63 && (load_relaxed(err) != 0 || spin >= kClusterSpinLimit)) {
           fail = true;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      uint64_t key = (!fail && lane < G) ? load_relaxed(ws + (G + lane) * kClusterLine + (it & 1)) : kNoMove;
-      key = wave_min_u64(key);
+      const uint64_t key = wave_min_u64(w & ~kClusterTag);
       if (lane == 0) {
         if (fail) store_relaxed(err, 1ull);
-        s_best[0] = fail ? kClusterFail : key;
+        s_best[0] = fail ? kClusterFail : (key == kClusterNone ? kNoMove : key);
       }
     }
     __syncthreads();
@@ -463,7 +470,25 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
       int64_t part = 0;
       for (int x = tid; x < n; x += BLOCK) part += s_nd[x].z;
       const int64_t tot = block_sum_i64(part, red64);
-      if (tid == 0) *cost = (double)tot;
+      if (tid == 0) store_relaxed(reinterpret_cast<uint64_t*>(cost), (uint64_t)__double_as_longlong((double)tot));
+    }
+  }
+  if (best != nullptr && cost_all != nullptr && g == 0 && tid == 0) {
+    uint64_t* count = ws0 + 2 * kClusterMaxG * kClusterLine + 2;
+    __builtin_amdgcn_s_waitcnt(0);  // this tour's cost is stored before it is counted
+    const uint64_t before = __hip_atomic_fetch_add(count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (before == (uint64_t)(P - 1)) {
+      int arg = 0;
+      double lo = 0.0;
+      for (int t = 0; t < P; ++t) {
+        const double c = __longlong_as_double((long long)load_relaxed(reinterpret_cast<uint64_t*>(cost_all + t)));
+        if (t == 0 || c < lo) {
+          lo = c;
+          arg = t;
+        }
+      }
+      best[0] = arg;
+      store_relaxed(count, 0ull);
     }
   }
   if (stats != nullptr) {
@@ -480,7 +505,10 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
     uint64_t* dep = ws + 2 * kClusterMaxG * kClusterLine + 1;
     const uint64_t done = __hip_atomic_fetch_add(dep, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (done == (uint64_t)(G - 1)) {
-      for (int w = 0; w < G; ++w) store_relaxed(ws + w * kClusterLine, 0ull);
+      for (int w = 0; w < G; ++w) {
+        store_relaxed(ws + w * kClusterLine, 0ull);
+        store_relaxed(ws + w * kClusterLine + 1, 0ull);
+      }
       store_relaxed(dep, 0ull);
     }
   }

@@ -13,7 +13,8 @@ step-by-step ``GeneticAlgorithm`` pipeline.  Per generation the device runs:
   3. tspmi_gather_rows : survivors into the other population buffer (ping-pong)
   4. elite 2-opt       : tspmi_two_opt_elite (search and re-score in one launch, one cluster of
                          workgroups per tour) on the sorted survivors' prefix
-  5. rank_select(1)    : index of the generation's best row, kept on device for the history
+  5. best row          : index of the generation's best row, kept on device for the history (taken
+                         by the elite launch when it runs as workgroup clusters)
 Whole runs of generations can be captured into one HIP graph (``capture``).
 """
 
@@ -136,6 +137,7 @@ class DeviceMemeticLoop:
                        draws.probs[g], draws.pos1[g], draws.pos2[g], self.rate, self.off, ka[P:])
     _native.rank_select(ka, P, self.order)
     _native.gather_rows(self.pop[a], ka[:P], self.off, ka[P:], self.order, self.pop[b], kb[:P])
+    best_done = False
     if self.elite:
       E = self.elite
       elites = self.pop[b][:E]
@@ -143,14 +145,16 @@ class DeviceMemeticLoop:
       if fused and self.xy is not None:
         _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
       elif fused:
-        _native.two_opt_elite(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E], self.sync)
+        _native.two_opt_elite(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E], self.sync,
+                              best_out)
+        best_done = True
       else:
         if self.table is not None:
           _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
         else:
           _native.two_opt(elites, self.D, self.cand, TSPMI_MODE_BEST, self.ls_iters, self.stats)
         _native.tour_cost(elites, self.D, kb[:E])
-    if best_out is not None:
+    if best_out is not None and not best_done:
       _native.rank_select(kb[: (self.elite or 1)], 1, best_out)
     self.cur = b
 
