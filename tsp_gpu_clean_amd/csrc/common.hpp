@@ -38,13 +38,31 @@ constexpr int kMaxLds = 160 * 1024;
 
 // ---- wave64 reductions -------------------------------------------------------
 
+// One DPP step of a 64-bit minimum: lanes of the rows in `row_mask` take min(v, v of the lane that
+// CTRL selects); other lanes keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_min_u64(uint64_t v) {
+  const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+  const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  const uint64_t o = ((uint64_t)ohi << 32) | olo;
+  return o < v ? o : v;
+}
+
+// Minimum over the 64 lanes of a wave, returned to every lane; the whole wave must be active.
+// Data-parallel-primitive moves instead of LDS permutes: quad swaps (xor 1, xor 2), half-row
+// and row mirrors (each row of 16 then holds its minimum), then row broadcasts 15 and 31 carry
+// rows 0-2 into row 3, and lane 63 is read out.
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    uint64_t o = __shfl_xor(v, m, kWave);
-    v = o < v ? o : v;
-  }
-  return v;
+  v = dpp_min_u64<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_min_u64<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_min_u64<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_min_u64<0x140, 0xf>(v);  // row_mirror
+  v = dpp_min_u64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_min_u64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {

@@ -416,12 +416,17 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
         best = key < best ? key : best;
       }
     }
-    best = block_min_u64(best, red);
+    // Only wave 0 needs the workgroup's minimum (it publishes and polls): one barrier, not two.
+    // red[] was last read by wave 0 before the previous pass's s_best barrier.
+    best = wave_min_u64(best);
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = best;
+    __syncthreads();
     uint64_t* err = ws + 2 * kClusterMaxG * kClusterLine;
     const uint64_t tag = ((it >> 1) & 1) ? 0ull : kClusterTag;
-    if (tid == 0) store_relaxed(ws + g * kClusterLine + (it & 1), (best == kNoMove ? kClusterNone : best) | tag);
     if (tid < kWave) {
       const int lane = tid;
+      best = wave_min_u64(lane < BLOCK / kWave ? red[lane] : kNoMove);
+      if (lane == 0) store_relaxed(ws + g * kClusterLine + (it & 1), (best == kNoMove ? kClusterNone : best) | tag);
       bool fail = false;
       uint64_t w = kClusterNone | tag;
       for (unsigned spin = 0;; ++spin) {
