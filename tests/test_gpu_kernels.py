@@ -443,6 +443,29 @@ def test_packed_candidate_n1000_to_optimum(torch_mod, nat, P):
   np.testing.assert_array_equal(same, got)
 
 
+@pytest.mark.parametrize("k", [5, 9, 10, 11, 12, 13, 16, 17, 20, 21, 32])
+def test_packed_candidate_lane_groups_match_c_oracle(torch_mod, nat, k):
+  """Every lanes-per-row group of the many-tour candidate kernel (P > CUs, 512- and 256-thread
+  workgroups) and of its coordinate twin: the C oracle's tours and counters."""
+  say(f"two_opt_packed lane group k={k}")
+  n = 200
+  xy = uniform_coords(n, seed=k)
+  d = compute_distance_matrix(xy)
+  cand = compute_candidate_list(d, k)
+  for P, iters in ((300, 6), (600, -1)):
+    tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(k + P), P))
+    got, stats, _ = _run_packed(nat, torch_mod, tours, d, cand, iters)
+    want, wstats = c_two_opt.improve_batch(tours, d, cand, "best", iters)
+    np.testing.assert_array_equal(got, want)
+    assert stats.tolist() == wstats.tolist()
+    D = cu(d)
+    table = torch_mod.empty((n, nat.candidate_lanes(k), 2), dtype=torch_mod.int32, device="cuda")
+    nat.candidate_table(cu(cand), D, table)
+    t = cu(tours)
+    nat.two_opt_packed_euc2d(t, cu(xy), table, k, iters)
+    np.testing.assert_array_equal(t.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("n,P,k,iters", [(1000, 16, 20, 10), (1024, 16, 20, -1), (1025, 16, 20, 10), (300, 300, 8, 5),
                                          (6, 3, 5, 4)])
 def test_packed_two_opt_fused_cost(torch_mod, nat, n, P, k, iters):

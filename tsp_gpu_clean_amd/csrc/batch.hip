@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const int64_t* __rest
 }
 
 // Elite candidate 2-opt of every instance: persistent grid over (instance, elite row) items.
-template <int BLOCK>
+template <int BLOCK, int KL>
 __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __restrict__ desc,
                                                               const int32_t* __restrict__ item_inst,
                                                               const int32_t* __restrict__ item_idx, int nitems,
@@ -211,9 +211,9 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __r
     const Inst I = load_inst(desc, item_inst[it]);
     const int ke = k < I.n - 1 ? k : I.n - 1;
     if (I.n >= 4 && ke >= 1)
-      cand_two_opt_tour<BLOCK, 32, 4>(pop + I.pop + (size_t)item_idx[it] * I.n, I.n, MatrixDist{D_all + I.d, I.n},
-                                      ct_all + I.ct, ke,
-                                      max_iter, stats, lds, red, red64);
+      cand_two_opt_tour<BLOCK, 32, 4, KL>(pop + I.pop + (size_t)item_idx[it] * I.n, I.n,
+                                          MatrixDist{D_all + I.d, I.n}, ct_all + I.ct, ke, max_iter, stats, lds, red,
+                                          red64);
     __syncthreads();
   }
 }
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __r
 // As batch_two_opt_kernel, but instances with n >= coord_min_n compute their distances from the
 // EUC_2D coordinates (same tours) instead of gathering them from a matrix that no longer fits the
 // caches.
-template <int BLOCK>
+template <int BLOCK, int KL>
 __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst, const int32_t* __restrict__ item_idx,
     int nitems, int32_t* __restrict__ pop, const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
@@ -235,11 +235,11 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     int32_t* tour = pop + I.pop + (size_t)item_idx[it] * I.n;
     if (I.n >= 4 && ke >= 1) {
       if (I.n >= coord_min_n)
-        cand_two_opt_tour<BLOCK, 32, 4>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
-                                        ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
+                                            ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
       else
-        cand_two_opt_tour<BLOCK, 32, 4>(tour, I.n, MatrixDist{D_all + I.d, I.n}, ct_all + I.ct, ke, max_iter, stats,
-                                        lds, red, red64);
+        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, MatrixDist{D_all + I.d, I.n}, ct_all + I.ct, ke, max_iter,
+                                            stats, lds, red, red64);
     }
     __syncthreads();
   }
@@ -382,16 +382,16 @@ extern "C" int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* it
   const int g = grid > 0 && grid < nitems ? grid : nitems;
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table_all);
-  if (block == 256)
-    hipLaunchKernelGGL(batch_two_opt_kernel<256>, dim3(g), dim3(256), lds, s, desc, item_inst, item_idx, nitems, pop,
-                       D_all, ct, k, max_iter, stats);
-  else if (block == 512)
-    hipLaunchKernelGGL(batch_two_opt_kernel<512>, dim3(g), dim3(512), lds, s, desc, item_inst, item_idx, nitems, pop,
-                       D_all, ct, k, max_iter, stats);
-  else if (block == 1024)
-    hipLaunchKernelGGL(batch_two_opt_kernel<1024>, dim3(g), dim3(1024), lds, s, desc, item_inst, item_idx, nitems, pop,
-                       D_all, ct, k, max_iter, stats);
-  else
+#define TSPMI_BATCH_LS(BL, KPV, KLV)                                                                          \
+  hipLaunchKernelGGL((batch_two_opt_kernel<BL, KLV>), dim3(g), dim3(BL), lds, s, desc, item_inst, item_idx, nitems, \
+                     pop, D_all, ct, k, max_iter, stats)
+  if (block == 256) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_LS, 256, k)
+  } else if (block == 512) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_LS, 512, k)
+  } else if (block == 1024) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_LS, 1024, k)
+  } else
     TSPMI_CHECK_ARG(false, "tspmi_batch_two_opt_packed: block %d not in {256, 512, 1024}", block);
   TSPMI_CHECK_LAUNCH("tspmi_batch_two_opt_packed launch");
   return TSPMI_OK;
@@ -423,16 +423,16 @@ extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32
   const int g = grid > 0 && grid < nitems ? grid : nitems;
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table_all);
-  if (block == 256)
-    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<256>, dim3(g), dim3(256), lds, s, desc, item_inst, item_idx, nitems,
-                       pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
-  else if (block == 512)
-    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<512>, dim3(g), dim3(512), lds, s, desc, item_inst, item_idx, nitems,
-                       pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
-  else if (block == 1024)
-    hipLaunchKernelGGL(batch_two_opt_mixed_kernel<1024>, dim3(g), dim3(1024), lds, s, desc, item_inst, item_idx,
-                       nitems, pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats);
-  else
+#define TSPMI_BATCH_MIXED(BL, KPV, KLV)                                                                        \
+  hipLaunchKernelGGL((batch_two_opt_mixed_kernel<BL, KLV>), dim3(g), dim3(BL), lds, s, desc, item_inst, item_idx,   \
+                     nitems, pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats)
+  if (block == 256) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_MIXED, 256, k)
+  } else if (block == 512) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_MIXED, 512, k)
+  } else if (block == 1024) {
+    TSPMI_CAND_BY_K(TSPMI_BATCH_MIXED, 1024, k)
+  } else
     TSPMI_CHECK_ARG(false, "tspmi_batch_two_opt_packed_mixed: block %d not in {256, 512, 1024}", block);
   TSPMI_CHECK_LAUNCH("tspmi_batch_two_opt_packed_mixed launch");
   return TSPMI_OK;

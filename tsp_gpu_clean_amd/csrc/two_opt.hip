@@ -485,7 +485,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
   }
 }
 
-template <int BLOCK, int KP, int UNROLL>
+template <int BLOCK, int KP, int UNROLL, int KL>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict__ tours, int n,
                                                              const int32_t* __restrict__ D, int ld,
                                                              const int2* __restrict__ ct, int k, int max_iter,
@@ -493,8 +493,8 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, MatrixDist{D, ld}, ct, k, max_iter, stats,
-                                       lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
+  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, MatrixDist{D, ld}, ct, k, max_iter,
+                                           stats, lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
 }
 
 // One tour per workgroup of 1024 threads, pairs in registers (reg_two_opt_tour).
@@ -844,26 +844,20 @@ extern "C" int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int
     TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed launch");
     return TSPMI_OK;
   }
-#define TSPMI_LAUNCH_CAND(BL, KPV, U)                                                                           \
-  hipLaunchKernelGGL((two_opt_cand_kernel<BL, KPV, U>), dim3(P), dim3(BL), lds, s, tours, n, D, ld, ct, k, max_iter, \
-                     stats, cost)
+#define TSPMI_LAUNCH_CAND(BL, KPV, KLV)                                                                          \
+  hipLaunchKernelGGL((two_opt_cand_kernel<BL, KPV, 4, KLV>), dim3(P), dim3(BL), lds, s, tours, n, D, ld, ct, k,       \
+                     max_iter, stats, cost)
   // (A 16-way unroll at 1024 threads was measured slower on 16-tour elite calls: 16 vs 14.5 us per pass.)
   // (Holding the table in LDS for these launches saved 7 % per pass but cost 5.6 us per launch to
   // fill, measured on 16-tour elite calls; not kept.  A node-indexed streaming scan -- two 16-byte
   // node records per pair, no KP padding, a 64-bit key minimum per pair -- ran 1.36e11 deltas/s
   // against this kernel's 2.0e11 at P = 1024, n = 1000, k = 20; not kept either.)
   if (block == 1024) {
-    if (kp == 8) TSPMI_LAUNCH_CAND(1024, 8, 4);
-    else if (kp == 16) TSPMI_LAUNCH_CAND(1024, 16, 4);
-    else TSPMI_LAUNCH_CAND(1024, 32, 4);
+    TSPMI_CAND_BY_K(TSPMI_LAUNCH_CAND, 1024, k)
   } else if (block == 512) {
-    if (kp == 8) TSPMI_LAUNCH_CAND(512, 8, 4);
-    else if (kp == 16) TSPMI_LAUNCH_CAND(512, 16, 4);
-    else TSPMI_LAUNCH_CAND(512, 32, 4);
+    TSPMI_CAND_BY_K(TSPMI_LAUNCH_CAND, 512, k)
   } else {
-    if (kp == 8) TSPMI_LAUNCH_CAND(256, 8, 4);
-    else if (kp == 16) TSPMI_LAUNCH_CAND(256, 16, 4);
-    else TSPMI_LAUNCH_CAND(256, 32, 4);
+    TSPMI_CAND_BY_K(TSPMI_LAUNCH_CAND, 256, k)
   }
 #undef TSPMI_LAUNCH_CAND
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed launch");

@@ -61,30 +61,47 @@ __device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node)
   return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(row) + ((uint32_t)node << 2));
 }
 
+// Lanes per table row of cand_two_opt_tour for k candidates: the smallest instantiated width >= k.
+inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ? 12 : k <= 16 ? 16 : k <= 20 ? 20 : 32; }
+
+// Expands LAUNCH(BL, KP, KL) for the lane group of runtime `k` (KP = the packed table's row width).
+#define TSPMI_CAND_BY_K(LAUNCH, BL, k)    \
+  switch (cand_lane_group(k)) {           \
+    case 8: LAUNCH(BL, 8, 8); break;      \
+    case 10: LAUNCH(BL, 16, 10); break;   \
+    case 12: LAUNCH(BL, 16, 12); break;   \
+    case 16: LAUNCH(BL, 16, 16); break;   \
+    case 20: LAUNCH(BL, 32, 20); break;   \
+    default: LAUNCH(BL, 32, 32); break;   \
+  }
+
 // Candidate-list steepest 2-opt on a packed candidate table.
 //
 // `ct` is (n, KP) int2 {neighbour, D[node, neighbour]} (neighbour -1 past the k real entries), so
 // the first distance of every candidate pair comes with the neighbour id in one 8-byte load.  A
-// wave covers 64 / KP consecutive rows per step (lane = row offset * KP + r), with no integer
-// division; UNROLL steps are issued together so their loads overlap.  Within a lane the pairs
-// are visited in increasing (i, r) scan order, so a strict `<` keeps the lane's first minimum and
-// the packed (delta, i*k + r) key restores the reference tie-break across lanes.
+// wave covers RPW = 64 / KL consecutive rows per step (lane = row offset * KL + r, KL <= KP lanes
+// per row, KL = k where an instance exists so that no lane idles on the table's padding; the
+// 64 - RPW * KL spare lanes sit out), with the division by the constant KL done at compile time;
+// UNROLL steps are issued together so their loads overlap.  Within a lane the pairs are visited
+// in increasing (i, r) scan order, so a strict `<` keeps the lane's first minimum and the packed
+// (delta, i*k + r) key restores the reference tie-break across lanes.
 //
 // `cost` (nullable) receives the closed cost of the final tour, the sum of its edge costs.
-template <int BLOCK, int KP, int UNROLL, class Dist>
+template <int BLOCK, int KP, int UNROLL, int KL = KP, class Dist>
 __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                   int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
                                   int64_t* red64, double* __restrict__ cost = nullptr) {
+  static_assert(KL <= KP && KL <= kWave, "lanes per row exceed the table row");
   constexpr int NW = BLOCK / kWave;
-  constexpr int RPW = kWave / KP;  // rows per wave step
+  constexpr int RPW = kWave / KL;  // rows per wave step
   int32_t* s_tour = lds;           // n + 1 (s_tour[n] mirrors s_tour[0])
   int32_t* s_edge = lds + n + 1;   // n
   int32_t* s_pos = s_edge + n;     // n
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid / kWave;
-  const int sub = lane / KP;       // row offset inside the wave step
-  const int r = lane - sub * KP;   // candidate rank handled by this lane
+  const int sub = lane / KL;       // row offset inside the wave step (RPW for a spare lane)
+  const int r = lane - sub * KL;   // candidate rank handled by this lane
   for (int x = tid; x < n; x += BLOCK) {
     const int v = clamp_id(tour[x], n);
     s_tour[x] = v;
@@ -110,7 +127,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
       int row[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
-        row[u] = base + u * RPW + sub;
+        row[u] = sub < RPW ? base + u * RPW + sub : rows;
         const int a = s_tour[row[u] < rows ? row[u] : 0];
         c[u] = ct[(size_t)a * KP + r];
       }

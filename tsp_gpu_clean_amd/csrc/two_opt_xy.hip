@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
   }
 }
 
-template <int BLOCK, int KP, int UNROLL>
+template <int BLOCK, int KP, int UNROLL, int KL>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n,
                                                                 const double2* __restrict__ xy,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restr
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  cand_two_opt_tour<BLOCK, KP, UNROLL>(tours + (size_t)blockIdx.x * n, n, CoordDist{xy}, ct, k, max_iter, stats, lds,
-                                       red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
+  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, CoordDist{xy}, ct, k, max_iter, stats,
+                                           lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
 }
 
 }  // namespace tspmi
@@ -175,24 +175,17 @@ extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, con
   hipStream_t s = (hipStream_t)stream;
   const double2* p = reinterpret_cast<const double2*>(xy);
   const int2* ct = reinterpret_cast<const int2*>(table);
-  const int kp = k <= 8 ? 8 : (k <= 16 ? 16 : 32);
   const int cus = device_cu_count();
   const int block = P <= cus ? 1024 : (P <= 2 * cus ? 512 : 256);
-#define TSPMI_CXY(BL, KPV)                                                                                         \
-  hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
+#define TSPMI_CXY(BL, KPV, KLV)                                                                                      \
+  hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
                      stats, cost)
   if (block == 1024) {
-    if (kp == 8) TSPMI_CXY(1024, 8);
-    else if (kp == 16) TSPMI_CXY(1024, 16);
-    else TSPMI_CXY(1024, 32);
+    TSPMI_CAND_BY_K(TSPMI_CXY, 1024, k)
   } else if (block == 512) {
-    if (kp == 8) TSPMI_CXY(512, 8);
-    else if (kp == 16) TSPMI_CXY(512, 16);
-    else TSPMI_CXY(512, 32);
+    TSPMI_CAND_BY_K(TSPMI_CXY, 512, k)
   } else {
-    if (kp == 8) TSPMI_CXY(256, 8);
-    else if (kp == 16) TSPMI_CXY(256, 16);
-    else TSPMI_CXY(256, 32);
+    TSPMI_CAND_BY_K(TSPMI_CXY, 256, k)
   }
 #undef TSPMI_CXY
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d launch");
