@@ -290,16 +290,18 @@ int tspmi_batch_knn(const int64_t* desc, const int32_t* item_inst, const int32_t
 int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
                             const uint64_t* seeds, int32_t* pop, void* stream);
 
-/* Tour costs of listed (instance, row) items into key slot 2 * row offset + row.  Replaces
- * tour_cost.py:17-65 per instance. */
+/* Tour costs of listed (instance, row) items into key slot 2 * row offset + row; with xy_all
+ * (may be NULL) the EUC_2D distances are computed from the coordinates instead of read from D_all
+ * (same values).  Replaces tour_cost.py:17-65 per instance. */
 int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
-                     const int32_t* pop, const int32_t* D_all, double* keys, void* stream);
+                     const int32_t* pop, const int32_t* D_all, const double* xy_all, double* keys, void* stream);
 
 /* Fused generation of every instance: items are (instance, child row r); block in
- * {128, 256, 512, 1024}; grid <= 0 means one workgroup per item, else a persistent grid.
+ * {128, 256, 512, 1024}; grid <= 0 means one workgroup per item, else a persistent grid; child
+ * costs from xy_all when given (as tspmi_batch_cost).
  * Replaces genetic_algorithm.py:120-213 per instance. */
 int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
-                           int max_n, const int32_t* popA, double* keysA, const int32_t* D_all,
+                           int max_n, const int32_t* popA, double* keysA, const int32_t* D_all, const double* xy_all,
                            const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
                            const double* probs, const int32_t* pos1, const int32_t* pos2, double rate, int32_t* off,
                            int block, int grid, void* stream);

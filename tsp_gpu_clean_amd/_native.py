@@ -49,8 +49,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_dist_euc2d": [_P, _P, _P, _I, _P, _P, _P],
   "tspmi_batch_knn": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_batch_random_init": [_P, _P, _P, _I, _P, _P, _P],
-  "tspmi_batch_cost": [_P, _P, _P, _I, _P, _P, _P, _P],
-  "tspmi_batch_generation": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _I, _P],
+  "tspmi_batch_cost": [_P, _P, _P, _I, _P, _P, _P, _P, _P],
+  "tspmi_batch_generation": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _I, _P],
   "tspmi_batch_survival": [_P, _I, _P, _P, _P],
   "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],

@@ -19,12 +19,16 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .context import coordinate_candidates_pay
 from .rng import draw_row_seeds, make_rng
 
 FIELDS = 9  # TSPMI_BATCH_FIELDS
 KP = 32  # packed candidate row width used by the batch kernels
-COORD_MIN_N = 4097  # instances this large compute candidate-scan distances from coordinates
+# Instances at least this large compute their distances from coordinates in the elite search (and
+# every instance does in the generation and cost kernels).  With 256 instances the matrices (1.5 GB
+# at n <= 2000) thrash the caches while each instance's coordinates are at most 32 KB: measured on
+# an MI355X at 256 instances, n in [50, 2000], 100 generations, all-coordinates ran 464 gens/s
+# against 173 with matrices only (256 / 368 / 449 with the threshold at 1500 / 1000 / 600).
+COORD_MIN_N = 0
 
 
 @dataclass
@@ -118,7 +122,8 @@ class BatchedMemeticGA:
     """SplitMix64 random permutations from each instance's generator (random_init.py:30-55)."""
     seeds = np.concatenate([draw_row_seeds(r, self.P) for r in self.rngs])
     _native.batch_call("tspmi_batch_random_init", self.desc, *self.w_row, _cu(seeds.view(np.int64)), self.pop[self.cur])
-    _native.batch_call("tspmi_batch_cost", self.desc, *self.w_row, self.pop[self.cur], self.D, self.keys[self.cur])
+    _native.batch_call("tspmi_batch_cost", self.desc, *self.w_row, self.pop[self.cur], self.D, self.xy,
+                       self.keys[self.cur])
 
   def population(self, i: int):
     """Device view (P, n_i) of instance i's current population."""
@@ -168,16 +173,12 @@ class BatchedMemeticGA:
   # ---- one generation ---------------------------------------------------------------------------
   def _generation(self, a, d: BatchDraws, g: int, block: int, grid: int) -> None:
     _native.batch_call("tspmi_batch_generation", self.desc, *self.w_row, self.max_n, self.pop[a], self.keys[a], self.D,
-                       d.tour_cand[g], self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g],
+                       self.xy, d.tour_cand[g], self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g],
                        self.rate, self.off, block, grid)
 
   def _elite_ls(self, pop, block: int, grid: int) -> None:
-    if coordinate_candidates_pay(self.max_n):  # large instances compute distances from coordinates
-      _native.batch_call("tspmi_batch_two_opt_packed_mixed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.xy,
-                         self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
-    else:
-      _native.batch_call("tspmi_batch_two_opt_packed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.table,
-                         self.kc, self.ls_iters, block, grid, self.stats)
+    _native.batch_call("tspmi_batch_two_opt_packed_mixed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.xy,
+                       self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
 
   def step(self, d: BatchDraws, g: int, best=None) -> None:
     a, b = self.cur, 1 - self.cur
@@ -188,7 +189,7 @@ class BatchedMemeticGA:
                        self.pop[b], self.keys[b])
     if self.elite > 0:
       self._elite_ls(self.pop[b], s["ls_block"], s["ls_grid"])
-      _native.batch_call("tspmi_batch_cost", self.desc, *self.w_elite, self.pop[b], self.D, self.keys[b])
+      _native.batch_call("tspmi_batch_cost", self.desc, *self.w_elite, self.pop[b], self.D, self.xy, self.keys[b])
     if best is not None:
       _native.batch_call("tspmi_batch_best", self.desc, self.B, self.keys[b], max(1, self.elite), best)
     self.cur = b

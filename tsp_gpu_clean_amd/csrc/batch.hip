@@ -122,19 +122,20 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restri
                                                          const int32_t* __restrict__ item_inst,
                                                          const int32_t* __restrict__ item_idx, int nitems,
                                                          const int32_t* __restrict__ pop, const int32_t* __restrict__ D_all,
-                                                         double* __restrict__ keys) {
+                                                         const double* __restrict__ xy_all, double* __restrict__ keys) {
   const int it = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (it >= nitems) return;
   const Inst I = load_inst(desc, item_inst[it]);
   const int r = item_idx[it];
   const int32_t* t = pop + I.pop + (size_t)r * I.n;
-  const int32_t* D = D_all + I.d;
+  const MatrixDist md{D_all + I.d, I.n};
+  const CoordDist cd{reinterpret_cast<const double2*>(xy_all + I.xy)};
   int64_t s = 0;
   for (int x = lane; x < I.n; x += kWave) {
     const int a = clamp_id(t[x], I.n);
     const int b = clamp_id(t[x + 1 == I.n ? 0 : x + 1], I.n);
-    if (I.n > 1) s += D[(size_t)a * I.n + b];
+    if (I.n > 1) s += xy_all != nullptr ? cd(a, b) : md(a, b);
   }
   s = wave_sum_i64(s);
   if (lane == 0) keys[2 * I.row + r] = (double)s;
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restri
 __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst,
                                         const int32_t* __restrict__ item_idx, int nitems,
                                         const int32_t* __restrict__ popA, double* __restrict__ keysA,
-                                        const int32_t* __restrict__ D_all, const int32_t* __restrict__ tour_cand, int k,
+                                        const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
+                                        const int32_t* __restrict__ tour_cand, int k,
                                         const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
                                         const double* __restrict__ probs, const int32_t* __restrict__ pos1,
                                         const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ off) {
@@ -154,9 +156,15 @@ __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Inst I = load_inst(desc, item_inst[it]);
     double* keys = keysA + 2 * I.row;
-    generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n, D_all + I.d, I.n, tour_cand + (size_t)I.row * k, k,
-                     cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
-                     keys + I.P, lds, red, s_w);
+    if (xy_all != nullptr)
+      generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n,
+                       CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)}, tour_cand + (size_t)I.row * k, k,
+                       cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
+                       keys + I.P, lds, red, s_w);
+    else
+      generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n, MatrixDist{D_all + I.d, I.n},
+                       tour_cand + (size_t)I.row * k, k, cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row,
+                       pos2 + I.row, rate, off + I.pop, keys + I.P, lds, red, s_w);
     __syncthreads();
   }
 }
@@ -318,32 +326,37 @@ extern "C" int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_
 }
 
 extern "C" int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
-                                const int32_t* pop, const int32_t* D_all, double* keys, void* stream) {
+                                const int32_t* pop, const int32_t* D_all, const double* xy_all, double* keys,
+                                void* stream) {
   TSPMI_BATCH_ARGS("tspmi_batch_cost");
   if (nitems == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(pop && D_all && keys, "tspmi_batch_cost: null pointer");
+  TSPMI_CHECK_ARG(pop && (D_all || xy_all) && keys, "tspmi_batch_cost: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_cost: xy must be 16-byte aligned");
   hipLaunchKernelGGL(batch_cost_kernel, dim3((nitems + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, item_inst,
-                     item_idx, nitems, pop, D_all, keys);
+                     item_idx, nitems, pop, D_all, xy_all, keys);
   TSPMI_CHECK_LAUNCH("tspmi_batch_cost launch");
   return TSPMI_OK;
 }
 
 extern "C" int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
                                       int nitems, int max_n, const int32_t* popA, double* keysA, const int32_t* D_all,
-                                      const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                                      const double* xy_all, const int32_t* tour_cand, int k, const int32_t* cut_lo,
+                                      const int32_t* cut_hi,
                                       const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
                                       int32_t* off, int block, int grid, void* stream) {
   TSPMI_BATCH_ARGS("tspmi_batch_generation");
   TSPMI_CHECK_ARG(block == 128 || block == 256 || block == 512 || block == 1024,
                   "tspmi_batch_generation: block %d not in {128, 256, 512, 1024}", block);
   if (nitems == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(popA && keysA && D_all && tour_cand && cut_lo && cut_hi && probs && pos1 && pos2 && off && k >= 1,
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_generation: xy must be 16-byte aligned");
+  TSPMI_CHECK_ARG(popA && keysA && (D_all || xy_all) && tour_cand && cut_lo && cut_hi && probs && pos1 && pos2 && off &&
+                      k >= 1,
                   "tspmi_batch_generation: null pointer");
   const size_t lds = (size_t)6 * max_n * sizeof(int32_t);
   TSPMI_CHECK_ARG(max_n >= 2 && lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_generation: max_n=%d exceeds LDS", max_n);
   const int g = grid > 0 && grid < nitems ? grid : nitems;
   hipLaunchKernelGGL(batch_generation_kernel, dim3(g), dim3(block), lds, (hipStream_t)stream, desc, item_inst, item_idx,
-                     nitems, popA, keysA, D_all, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, off);
+                     nitems, popA, keysA, D_all, xy_all, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, off);
   TSPMI_CHECK_LAUNCH("tspmi_batch_generation launch");
   return TSPMI_OK;
 }

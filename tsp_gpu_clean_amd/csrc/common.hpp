@@ -131,4 +131,21 @@ __device__ __forceinline__ int euc2d_nint(double xu, double yu, double xv, doubl
   return (int)(__builtin_sqrt(s) + 0.5);
 }
 
+// Distance policies: a row-major int32 matrix, or EUC_2D coordinates with the distance computed
+// on the fly by the same FP64 function that builds the matrix (bit-identical values).
+struct MatrixDist {
+  const int32_t* __restrict__ D;
+  int ld;
+  __device__ __forceinline__ int operator()(int u, int v) const { return D[(size_t)u * ld + v]; }
+};
+
+struct CoordDist {
+  const double2* __restrict__ xy;
+  __device__ __forceinline__ int operator()(int u, int v) const {
+    const double2 a = xy[u], b = xy[v];
+    return euc2d_nint(a.x, a.y, b.x, b.y);
+  }
+};
+
+
 }  // namespace tspmi

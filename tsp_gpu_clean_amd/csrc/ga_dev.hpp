@@ -115,18 +115,19 @@ __device__ __forceinline__ void reverse_range(int32_t* t, int lo, int hi) {
   }
 }
 
-__device__ __forceinline__ double block_cost(const int32_t* t, int n, const int32_t* __restrict__ D, int ld,
-                                             int64_t* red) {
+template <class Dist>
+__device__ __forceinline__ double block_cost(const int32_t* t, int n, const Dist& dist, int64_t* red) {
   int64_t s = 0;
   for (int x = threadIdx.x; x < n; x += blockDim.x) {
     const int nx = (x + 1 == n) ? 0 : x + 1;
-    if (n > 1) s += D[(size_t)t[x] * ld + t[nx]];
+    if (n > 1) s += dist(t[x], t[nx]);
   }
   return (double)block_sum_i64(s, red);
 }
 
 // Inversion (when drawn), cost and store of child row r held in LDS `ch`.
-__device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const int32_t* __restrict__ D, int ld,
+template <class Dist>
+__device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const Dist& dist,
                                              const double* __restrict__ probs, const int32_t* __restrict__ pos1,
                                              const int32_t* __restrict__ pos2, double rate,
                                              int32_t* __restrict__ offspring, double* __restrict__ off_fit,
@@ -143,7 +144,7 @@ __device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const in
     reverse_range(ch, lo, hi);
     __syncthreads();
   }
-  const double cost = block_cost(ch, n, D, ld, red);
+  const double cost = block_cost(ch, n, dist, red);
   for (int x = threadIdx.x; x < n; x += blockDim.x) offspring[(size_t)r * n + x] = ch[x];
   if (threadIdx.x == 0) off_fit[r] = cost;
 }
@@ -154,8 +155,10 @@ __device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const in
 // pair load both tournament winners; row 2q is PMX(p1, p2), row 2q + 1 is PMX(p2, p1), and an
 // odd last row copies its own winner (pmx_crossover.py:128-171).  One workgroup per child halves
 // the serial chain of a pair's workgroup (two PMX builds, costs and stores).
+// Child costs come from `dist` (a MatrixDist or, for EUC_2D instances, a CoordDist: same values).
+template <class Dist>
 __device__ void generation_child(int r, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P,
-                                 int n, const int32_t* __restrict__ D, int ld, const int32_t* __restrict__ tour_cand,
+                                 int n, const Dist dist, const int32_t* __restrict__ tour_cand,
                                  int k, const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
                                  const double* __restrict__ probs, const int32_t* __restrict__ pos1,
                                  const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ offspring,
@@ -177,7 +180,7 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
     const int32_t* src = pop + (size_t)s_w[0] * n;
     for (int x = threadIdx.x; x < n; x += blockDim.x) ch[x] = clamp_id(src[x], n);
     __syncthreads();
-    finish_child(r, ch, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+    finish_child(r, ch, n, dist, probs, pos1, pos2, rate, offspring, off_fit, red);
     return;
   }
   load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
@@ -194,7 +197,7 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
   if (r == r0) pmx_child(p1, p2, inv1, inv2, n, c1, c2, ch, jump);
   else pmx_child(p2, p1, inv2, inv1, n, c1, c2, ch, jump);
   __syncthreads();
-  finish_child(r, ch, n, D, ld, probs, pos1, pos2, rate, offspring, off_fit, red);
+  finish_child(r, ch, n, dist, probs, pos1, pos2, rate, offspring, off_fit, red);
 }
 
 

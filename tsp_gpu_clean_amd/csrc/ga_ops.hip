@@ -92,8 +92,8 @@ __global__ __launch_bounds__(kRowBlock) void generation_kernel(
   extern __shared__ int32_t lds[];
   __shared__ int64_t red[kRowBlock / kWave];
   __shared__ int s_w[2];
-  generation_child(blockIdx.x, pop, fitness, P, n, D, ld, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate,
-                   offspring, off_fit, lds, red, s_w);
+  generation_child(blockIdx.x, pop, fitness, P, n, MatrixDist{D, ld}, tour_cand, k, cut_lo, cut_hi, probs, pos1,
+                   pos2, rate, offspring, off_fit, lds, red, s_w);
 }
 
 // Stable ascending order of M float64 keys: order[rank(i)] = i for rank(i) < keep, with

@@ -16,22 +16,6 @@ struct TwoOptShared {
 
 __device__ __forceinline__ int wrap_next(int x, int n) { return (x + 1 == n) ? 0 : x + 1; }
 
-// Distance policies: a row-major int32 matrix, or EUC_2D coordinates with the distance computed
-// on the fly by the same FP64 function that builds the matrix (bit-identical values).
-struct MatrixDist {
-  const int32_t* __restrict__ D;
-  int ld;
-  __device__ __forceinline__ int operator()(int u, int v) const { return D[(size_t)u * ld + v]; }
-};
-
-struct CoordDist {
-  const double2* __restrict__ xy;
-  __device__ __forceinline__ int operator()(int u, int v) const {
-    const double2 a = xy[u], b = xy[v];
-    return euc2d_nint(a.x, a.y, b.x, b.y);
-  }
-};
-
 // Reverse s_tour[i+1 .. j] and refresh edge costs (and positions) of the touched range.
 template <class Dist>
 __device__ __forceinline__ void apply_move(int32_t* s_tour, int32_t* s_edge, int32_t* s_pos, int i, int j, int n,
