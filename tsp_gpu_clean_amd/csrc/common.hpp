@@ -133,10 +133,17 @@ __device__ __forceinline__ int euc2d_nint(double xu, double yu, double xv, doubl
 
 // Distance policies: a row-major int32 matrix, or EUC_2D coordinates with the distance computed
 // on the fly by the same FP64 function that builds the matrix (bit-identical values).
+// Every entry point that reads a matrix through MatrixDist requires n * ld < 2^30, so an element's
+// byte offset fits 32 bits and the gather is addressed from the matrix base with one 32-bit offset.
+constexpr long kMaxMatrixElems = 1L << 30;
+
 struct MatrixDist {
   const int32_t* __restrict__ D;
   int ld;
-  __device__ __forceinline__ int operator()(int u, int v) const { return D[(size_t)u * ld + v]; }
+  __device__ __forceinline__ int operator()(int u, int v) const {
+    const uint32_t off = ((uint32_t)u * (uint32_t)ld + (uint32_t)v) << 2;
+    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(D) + off);
+  }
 };
 
 struct CoordDist {

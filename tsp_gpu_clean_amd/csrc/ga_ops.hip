@@ -294,7 +294,8 @@ extern "C" int tspmi_generation(const int32_t* pop, const double* fitness, int P
                                 const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
                                 const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
                                 int32_t* offspring, double* off_fit, void* stream) {
-  TSPMI_CHECK_ARG(P > 0 && n >= 2 && k >= 1 && ld >= n, "tspmi_generation: bad shape P=%d n=%d k=%d ld=%d", P, n, k, ld);
+  TSPMI_CHECK_ARG(P > 0 && n >= 2 && k >= 1 && ld >= n && (long)n * ld < kMaxMatrixElems,
+                  "tspmi_generation: bad shape P=%d n=%d k=%d ld=%d", P, n, k, ld);
   TSPMI_CHECK_ARG(pop && fitness && D && tour_cand && probs && pos1 && pos2 && offspring && off_fit &&
                       (P < 2 || (cut_lo && cut_hi)),
                   "tspmi_generation: null pointer");

@@ -731,6 +731,7 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;  // reference: n < 4 or no budget is a copy
   TSPMI_CHECK_ARG(tours != nullptr && D != nullptr, "tspmi_two_opt: null tours or distances");
   TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt: ld=%d < n=%d", ld, n);
+  TSPMI_CHECK_ARG((long)n * ld < kMaxMatrixElems, "tspmi_two_opt: n * ld = %ld >= 2^30", (long)n * ld);
   TSPMI_CHECK_ARG((long)n * (long)n < (1L << 32), "tspmi_two_opt: n=%d too large for 32-bit scan indices", n);
   const bool use_cand = cand != nullptr && k > 0;
   if (use_cand) {
@@ -817,7 +818,8 @@ extern "C" int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int
     return cost != nullptr ? tspmi_tour_cost(tours, P, n, D, ld, 1, cost, stream) : TSPMI_OK;
   }
   TSPMI_CHECK_ARG(tours && D && table, "tspmi_two_opt_packed: null pointer");
-  TSPMI_CHECK_ARG(ld >= n && (long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed: bad ld/n/k");
+  TSPMI_CHECK_ARG(ld >= n && (long)(n - 2) * k < (1L << 31) && (long)n * ld < kMaxMatrixElems,
+                  "tspmi_two_opt_packed: bad ld/n/k (n * ld must stay below 2^30)");
   TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0, "tspmi_two_opt_packed: table must be 8-byte aligned");
   const size_t lds = (size_t)(3 * n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_packed: n=%d exceeds the LDS limit", n);
@@ -892,7 +894,7 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
     return tspmi_rank_select(cost, P, 1, best, stream);
   }
   TSPMI_CHECK_ARG(tours && D && table && sync, "tspmi_two_opt_elite: null pointer");
-  TSPMI_CHECK_ARG(ld >= n, "tspmi_two_opt_elite: ld=%d < n=%d", ld, n);
+  TSPMI_CHECK_ARG(ld >= n && (long)n * ld < kMaxMatrixElems, "tspmi_two_opt_elite: bad ld=%d for n=%d", ld, n);
   TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0,
                   "tspmi_two_opt_elite: table and sync must be 8-byte aligned");
   hipStream_t s = (hipStream_t)stream;

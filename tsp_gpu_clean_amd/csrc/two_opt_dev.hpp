@@ -113,16 +113,19 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
       for (int u = 0; u < UNROLL; ++u) {
         row[u] = sub < RPW ? base + u * RPW + sub : rows;
         const int a = s_tour[row[u] < rows ? row[u] : 0];
-        c[u] = ct[(size_t)a * KP + r];
+        // 32-bit byte offset from the table base (n * KP * 8 < 2^32 for every n served here)
+        c[u] = *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(ct) +
+                                              ((((uint32_t)a * KP) + (uint32_t)r) << 3));
       }
       int g[UNROLL], jj[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const int i = row[u];
         const int nb = c[u].x;
-        int j = (nb >= 0 && nb < n) ? s_pos[nb] : -1;
+        const bool nbv = (unsigned)nb < (unsigned)n;
+        int j = s_pos[nbv ? nb : 0];  // read unconditionally (no branch), discarded when nb is padding
         const int lim = (i == 0) ? n - 1 : n;
-        j = (r < k && i < rows && j > i + 1 && j < lim) ? j : -1;
+        j = (nbv && r < k && i < rows && j > i + 1 && j < lim) ? j : -1;
         jj[u] = j;
         const int b = s_tour[i < rows ? i + 1 : 0];
         g[u] = dist(b, s_tour[j >= 0 ? j + 1 : 0]);
