@@ -22,12 +22,28 @@ MAX_DISTANCE = (1 << 29) - 1
 DEFAULT_MAX_MATRIX_DIMENSION = 10_000
 
 
-def coordinate_candidates_pay(n: int) -> bool:
+def coordinate_candidates_pay(n: int, tours: int | None = None) -> bool:
   """Whether the candidate-list 2-opt should compute EUC_2D distances from coordinates instead of
-  gathering them from the matrix: measured on an MI355X, the matrix form is faster while the
-  n x n int32 matrix stays cache-resident (n = 1000: 63 vs 85 ms per C2 k=20 launch) and the
-  coordinate form once it does not (n = 10000, k = 8, 256 tours, 200 passes: 9.4 vs 35.2 ms)."""
+  gathering them from the matrix.  Measured on an MI355X with identical tours:
+  * many tours (more than CUs, the throughput regime; P = 1024, k = 20, to local optimum): the
+    matrix form wins at n = 500 (2.4e11 vs 1.9e11 deltas/s), ties at n = 1000 (2.0e11 both) and
+    loses 2.5x at n = 2000 (0.84e11 vs 2.1e11), once the n x n int32 matrix outgrows an XCD's
+    4 MB L2 -- hence n > 1024;
+  * a few tours (one per CU or fewer, latency-bound): the forms run within 10 % of each other
+    until the matrix leaves the caches altogether (n = 10000, k = 8, 256 tours, 200 passes: 9.4
+    vs 35.2 ms) -- hence n > 4096."""
+  if tours is not None and tours > _cu_count():
+    return n > 1024
   return n > 4096
+
+
+def _cu_count() -> int:
+  try:
+    import torch
+
+    return int(torch.cuda.get_device_properties(0).multi_processor_count)
+  except Exception:  # no device visible: the few-tour rule applies
+    return 1 << 30
 
 
 class ProblemContext:

@@ -101,7 +101,7 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
       if table is None:
         table = context.get_gpu_candidate_table(int(self.candidate_k))
-      xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(n) else None
+      xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(n, int(tours.shape[0])) else None
       if xy is not None:  # same tours; distances from coordinates once the matrix leaves the caches
         _native.two_opt_packed_euc2d(tours, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
       else:
