@@ -25,15 +25,15 @@ DEFAULT_MAX_MATRIX_DIMENSION = 10_000
 def coordinate_candidates_pay(n: int, tours: int | None = None) -> bool:
   """Whether the candidate-list 2-opt should compute EUC_2D distances from coordinates instead of
   gathering them from the matrix.  Measured on an MI355X with identical tours:
-  * many tours (more than CUs, the throughput regime; P = 1024, k = 20, to local optimum): the
-    matrix form wins at n = 500 (2.4e11 vs 1.9e11 deltas/s), ties at n = 1000 (2.0e11 both) and
-    loses 2.5x at n = 2000 (0.84e11 vs 2.1e11), once the n x n int32 matrix outgrows an XCD's
-    4 MB L2 -- hence n > 1024;
+  * many tours (more than CUs, the throughput regime; P = 1024, k = 20): the matrix form wins at
+    n = 500 (2.4e11 vs 1.9e11 deltas/s) and loses from n = 1000 on (1.06 vs 0.90 ms per 20-pass
+    launch at n = 1000, 1.3x at P = 4096; 2.5x at n = 2000, where the n x n int32 matrix has
+    outgrown an XCD's 4 MB L2) -- hence n > 768;
   * a few tours (one per CU or fewer, latency-bound): the forms run within 10 % of each other
     until the matrix leaves the caches altogether (n = 10000, k = 8, 256 tours, 200 passes: 9.4
     vs 35.2 ms) -- hence n > 4096."""
   if tours is not None and tours > _cu_count():
-    return n > 1024
+    return n > 768
   return n > 4096
 
 

@@ -175,8 +175,10 @@ extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, con
   hipStream_t s = (hipStream_t)stream;
   const double2* p = reinterpret_cast<const double2*>(xy);
   const int2* ct = reinterpret_cast<const int2*>(table);
+  // One tour per CU: 16 waves each; up to four per CU: 8 waves (measured at P = 1024, n = 1000 and
+  // 2000, k = 20: 0.90 / 1.73 ms per 20-pass launch at 512 threads vs 0.97 / 1.93 at 256); more: 4.
   const int cus = device_cu_count();
-  const int block = P <= cus ? 1024 : (P <= 2 * cus ? 512 : 256);
+  const int block = P <= cus ? 1024 : (P <= 4 * cus ? 512 : 256);
 #define TSPMI_CXY(BL, KPV, KLV)                                                                                      \
   hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
                      stats, cost)
