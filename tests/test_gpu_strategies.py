@@ -245,6 +245,28 @@ def test_local_search_strategy_large_n_computes_distances_from_coordinates(t):
   assert ls.read_stats()[1] > 0
 
 
+def test_local_search_strategy_many_tours_coordinate_scan_matches_c_oracle(t):
+  """n = 1000 with more tours than CUs: the strategy takes the coordinate candidate scan (512-thread
+  workgroups); tours and counters equal the C oracle's."""
+  import torch
+
+  from tsp_gpu_clean_amd.context import coordinate_candidates_pay
+
+  say("LocalSearchGPUTourTwoOpt n=1000 P=300 k=20 (many-tour coordinate scan)")
+  P = max(300, torch.cuda.get_device_properties(0).multi_processor_count + 1)
+  assert coordinate_candidates_pay(1000, P)
+  rng = np.random.default_rng(41)
+  xy = rng.integers(0, 1_000_001, (1000, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("u1000", xy), seed=2)
+  routes = np.stack([rng.permutation(1000) for _ in range(P)]).astype(np.int32)
+  ls = t.LocalSearchGPUTourTwoOpt(max_iterations=15, mode="best", candidate_k=20)
+  got = ls.improve_batch(routes, ctx)
+  rows = [0, P // 2, P - 1]
+  cand = ctx.get_gpu_candidate_list(20).cpu().numpy()  # the table the strategy scans
+  want, _ = c_two_opt.improve_batch(routes[rows], ctx.get_cpu_distances(), cand, "best", 15)
+  np.testing.assert_array_equal(got[rows], want)
+
+
 def test_chapter4_operators_match_oracle_with_same_generator(t):
   """Order crossover and swap mutation strategies: the same draws and children as the oracle."""
   say("OX / swap strategies")
