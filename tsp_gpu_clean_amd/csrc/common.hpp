@@ -71,12 +71,6 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
   return v;
 }
 
-__device__ __forceinline__ int wave_sum_i32(int v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-  return v;
-}
-
 // Workgroup-wide minimum of a 64-bit key; every thread gets the result.
 // `scratch` must hold blockDim.x / 64 entries.
 __device__ __forceinline__ uint64_t block_min_u64(uint64_t v, uint64_t* scratch) {
@@ -110,7 +104,6 @@ __device__ __forceinline__ int64_t block_sum_i64(int64_t v, int64_t* scratch) {
 __device__ __forceinline__ uint64_t pack_key(int delta, uint32_t idx) {
   return ((uint64_t)((uint32_t)delta ^ 0x80000000u) << 32) | (uint64_t)idx;
 }
-__device__ __forceinline__ int key_delta(uint64_t key) { return (int)((uint32_t)(key >> 32) ^ 0x80000000u); }
 __device__ __forceinline__ uint32_t key_index(uint64_t key) { return (uint32_t)key; }
 
 constexpr uint64_t kNoMove = ~0ull;

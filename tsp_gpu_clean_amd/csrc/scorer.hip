@@ -38,8 +38,6 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-// Unit of K slot q (0..7) of lane half h in chunk (T, c).
-__device__ __forceinline__ int kunit(int T, int c, int h, int q) { return 32 * T + 16 * c + 8 * (q >> 2) + 4 * h + (q & 3); }
 // Unit (row) of accumulator element i of lane half h in output tile T.
 __device__ __forceinline__ int dunit(int T, int h, int i) { return 32 * T + 8 * (i >> 2) + 4 * h + (i & 3); }
 

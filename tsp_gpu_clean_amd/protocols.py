@@ -72,14 +72,6 @@ class RouteAssessment:
   num_vehicles: Any | None = None
 
 
-def _zeros_like_backend(objectives, dtype: str):
-  if is_gpu_array(objectives):
-    import torch
-
-    return torch.zeros(objectives.shape[0], dtype=getattr(torch, dtype), device=objectives.device)
-  return np.zeros(objectives.shape[0], dtype=getattr(np, dtype))
-
-
 def tour_assessment(costs) -> RouteAssessment:
   """Feasible assessment of Hamiltonian tours (tour_cost.py:68-82)."""
   if is_gpu_array(costs):
