@@ -57,14 +57,18 @@ def c3_coords():
   return np.rint(np.clip(xy, 0, 1e6))
 
 
-def time_two_opt(torch, native, D, start, cand, steps, warmup):
-  """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch)."""
+def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
+  """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch).  With
+  candidates and device coordinates `xy`, the scan computes distances from the coordinates."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
   if cand is not None:
     table = torch.empty((cand.shape[0], native.candidate_lanes(cand.shape[1]), 2), dtype=torch.int32, device="cuda")
     native.candidate_table(cand, D, table)
-    run = lambda: native.two_opt_packed(work, D, table, cand.shape[1], -1, stats)  # noqa: E731
+    if xy is not None:
+      run = lambda: native.two_opt_packed_euc2d(work, xy, table, cand.shape[1], -1, stats)  # noqa: E731
+    else:
+      run = lambda: native.two_opt_packed(work, D, table, cand.shape[1], -1, stats)  # noqa: E731
   else:
     run = lambda: native.two_opt(work, D, None, 0, -1, stats)  # noqa: E731
   for _ in range(warmup):
@@ -385,7 +389,7 @@ def main() -> int:
     print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20, "copies": 3}))
     return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
-  _, D, start = c2_inputs(torch, _native)
+  xy, D, start = c2_inputs(torch, _native)
   sec, deltas, moves, kms, refined = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
   print(f"bench: full neighbourhood {deltas / sec:.3e} deltas/s, {kms:.1f} ms/launch", file=sys.stderr, flush=True)
   value = deltas / sec
@@ -418,11 +422,15 @@ def main() -> int:
   }
   if not args.no_extras:
     print("bench: candidate_k=20", file=sys.stderr, flush=True)
+    from tsp_gpu_clean_amd.context import coordinate_candidates_pay
+
     cand = torch.empty((N_CITIES, 20), dtype=torch.int32, device="cuda")
     _native.knn(D, 20, cand)
-    s20, d20, _, k20ms, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1)
+    # the distance source the local-search strategy picks for this shape (coordinates here)
+    xy_dev = torch.from_numpy(xy).cuda() if coordinate_candidates_pay(N_CITIES, POP) else None
+    s20, d20, _, k20ms, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, xy_dev)
     line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
-                             "deltas_per_step": d20}
+                             "deltas_per_step": d20, "distances": "coordinates" if xy_dev is not None else "matrix"}
     print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)
     gps, gwall, gbest, gdel = time_generations(torch, _native, args.generations)
     line["gens_per_s"] = gps
