@@ -128,7 +128,8 @@ int tspmi_two_opt_packed_cost(int32_t* tours, int P, int n, const int32_t* D, in
  *   sync: device int64[528 * P], zeroed before the first call; every call leaves it ready for the
  *         next (its flags and counters cleared).  Word 528 p + 512 is set when tour p's cluster
  *         was not co-resident and its barrier timed out (that tour and its cost are then left
- *         unchanged): check it after the work completes and clear it before the next call.
+ *         unchanged): check it after the work completes and clear it before the next call (a
+ *         word still set makes that tour's cluster leave at once, its tour and cost untouched).
  *         Calls that overlap on one `sync` race.
  * Replaces: the elite local search of pipeline_steps.py:166-183 (EliteOnlyLocalSearchStep) with
  *           two_opt.py:831-899 in candidate mode on the elite rows.

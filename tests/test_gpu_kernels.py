@@ -541,6 +541,44 @@ def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters)
   np.testing.assert_array_equal(got.cpu().numpy()[rows], again)
 
 
+def test_two_opt_elite_error_word_leaves_cluster_untouched(torch_mod, nat):
+  """The failure path of the bounded cluster barrier, forced from the host: a tour whose error
+  word is set leaves at its first wait (tour and cost untouched, the word reported and kept);
+  the other tours are searched as usual, and after the word is cleared the next call refines
+  the skipped tour too."""
+  say("two_opt_elite error word")
+  n, P, k, iters = 600, 8, 12, 6
+  xy = uniform_coords(n, seed=77)
+  D, _ = device_dist(nat, torch_mod, xy)
+  cand = torch_mod.empty((n, k), dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, k, cand)
+  table = torch_mod.empty((n, nat.candidate_lanes(k), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cand, D, table)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(5), P))
+  want = cu(tours)
+  nat.two_opt_packed(want, D, table, k, iters)
+  sync = torch_mod.zeros(nat.ELITE_SYNC_WORDS * P, dtype=torch_mod.int64, device="cuda")
+  words = sync.view(P, nat.ELITE_SYNC_WORDS)
+  words[3, nat.ELITE_ERROR_WORD] = 1
+  got = cu(tours)
+  cost = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
+  nat.two_opt_elite(got, D, table, k, iters, None, cost, sync)
+  torch_mod.cuda.synchronize()
+  assert nat.elite_failed(sync)
+  g, w = got.cpu().numpy(), want.cpu().numpy()
+  np.testing.assert_array_equal(g[3], tours[3])
+  assert cost[3].item() == -1.0
+  others = [r for r in range(P) if r != 3]
+  np.testing.assert_array_equal(g[others], w[others])
+  assert words[:, 0:256:16].sum().item() == 0 and words[:, 1:256:16].sum().item() == 0
+  assert words[:, 513].sum().item() == 0
+  words[3, nat.ELITE_ERROR_WORD] = 0
+  nat.two_opt_elite(got, D, table, k, iters, None, cost, sync)
+  torch_mod.cuda.synchronize()
+  assert not nat.elite_failed(sync)
+  np.testing.assert_array_equal(got.cpu().numpy()[3], w[3])
+
+
 @pytest.mark.parametrize("n,iters,rows", [(2100, 60, 3), (3000, 40, 2), (6147, 10, 2), (10000, 6, 2), (10243, 4, 1),
                                            (12000, 3, 1)])
 def test_two_opt_wide_large_n_matches_c_oracle(torch_mod, nat, n, iters, rows):

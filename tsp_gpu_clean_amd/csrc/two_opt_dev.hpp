@@ -321,7 +321,8 @@ __device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist d
 // reaches pass t after every other one has published pass t - 1, which each does after reading
 // pass t - 2.  The wait is bounded: a workgroup that spins kClusterSpinLimit times (its cluster
 // was not co-resident) raises the tour's error word, the others see it and leave, and the tour
-// is not written; the caller reads the word.  The last workgroup to leave (a departure counter)
+// is not written; the caller reads the word.  The word is checked on the first poll of every
+// wait too, so a word left set (or set by the caller) makes the cluster leave at once.  The last workgroup to leave (a departure counter)
 // clears the slots and the counter for the next launch.
 //
 // Workspace per tour (int64 words, lines of 16): line g < G = slots of workgroup g (2 words),
@@ -434,12 +435,15 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
       bool fail = false;
       uint64_t w = kClusterNone | tag;
       for (unsigned spin = 0;; ++spin) {
+        // Every 64th poll (the first included) also reads the error word, in flight with the keys.
+        const bool check = (spin & 63) == 0;
+        const uint64_t e = check ? load_relaxed(err) : 0ull;
         if (lane < G) w = load_relaxed(ws + lane * kClusterLine + (it & 1));
-        if (__all((w & kClusterTag) == tag)) break;
-        if ((spin & 63) == 63 && (load_relaxed(err) != 0 || spin >= kClusterSpinLimit)) {
+        if (check && (e != 0 || spin >= kClusterSpinLimit)) {
           fail = true;
           break;
         }
+        if (__all((w & kClusterTag) == tag)) break;
         __builtin_amdgcn_s_sleep(1);
       }
       const uint64_t key = wave_min_u64(w & ~kClusterTag);
