@@ -368,3 +368,28 @@ def test_runner_record_for_reference_config(t):
   tour, stats = factory(43).evolve(t.ProblemContext(prob, seed=43), list(range(52)), 40, optimal_cost=7542,
                                    patience=int(2 * 52 ** 0.5))
   assert stats["best_fitness"] == rec["raw_costs"][1]
+
+
+def test_reference_cpu_hybrid_and_gpu_configs_agree(t):
+  """The reference's four algorithm configurations (CPU, HybridNaive, HybridOptimized,
+  FullGPUModular in tests/golden/algorithms.json) differ only in class names and residency; with
+  one seed they run the same GA, so their histories and best tours are identical."""
+  import json
+  import os
+
+  from tests.conftest import GOLDEN
+  from tsp_gpu_clean_amd.runner import build_algorithm
+
+  say("reference CPU / Hybrid* / FullGPUModular configs on berlin52")
+  with open(os.path.join(GOLDEN, "algorithms.json"), encoding="utf-8") as fh:
+    cfg = json.load(fh)
+  prob = t.load_tsplib_file(os.path.join(GOLDEN, "berlin52.tsp"))
+  runs = {}
+  for name in ("CPU", "HybridNaive", "HybridOptimized", "FullGPUModular"):
+    ga = build_algorithm(cfg["algorithm_configs"][name], cfg["default_params"], seed=42)
+    tour, stats = ga.evolve(t.ProblemContext(prob, seed=42), list(range(52)), 25, optimal_cost=7542, patience=10**6)
+    runs[name] = (np.asarray(tour.cpu() if hasattr(tour, "cpu") else tour), stats["convergence_history"])
+  want_tour, want_hist = runs["CPU"]
+  for name, (tour, hist) in runs.items():
+    assert hist == want_hist, name
+    np.testing.assert_array_equal(tour, want_tour, err_msg=name)

@@ -71,5 +71,29 @@ def test_record_json_and_configs(tmp_path):
   ga = build_algorithm(cfg["algorithm_configs"]["FullGPUModular"], cfg["default_params"])
   assert type(ga.crossover).__name__ == "CrossoverGPUPermOrder" and ga.population_size == 32
   assert ga.selection.k == 3 and ga.local_search.max_iterations == 5 and ga.local_search.candidate_k == 20
-  with pytest.raises(KeyError):  # host (CPU) strategy classes are the reference's, not this package's
-    build_algorithm(cfg["algorithm_configs"]["CPU"], cfg["default_params"])
+  # the reference's CPU-named classes: the same device kernels, the CPU constructors' defaults
+  cpu = build_algorithm(cfg["algorithm_configs"]["CPU"], cfg["default_params"])
+  assert type(cpu.local_search).__name__ == "LocalSearchCPUTourTwoOpt" and cpu.local_search.mode == "best"
+  assert type(cpu.selection).__name__ == "SelectionCPUTournament" and cpu.selection.k == 3
+  assert type(cpu.evaluation).__name__ == "EvaluationCPUCost" and not cpu.evaluation.return_gpu
+  for name in ("HybridNaive", "HybridOptimized"):
+    build_algorithm(cfg["algorithm_configs"][name], cfg["default_params"])
+  with pytest.raises(KeyError):  # classes outside the TSP hot path stay unregistered
+    build_algorithm({"strategies": {"selection": {"class": "SelectionCPURouletteWheel"}}}, cfg["default_params"])
+
+
+def test_cpu_named_classes_follow_the_cpu_signatures():
+  import tsp_gpu_clean_amd as t
+
+  assert t.SelectionCPUTournament().k == 3 and t.InitializationCPUTourNearestNeighbor().k == 20
+  assert not t.InitializationCPUPermRandom().return_gpu
+  with pytest.raises(ValueError):
+    t.LocalSearchCPUTourTwoOpt(mode="batch")  # GPU-only mode name
+  with pytest.raises(ValueError):
+    t.LocalSearchCPUTourTwoOpt(max_iterations=-1)
+  with pytest.raises(NotImplementedError):
+    t.EvaluationCPUCost(decoder=object())
+  for name in ("SelectionCPUTournament", "CrossoverCPUPermOrder", "CrossoverCPUPermPartiallyMapped",
+               "MutationCPUPermSwap", "MutationCPUPermInversion", "InitializationCPUPermRandom",
+               "InitializationCPUTourNearestNeighbor", "EvaluationCPUCost", "LocalSearchCPUTourTwoOpt"):
+    assert t.STRATEGY_REGISTRY[name] is getattr(t, name)

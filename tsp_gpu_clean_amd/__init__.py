@@ -18,15 +18,24 @@ from .pipeline_steps import (
 from .problem import Problem, load_tsplib_file
 from .strategies import (
   STRATEGY_REGISTRY,
+  CrossoverCPUPermOrder,
+  CrossoverCPUPermPartiallyMapped,
   CrossoverGPUPermOrder,
   CrossoverGPUPermPartiallyMapped,
+  EvaluationCPUCost,
   EvaluationGPUCost,
+  InitializationCPUPermRandom,
+  InitializationCPUTourNearestNeighbor,
   InitializationGPUPermRandom,
   InitializationGPUTourNearestNeighbor,
+  LocalSearchCPUTourTwoOpt,
   LocalSearchGPUTourTwoOpt,
   LocalSearchNoOp,
+  MutationCPUPermInversion,
+  MutationCPUPermSwap,
   MutationGPUPermInversion,
   MutationGPUPermSwap,
+  SelectionCPUTournament,
   SelectionGPUTournament,
   StrategyBase,
   register_strategy,
@@ -36,6 +45,15 @@ __all__ = [
   "PIPELINE_STEP_REGISTRY",
   "STAGE_SPECS",
   "STRATEGY_REGISTRY",
+  "CrossoverCPUPermOrder",
+  "CrossoverCPUPermPartiallyMapped",
+  "EvaluationCPUCost",
+  "InitializationCPUPermRandom",
+  "InitializationCPUTourNearestNeighbor",
+  "LocalSearchCPUTourTwoOpt",
+  "MutationCPUPermInversion",
+  "MutationCPUPermSwap",
+  "SelectionCPUTournament",
   "CrossoverGPUPermPartiallyMapped",
   "CrossoverGPUPermOrder",
   "EliteOnlyLocalSearchStep",

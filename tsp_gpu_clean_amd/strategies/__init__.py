@@ -1,5 +1,9 @@
 """Strategy registry (reference src/algorithms/strategies/__init__.py:87-207).
 
+The reference's CPU strategy names for TSP tours are registered too (``cpu_named``): they run on
+the device with the CPU classes' signatures and give the same results, as the reference's own CPU
+and GPU classes do for one seed.
+
 ``register_strategy(name, cls)`` validates reflectively that ``cls`` implements every callable
 of at least one of the six protocols, exactly like the reference, so third-party strategies
 and the HIP ones plug into the same slots.
@@ -16,6 +20,17 @@ from ..protocols import (
   SelectionStrategy,
 )
 from .base import StrategyBase
+from .cpu_named import (
+  CrossoverCPUPermOrder,
+  CrossoverCPUPermPartiallyMapped,
+  EvaluationCPUCost,
+  InitializationCPUPermRandom,
+  InitializationCPUTourNearestNeighbor,
+  LocalSearchCPUTourTwoOpt,
+  MutationCPUPermInversion,
+  MutationCPUPermSwap,
+  SelectionCPUTournament,
+)
 from .crossover import CrossoverGPUPermOrder, CrossoverGPUPermPartiallyMapped
 from .evaluation import EvaluationGPUCost
 from .initialization import InitializationGPUPermRandom, InitializationGPUTourNearestNeighbor
@@ -24,6 +39,15 @@ from .mutation import MutationGPUPermInversion, MutationGPUPermSwap
 from .selection import SelectionGPUTournament
 
 STRATEGY_REGISTRY: dict[str, type] = {
+  "CrossoverCPUPermOrder": CrossoverCPUPermOrder,
+  "CrossoverCPUPermPartiallyMapped": CrossoverCPUPermPartiallyMapped,
+  "EvaluationCPUCost": EvaluationCPUCost,
+  "InitializationCPUPermRandom": InitializationCPUPermRandom,
+  "InitializationCPUTourNearestNeighbor": InitializationCPUTourNearestNeighbor,
+  "LocalSearchCPUTourTwoOpt": LocalSearchCPUTourTwoOpt,
+  "MutationCPUPermInversion": MutationCPUPermInversion,
+  "MutationCPUPermSwap": MutationCPUPermSwap,
+  "SelectionCPUTournament": SelectionCPUTournament,
   "CrossoverGPUPermPartiallyMapped": CrossoverGPUPermPartiallyMapped,
   "CrossoverGPUPermOrder": CrossoverGPUPermOrder,
   "EvaluationGPUCost": EvaluationGPUCost,
@@ -69,6 +93,15 @@ def register_strategy(name: str, strategy_cls: type, *, overwrite: bool = False)
 
 __all__ = [
   "STRATEGY_REGISTRY",
+  "CrossoverCPUPermOrder",
+  "CrossoverCPUPermPartiallyMapped",
+  "EvaluationCPUCost",
+  "InitializationCPUPermRandom",
+  "InitializationCPUTourNearestNeighbor",
+  "LocalSearchCPUTourTwoOpt",
+  "MutationCPUPermInversion",
+  "MutationCPUPermSwap",
+  "SelectionCPUTournament",
   "CrossoverGPUPermPartiallyMapped",
   "CrossoverGPUPermOrder",
   "EvaluationGPUCost",
