@@ -125,6 +125,36 @@ def time_generations(torch, native, generations=200, warmup=5):
   return generations / wall, wall, best_cost, elite_deltas
 
 
+def time_evolve(generations: int, step_generations: int = 40) -> dict:
+  """The same GA workload through the public ``GeneticAlgorithm.evolve`` (strategies, pipeline
+  steps, incumbent and stopping rules): on the graph-captured device route, and step by step."""
+  import tsp_gpu_clean_amd as t
+
+  ctx = t.ProblemContext(t.Problem.from_coordinates("c3", c3_coords()), seed=42)
+
+  def make(device_loop):
+    ls = t.LocalSearchGPUTourTwoOpt(mode="best", max_iterations=10, candidate_k=20)
+    ev = t.EvaluationGPUCost(return_gpu=True)
+    return t.GeneticAlgorithm(population_size=POP, mutation_rate=0.05, initialization=t.InitializationGPUPermRandom(),
+                              selection=t.SelectionGPUTournament(k=5), crossover=t.CrossoverGPUPermPartiallyMapped(),
+                              mutation=t.MutationGPUPermInversion(), evaluation=ev, local_search=ls,
+                              local_search_stage="none", pipeline=[t.EliteOnlyLocalSearchStep(ls, ev, 16)],
+                              enable_phase_timing=False, device_loop=device_loop)
+
+  out = {}
+  for name, dl, gens in (("device_loop", True, generations), ("step_pipeline", False, step_generations)):
+    make(dl).evolve(ctx, list(range(N_CITIES)), 4, patience=10**9)  # warm caches and kernels
+    ga = make(dl)
+    t0 = time.perf_counter()
+    _, stats = ga.evolve(ctx, list(range(N_CITIES)), gens, patience=10**9)
+    wall = time.perf_counter() - t0
+    out[name] = {"generations": stats["generations_completed"], "wall_s": wall,
+                 "gens_per_s": stats["generations_completed"] / wall, "best_cost": stats["best_fitness"]}
+  out["chunk"] = make(True).device_loop_chunk
+  out["note"] = "wall clock of evolve() incl. initialisation, initial evaluation and host draws"
+  return out
+
+
 def time_uneven_batch(torch, generations=100):
   """Config 4: 256 instances, n uniform in [50, 2000], P=64 each, 100 graph-captured generations."""
   from tsp_gpu_clean_amd.batch import BatchedMemeticGA, c4_instances
@@ -440,6 +470,8 @@ def main() -> int:
                   "achieved_GBps": gbytes * gps / 1e9, "hbm_frac": gbytes * gps / 1e9 / HBM_PEAK_GBS,
                   "workload": "n=1000 clustered (20 Gaussian clusters, sigma 2% of 1e6), P=1024, tournament 5, PMX, "
                               "inversion 0.05, elite-16 2-opt best k=20 max_iter 10"}
+    print("bench: GA through GeneticAlgorithm.evolve", file=sys.stderr, flush=True)
+    line["ga_evolve"] = time_evolve(args.generations)
     print("bench: uneven batch (256 instances)", file=sys.stderr, flush=True)
     line["uneven_batch"] = time_uneven_batch(torch)
     if not args.no_scorer_eval:

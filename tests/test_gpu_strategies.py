@@ -393,3 +393,77 @@ def test_reference_cpu_hybrid_and_gpu_configs_agree(t):
   for name, (tour, hist) in runs.items():
     assert hist == want_hist, name
     np.testing.assert_array_equal(tour, want_tour, err_msg=name)
+
+
+def _memetic_ga(t, P, elite, *, device_loop, chunk=8, cpu_named=False, post_init=True, ls_k=20, rate=0.05):
+  if cpu_named:
+    ls = t.LocalSearchCPUTourTwoOpt(mode="best", max_iterations=10, candidate_k=ls_k)
+    ev, sel = t.EvaluationCPUCost(), t.SelectionCPUTournament(k=5)
+    init, cx, mut = t.InitializationCPUTourNearestNeighbor(k=5), t.CrossoverCPUPermPartiallyMapped(), \
+      t.MutationCPUPermInversion()
+  else:
+    ls = t.LocalSearchGPUTourTwoOpt(mode="best", max_iterations=10, candidate_k=ls_k)
+    ev, sel = t.EvaluationGPUCost(return_gpu=True), t.SelectionGPUTournament(k=5)
+    init, cx, mut = t.InitializationGPUTourNearestNeighbor(k=5), t.CrossoverGPUPermPartiallyMapped(), \
+      t.MutationGPUPermInversion()
+  steps = ([t.PostInitLocalSearchStep(ls, ev)] if post_init else []) + \
+    ([t.EliteOnlyLocalSearchStep(ls, ev, elite_count=elite)] if elite else [])
+  return t.GeneticAlgorithm(population_size=P, mutation_rate=rate, seed=42, initialization=init, selection=sel,
+                            crossover=cx, mutation=mut, evaluation=ev, local_search=ls, local_search_stage="none",
+                            pipeline=steps, device_loop=device_loop, device_loop_chunk=chunk)
+
+
+def _host(x):
+  return np.asarray(x.cpu().numpy() if hasattr(x, "cpu") else x)
+
+
+@pytest.mark.parametrize("case", ["patience_stop", "optimal_stop", "remainder_no_stop", "no_elite", "cpu_named",
+                                  "coordinates"])
+def test_evolve_device_loop_equals_step_pipeline(t, case):
+  """evolve(device_loop=True) runs graph-captured chunks of generations and must end exactly where
+  the step pipeline ends: history, stop, best tour, final population, fitness and generator.  The
+  chunk is chosen so that a stop falls inside a chunk (the rewind path) and a run without a stop
+  ends in a partial chunk (the eager remainder)."""
+  from tsp_gpu_clean_amd.engine import device_route_plan
+
+  n, P, gens, elite, kw, opt, patience = 120, 64, 40, 8, {}, None, 3
+  if case == "remainder_no_stop":
+    gens, patience = 27, 10**6
+  elif case == "no_elite":
+    elite, kw, gens, patience = None, {"post_init": False}, 21, 10**6
+  elif case == "cpu_named":
+    kw = {"cpu_named": True}
+  elif case == "coordinates":
+    n, P, gens, patience = 4500, 16, 6, 10**6
+  prob = clustered_problem(t, n, seed=7)
+  if case == "optimal_stop":
+    ga = _memetic_ga(t, P, elite, device_loop=False)
+    _, s = ga.evolve(t.ProblemContext(prob, seed=3), list(range(n)), gens, patience=10**6)
+    opt = s["convergence_history"][len(s["convergence_history"]) // 2] / 1.005  # stops within 1% mid-run
+    patience = 10**6
+  say(f"evolve device loop vs step pipeline: {case} n={n} P={P} gens={gens}")
+  ga0 = _memetic_ga(t, P, elite, device_loop=False, **kw)
+  c0 = t.ProblemContext(prob, seed=3)
+  b0, s0 = ga0.evolve(c0, list(range(n)), gens, optimal_cost=opt, patience=patience)
+  done = s0["generations_completed"]
+  if s0["stopped_early"]:
+    chunk = next(c for c in (6, 8, 10, 14) if done % c)
+  else:
+    chunk = next(c for c in (4, 8, 10) if done % c)
+  say(f"  step pipeline: {done} generations, {s0['stop_reason']}; device chunk {chunk}")
+  ga1 = _memetic_ga(t, P, elite, device_loop=True, chunk=chunk, **kw)
+  c1 = t.ProblemContext(prob, seed=3)
+  assert device_route_plan(ga1, c1, np.zeros((P, n), np.int32)) is not None
+  b1, s1 = ga1.evolve(c1, list(range(n)), gens, optimal_cost=opt, patience=patience)
+  assert "device_loop" in s1["phase_timings"] and "device_loop" not in s0["phase_timings"]
+  for key in ("convergence_history", "generations_completed", "stop_reason", "stopped_early", "best_fitness",
+              "initial_fitness"):
+    assert s1[key] == s0[key], key
+  if case in ("optimal_stop",):
+    assert s0["stop_reason"] == "within_1pct_optimal"
+  np.testing.assert_array_equal(_host(b1), _host(b0))
+  assert type(ga1.final_population) is type(ga0.final_population)
+  assert type(ga1.final_fitness) is type(ga0.final_fitness)
+  np.testing.assert_array_equal(_host(ga1.final_population), _host(ga0.final_population))
+  np.testing.assert_array_equal(_host(ga1.final_fitness), _host(ga0.final_fitness))
+  assert c1.rng.bit_generator.state == c0.rng.bit_generator.state

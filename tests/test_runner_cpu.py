@@ -97,3 +97,50 @@ def test_cpu_named_classes_follow_the_cpu_signatures():
                "MutationCPUPermSwap", "MutationCPUPermInversion", "InitializationCPUPermRandom",
                "InitializationCPUTourNearestNeighbor", "EvaluationCPUCost", "LocalSearchCPUTourTwoOpt"):
     assert t.STRATEGY_REGISTRY[name] is getattr(t, name)
+
+
+def _route_ga(t, P=64, stage="none", cx=None, mut=None, ls_kw=None, extra=None, elite=True):
+  ls = t.LocalSearchGPUTourTwoOpt(**({"mode": "best", "candidate_k": 20} | (ls_kw or {})))
+  ev = t.EvaluationGPUCost(return_gpu=True)
+  steps = [t.PostInitLocalSearchStep(ls, ev)] + ([t.EliteOnlyLocalSearchStep(ls, ev, 8)] if elite else [])
+  return t.GeneticAlgorithm(population_size=P, initialization=t.InitializationGPUPermRandom(),
+                            selection=t.SelectionGPUTournament(), crossover=cx or t.CrossoverGPUPermPartiallyMapped(),
+                            mutation=mut or t.MutationGPUPermInversion(), evaluation=ev, local_search=ls,
+                            local_search_stage=stage, pipeline=steps + (extra or []), device_loop=True)
+
+
+def test_device_route_plan_accepts_only_the_device_loop_pipeline():
+  """The graph-captured route is taken only for the exact generation the device loop runs; every
+  other pipeline, strategy or shape keeps the step pipeline (plan None)."""
+  import tsp_gpu_clean_amd as t
+  from tsp_gpu_clean_amd.engine import GENERATION_MAX_N, device_route_plan
+
+  xy = np.random.default_rng(1).integers(0, 1000, (100, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("u100", xy), seed=1)
+  pop = np.zeros((64, 100), np.int32)
+  plan = device_route_plan(_route_ga(t), ctx, pop)
+  assert plan is not None and (plan["P"], plan["k"], plan["elite"], plan["iters"], plan["cand_k"]) == (64, 5, 8, 10, 20)
+  assert device_route_plan(_route_ga(t, elite=False), ctx, pop)["elite"] is None
+  rejected = {
+    "stage": _route_ga(t, stage="post_mutation"),
+    "order crossover": _route_ga(t, cx=t.CrossoverGPUPermOrder()),
+    "swap mutation": _route_ga(t, mut=t.MutationGPUPermSwap()),
+    "first improvement": _route_ga(t, ls_kw={"mode": "first"}),
+    "full neighbourhood": _route_ga(t, ls_kw={"candidate_k": None}),
+    "candidate k > 32": _route_ga(t, ls_kw={"candidate_k": 40}),
+    "no passes": _route_ga(t, ls_kw={"max_iterations": 0}),
+    "periodic step": _route_ga(t, extra=[t.PeriodicLocalSearchStep(t.LocalSearchGPUTourTwoOpt(), 5, "offspring")]),
+  }
+  for why, ga in rejected.items():
+    assert device_route_plan(ga, ctx, pop) is None, why
+  assert device_route_plan(_route_ga(t, P=63), ctx, np.zeros((63, 100), np.int32)) is None  # odd population
+  assert device_route_plan(_route_ga(t), ctx, np.zeros((64, 99), np.int32)) is None  # not whole-instance tours
+  no_rng = t.ProblemContext(t.Problem.from_coordinates("u100", xy), seed=1)
+  no_rng.rng = None
+  assert device_route_plan(_route_ga(t), no_rng, pop) is None
+  big = np.random.default_rng(2).integers(0, 10**6, (GENERATION_MAX_N + 1, 2)).astype(np.float64)
+  bctx = t.ProblemContext(t.Problem.from_coordinates("big", big), seed=1)
+  assert device_route_plan(_route_ga(t), bctx, np.zeros((64, GENERATION_MAX_N + 1), np.int32)) is None
+  with pytest.raises(ValueError):
+    t.GeneticAlgorithm(initialization=None, selection=None, crossover=None, mutation=None, evaluation=None,
+                       device_loop=True, device_loop_chunk=7)

@@ -20,7 +20,7 @@ Whole runs of generations can be captured into one HIP graph (``capture``).
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, fields
 
 import numpy as np
 
@@ -42,8 +42,10 @@ class GenerationDraws:
   pos2: object
 
 
-def draw_generations(rng: np.random.Generator, generations: int, P: int, n: int, k: int) -> GenerationDraws:
-  """Draw G generations in the reference order from ``rng`` (advances it exactly as G steps would)."""
+def draw_generations(rng: np.random.Generator, generations: int, P: int, n: int, k: int,
+                     states: list | None = None) -> GenerationDraws:
+  """Draw G generations in the reference order from ``rng`` (advances it exactly as G steps would);
+  ``states`` receives the generator state after each generation's draws."""
   half = P // 2
   cand = np.empty((generations, P, k), np.int32)
   lo = np.zeros((generations, max(1, half)), np.int32)
@@ -64,6 +66,8 @@ def draw_generations(rng: np.random.Generator, generations: int, P: int, n: int,
     sh = rng.integers(0, n - 1, size=P, dtype=np.int32)
     p1[g] = first
     p2[g] = sh + (sh >= first).astype(np.int32)
+    if states is not None:
+      states.append(rng.bit_generator.state)
   return GenerationDraws(cand, lo, hi, probs, p1, p2)
 
 
@@ -72,6 +76,15 @@ def upload(d: GenerationDraws) -> GenerationDraws:
 
   cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
   return GenerationDraws(cu(d.tour_cand), cu(d.cut_lo), cu(d.cut_hi), cu(d.probs), cu(d.pos1), cu(d.pos2))
+
+
+def upload_into(dst: GenerationDraws, src: GenerationDraws) -> None:
+  """Copy host draws of g generations into slots 0 .. g-1 of device draws of the same shape."""
+  import torch
+
+  g = int(src.tour_cand.shape[0])
+  for f in fields(GenerationDraws):
+    getattr(dst, f.name)[:g].copy_(torch.from_numpy(np.ascontiguousarray(getattr(src, f.name))))
 
 
 class DeviceMemeticLoop:
@@ -122,8 +135,9 @@ class DeviceMemeticLoop:
   def fitness(self):
     return self.keys[self.cur][: self.P]
 
-  def step(self, draws: GenerationDraws, g: int, best_out=None) -> None:
-    """One generation using draw slot ``g``; ``best_out`` (device int32 (1,)) receives the best row.
+  def step(self, draws: GenerationDraws, g: int, best_out=None, best_tour=None, best_cost=None) -> None:
+    """One generation using draw slot ``g``; ``best_out`` (device int32 (1,)) receives the best row,
+    and ``best_tour`` (int32 (1, n)) / ``best_cost`` (float64 (1,)), when given, its tour and cost.
 
     Survivors are gathered in their stable rank order, so the elite set of the reference's
     ``argsort(fitness)[:elite]`` (pipeline_steps.py:163) is simply rows 0..elite-1: the elite
@@ -156,6 +170,8 @@ class DeviceMemeticLoop:
         _native.tour_cost(elites, self.D, kb[:E])
     if best_out is not None and not best_done:
       _native.rank_select(kb[: (self.elite or 1)], 1, best_out)
+    if best_tour is not None:
+      _native.gather_rows(self.pop[b], kb[:P], None, None, best_out, best_tour, best_cost)
     self.cur = b
 
   def verify(self) -> None:
@@ -164,12 +180,19 @@ class DeviceMemeticLoop:
     if _native.elite_failed(self.sync):
       raise RuntimeError("elite 2-opt: a workgroup cluster was not co-resident; results are invalid")
 
-  def run(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0) -> None:
-    """Run ``generations`` steps from draw slot ``start``; ``best_idx`` collects each best row."""
+  def run(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0, best_tours=None,
+          best_costs=None) -> None:
+    """Run ``generations`` steps from draw slot ``start``; ``best_idx`` collects each best row and
+    ``best_tours`` (G, n) / ``best_costs`` (G,) its tour and cost (they need ``best_idx``)."""
+    if best_tours is not None and best_idx is None:
+      raise ValueError("best_tours needs best_idx")
     for t in range(generations):
-      self.step(draws, start + t, None if best_idx is None else best_idx[t : t + 1])
+      row = slice(t, t + 1)
+      self.step(draws, start + t, None if best_idx is None else best_idx[row],
+                None if best_tours is None else best_tours[row], None if best_costs is None else best_costs[row])
 
-  def capture(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0):
+  def capture(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0, best_tours=None,
+              best_costs=None):
     """Capture ``generations`` (even) steps into one HIP graph; replaying it advances the loop.
 
     The graph reads the buffers of the current population slot and, the count being even, ends
@@ -183,6 +206,162 @@ class DeviceMemeticLoop:
     graph = torch.cuda.CUDAGraph()
     cur0 = self.cur
     with torch.cuda.graph(graph):
-      self.run(draws, generations, best_idx, start)
+      self.run(draws, generations, best_idx, start, best_tours, best_costs)
     assert self.cur == cur0
     return graph
+
+
+#: Largest n the fused generation kernel holds in LDS (six int32 rows of n, 160 KiB per CU).
+GENERATION_MAX_N = (160 * 1024 - 64) // 24
+
+
+def device_route_plan(ga, context, population) -> dict | None:
+  """Settings for ``DeviceMemeticLoop`` when ``ga``'s generation is exactly the loop's, else None.
+
+  The loop reproduces the pipeline ``[PostInitLocalSearchStep] -> tournament selection -> PMX ->
+  inversion -> offspring evaluation -> (mu + lambda) survival -> [EliteOnlyLocalSearchStep]``
+  (genetic_algorithm.py:459-482 composition, pipeline_steps.py:85-195) with every strategy
+  drawing from the context generator, whole-instance permutations, and an elite 2-opt that takes
+  the steepest-move candidate kernel.  Any other pipeline, strategy, option or shape returns
+  None and the step pipeline runs instead.
+  """
+  from .ga import CrossoverStep, EvaluationStep, MutationStep, SelectionStep, SurvivalStep
+  from .pipeline_steps import EliteOnlyLocalSearchStep, PostInitLocalSearchStep
+  from .route_helpers import resolve_route_bounds
+  from .strategies.cpu_named import (CrossoverCPUPermPartiallyMapped, EvaluationCPUCost, LocalSearchCPUTourTwoOpt,
+                                     MutationCPUPermInversion, SelectionCPUTournament)
+  from .strategies.crossover import CrossoverGPUPermPartiallyMapped
+  from .strategies.evaluation import EvaluationGPUCost
+  from .strategies.local_search import _MODES, LocalSearchGPUTourTwoOpt
+  from .strategies.mutation import MutationGPUPermInversion
+  from .strategies.selection import SelectionGPUTournament
+
+  evaluations = (EvaluationGPUCost, EvaluationCPUCost)
+  if ga.local_search is not None and ga.local_search_stage != "none":
+    return None
+  if not isinstance(getattr(context, "rng", None), np.random.Generator):
+    return None
+  if type(ga.selection) not in (SelectionGPUTournament, SelectionCPUTournament) or int(ga.selection.k) < 1:
+    return None
+  if type(ga.crossover) not in (CrossoverGPUPermPartiallyMapped, CrossoverCPUPermPartiallyMapped):
+    return None
+  if type(ga.mutation) not in (MutationGPUPermInversion, MutationCPUPermInversion):
+    return None
+  if type(ga.evaluation) not in evaluations:
+    return None
+  P, n = int(ga.population_size), int(context.dimension)
+  shape = tuple(int(x) for x in getattr(population, "shape", ()))
+  if shape != (P, n) or P < 2 or P % 2 or n < 4 or n > GENERATION_MAX_N:
+    return None
+  if resolve_route_bounds(n, context=context) != (0, n):
+    return None
+  steps = list(ga.pipeline)
+  post_init = steps.pop(0) if steps and type(steps[0]) is PostInitLocalSearchStep else None
+  elite_step = steps.pop() if steps and type(steps[-1]) is EliteOnlyLocalSearchStep else None
+  core = [SelectionStep, CrossoverStep, MutationStep, EvaluationStep, SurvivalStep]
+  if [type(s) for s in steps] != core:
+    return None
+  sel, cx, mut, ev, surv = steps
+  if (sel.strategy is not ga.selection or sel.population_size != P or cx.strategy is not ga.crossover
+      or mut.strategy is not ga.mutation or mut.mutation_rate != ga.mutation_rate or ev.strategy is not ga.evaluation
+      or ev.target != "offspring" or surv.survivor_selector != ga._select_survivors):
+    return None
+  plan = {"P": P, "rate": float(ga.mutation_rate), "k": int(ga.selection.k), "elite": None, "iters": 0,
+          "cand_k": None, "post_init": post_init}
+  if elite_step is not None:
+    ls = elite_step.local_search
+    if (type(ls) not in (LocalSearchGPUTourTwoOpt, LocalSearchCPUTourTwoOpt) or _MODES[ls.mode] != TSPMI_MODE_BEST
+        or ls.candidate_scorer is not None or int(ls.max_iterations) <= 0 or not ls._uses_candidates(context, n)
+        or not 1 <= int(ls.candidate_k) <= 32 or type(elite_step.evaluation) not in evaluations):
+      return None
+    plan.update(elite=min(int(elite_step.elite_count), P), iters=int(ls.max_iterations), cand_k=int(ls.candidate_k))
+  return plan
+
+
+class DeviceEvolution:
+  """Runs ``GeneticAlgorithm.evolve``'s generations on ``DeviceMemeticLoop`` in chunks of
+  ``chunk`` generations, each replayed from one captured HIP graph.
+
+  A chunk's draws are made on the host, in the reference order from the context generator,
+  while the previous chunk runs.  After a chunk the best cost of each of its generations is read
+  back and handed to the caller, which applies the incumbent and stopping rules
+  (genetic_algorithm.py:858-905) and names the generation to stop at, if any.  A stop inside a
+  chunk rewinds: the population saved at the chunk start is run eagerly to the stopping
+  generation and the generator is set to its state after that generation's draws, so population,
+  fitness and generator end where the step pipeline ends.
+  """
+
+  def __init__(self, context, plan: dict, population, fitness, chunk: int = 64) -> None:
+    torch = _native.require_device()
+    if chunk < 2 or chunk % 2:
+      raise ValueError(f"chunk must be an even number >= 2, got {chunk}")
+    self.loop = DeviceMemeticLoop(context, plan["P"], plan["rate"], plan["k"], plan["elite"], plan["iters"],
+                                  plan["cand_k"])
+    self.loop.load(population, fitness)
+    self.rng = context.rng
+    self.chunk = int(chunk)
+    P, n, C, k = self.loop.P, self.loop.n, self.chunk, plan["k"]
+    half = max(1, P // 2)
+    dev = "cuda"
+    self.draws = GenerationDraws(torch.empty((C, P, k), dtype=torch.int32, device=dev),
+                                 torch.empty((C, half), dtype=torch.int32, device=dev),
+                                 torch.empty((C, half), dtype=torch.int32, device=dev),
+                                 torch.empty((C, P), dtype=torch.float64, device=dev),
+                                 torch.empty((C, P), dtype=torch.int32, device=dev),
+                                 torch.empty((C, P), dtype=torch.int32, device=dev))
+    self.best_idx = torch.empty(C, dtype=torch.int32, device=dev)
+    self.best_tours = torch.empty((C, n), dtype=torch.int32, device=dev)
+    self.best_costs = torch.empty(C, dtype=torch.float64, device=dev)
+    self.saved_pop = torch.empty((P, n), dtype=torch.int32, device=dev)
+    self.saved_fit = torch.empty(P, dtype=torch.float64, device=dev)
+    self.graph = None
+    self.k = k
+
+  def _draw(self, generations: int):
+    states: list = []
+    host = draw_generations(self.rng, generations, self.loop.P, self.loop.n, self.k, states)
+    return host, states
+
+  def tour(self, t: int) -> np.ndarray:
+    """Best tour of row ``t`` of the chunk last handed to the caller."""
+    return self.best_tours[t].cpu().numpy()
+
+  def run(self, max_generations: int, walk) -> int:
+    """Run up to ``max_generations``; ``walk(first_generation, costs)`` gets each chunk's best
+    costs (float64 per generation) and returns the row to stop at, or None.  Returns the number of
+    generations completed."""
+    done = 0
+    pending = self._draw(min(self.chunk, max_generations)) if max_generations > 0 else None
+    while pending is not None:
+      host, states = pending
+      g = int(host.tour_cand.shape[0])
+      upload_into(self.draws, host)
+      self.saved_pop.copy_(self.loop.population)
+      self.saved_fit.copy_(self.loop.fitness)
+      if g == self.chunk:
+        if self.graph is None:
+          self.graph = self.loop.capture(self.draws, g, self.best_idx, 0, self.best_tours, self.best_costs)
+        self.graph.replay()
+      else:
+        self.loop.run(self.draws, g, self.best_idx[:g], 0, self.best_tours[:g], self.best_costs[:g])
+      left = max_generations - done - g
+      pending = self._draw(min(self.chunk, left)) if left > 0 else None  # overlaps the device work
+      costs = self.best_costs[:g].cpu().numpy()
+      self.loop.verify()
+      stop = walk(done, costs)
+      if stop is not None:
+        if stop + 1 < g:
+          self.loop.load(self.saved_pop, self.saved_fit)
+          self.loop.run(self.draws, stop + 1)
+        self.rng.bit_generator.state = states[stop]
+        return done + stop + 1
+      done += g
+    return done
+
+  @property
+  def population(self):
+    return self.loop.population
+
+  @property
+  def fitness(self):
+    return self.loop.fitness

@@ -213,7 +213,16 @@ class GeneticAlgorithm:
     local_search_stage: str = "post_mutation",
     enable_phase_timing: bool = True,
     pipeline: list | None = None,
+    device_loop: bool = False,
+    device_loop_chunk: int = 64,
   ) -> None:
+    """The reference constructor (genetic_algorithm.py:381-457) plus ``device_loop``: when the
+    pipeline is exactly the device memetic loop's generation (``engine.device_route_plan``),
+    ``evolve`` runs its generations as HIP graphs of ``device_loop_chunk`` generations with one
+    host read-back per chunk, with results identical to the step pipeline's; any other pipeline
+    runs step by step.  On that route the strategies' own counters and timers see only the
+    initialisation and the optional post-initialisation local search, and ``phase_timings`` holds
+    the generations under ``"device_loop"``."""
     if local_search_stage not in STAGE_SPECS:
       raise ValueError(f"Invalid local_search_stage '{local_search_stage}'. Valid stages: {list(STAGE_SPECS)}")
     self.population_size = population_size
@@ -232,6 +241,10 @@ class GeneticAlgorithm:
     self.h2d_bytes = 0
     self.d2h_bytes = 0
     self.kernel_launches = 0
+    self.device_loop = bool(device_loop)
+    self.device_loop_chunk = int(device_loop_chunk)
+    if self.device_loop and (self.device_loop_chunk < 2 or self.device_loop_chunk % 2):
+      raise ValueError(f"device_loop_chunk must be an even number >= 2, got {device_loop_chunk}")
     self.custom_pipeline_steps = list(pipeline) if pipeline is not None else []
     if pipeline is not None:
       core = any(isinstance(s, (SelectionStep, CrossoverStep, MutationStep, SurvivalStep)) for s in pipeline)
@@ -365,6 +378,55 @@ class GeneticAlgorithm:
       return cand_cost <= best_cost
     return cand_cost < best_cost - 1e-6
 
+  @staticmethod
+  def _stop_reason(best_cost: float, best_feas: bool, optimal_cost, stale: int, patience: int) -> str | None:
+    """Early-stopping rules checked after every generation (:887-905)."""
+    if best_feas and optimal_cost is not None and optimal_cost > 0:
+      if best_cost <= optimal_cost + 1e-6:
+        return "hit_optimal"
+      if (best_cost - optimal_cost) / optimal_cost * 100.0 < 1.0:
+        return "within_1pct_optimal"
+    if stale >= patience:
+      return "no_improvements"
+    return None
+
+  def _evolve_on_device(self, plan: dict, state: GenerationState, max_generations: int, incumbent: dict,
+                        optimal_cost, patience: int) -> None:
+    """Generations of ``evolve`` on the device memetic loop; updates ``state`` and ``incumbent``."""
+    from .engine import DeviceEvolution
+
+    if plan["post_init"] is not None:
+      state.generation = 1
+      plan["post_init"].run(state, self.timer)
+    pop_like, fit_like = state.population, state.fitness
+    pop_d, _, _ = to_device_int32(state.population, copy=False)
+    fit_d, _ = to_device_float64(state.fitness)
+    runner = DeviceEvolution(state.context, plan, pop_d, fit_d, self.device_loop_chunk)
+
+    def walk(first: int, costs: np.ndarray) -> int | None:
+      last = stop = None
+      for t, cost in enumerate(costs.tolist()):
+        if self._candidate_beats_incumbent(cost, True, incumbent["cost"], incumbent["feas"]):
+          incumbent["cost"], incumbent["feas"], incumbent["stale"] = cost, True, 0
+          last = t
+        else:
+          incumbent["stale"] += 1
+        self.best_cost_history.append(incumbent["cost"])
+        reason = self._stop_reason(incumbent["cost"], incumbent["feas"], optimal_cost, incumbent["stale"], patience)
+        if reason is not None:
+          incumbent["reason"], stop = reason, t
+          break
+      if last is not None:
+        incumbent["tour"] = runner.tour(last)
+      return stop
+
+    self.generation = self.timer.measure("device_loop", runner.run, max_generations, walk)
+    state.generation = self.generation
+    pop = runner.population.clone()
+    state.population = pop if is_gpu_array(pop_like) else pop.cpu().numpy()
+    state.fitness = _like(runner.fitness.clone(), fit_like)
+    state.population_assessment = tour_assessment(state.fitness)
+
   def evolve(self, context, customers: Sequence[int], max_generations: int, optimal_cost: float | None = None,
              patience: int | None = None):
     """Run the pipeline (genetic_algorithm.py:789-965); returns (best_tour, stats)."""
@@ -389,7 +451,18 @@ class GeneticAlgorithm:
     stopped_early, stop_reason = False, "max_generations"
     state = GenerationState(population=population, fitness=fitness, context=context,
                             population_assessment=pop_assessment)
-    for gen in range(max_generations):
+    plan = None
+    if self.device_loop and max_generations > 0:
+      from .engine import device_route_plan
+
+      plan = device_route_plan(self, context, population)
+    if plan is not None:
+      incumbent = {"cost": best_cost, "feas": best_feas, "tour": best_tour, "stale": 0, "reason": None}
+      self._evolve_on_device(plan, state, max_generations, incumbent, optimal_cost, patience)
+      best_cost, best_feas, best_tour = incumbent["cost"], incumbent["feas"], incumbent["tour"]
+      if incumbent["reason"] is not None:
+        stopped_early, stop_reason = True, incumbent["reason"]
+    for gen in range(0 if plan is not None else max_generations):
       self.generation = gen + 1
       state.generation = self.generation
       for step in self.pipeline:
@@ -403,15 +476,9 @@ class GeneticAlgorithm:
       else:
         stale += 1
       self.best_cost_history.append(best_cost)
-      if best_feas and optimal_cost is not None and optimal_cost > 0:
-        if best_cost <= optimal_cost + 1e-6:
-          stopped_early, stop_reason = True, "hit_optimal"
-          break
-        if (best_cost - optimal_cost) / optimal_cost * 100.0 < 1.0:
-          stopped_early, stop_reason = True, "within_1pct_optimal"
-          break
-      if stale >= patience:
-        stopped_early, stop_reason = True, "no_improvements"
+      reason = self._stop_reason(best_cost, best_feas, optimal_cost, stale, patience)
+      if reason is not None:
+        stopped_early, stop_reason = True, reason
         break
     spec = STAGE_SPECS[self.local_search_stage]
     if self.local_search is not None and spec.post_loop_hook:
