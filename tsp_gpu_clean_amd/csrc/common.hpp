@@ -9,6 +9,8 @@
 namespace tspmi {
 
 constexpr int kWave = 64;
+// Accelerator complex dies (XCDs) of an MI355X; each has its own L2 and receives workgroups in turn.
+constexpr int kXcds = 8;
 
 // Thread-local last-error text, set by every failing entry point.
 void set_error(const char* fmt, ...);

@@ -521,13 +521,23 @@ __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t*
                                                                         int max_iter, int G, uint64_t* __restrict__ sync,
                                                                         int64_t* __restrict__ stats,
                                                                         double* __restrict__ cost, int P,
-                                                                        int32_t* __restrict__ best) {
+                                                                        int32_t* __restrict__ best, int xcds) {
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[kClusterBlock / kWave];
   __shared__ int64_t red64[kClusterBlock / kWave];
   __shared__ uint64_t s_best[1];
-  const int t = blockIdx.x / G;
-  const int g = blockIdx.x - t * G;
+  // Workgroups are handed to the XCDs in turn (workgroup b to XCD b % xcds); with xcds > 1 the G
+  // workgroups of a tour are taken from one XCD's share, so a cluster's barrier traffic stays on
+  // the L2 of one XCD.  Either mapping is a bijection of blocks onto (tour, member).
+  int t, g;
+  if (xcds > 1) {
+    const int x = blockIdx.x % xcds, slot = blockIdx.x / xcds;
+    t = x * (P / xcds) + slot / G;
+    g = slot - (slot / G) * G;
+  } else {
+    t = blockIdx.x / G;
+    g = blockIdx.x - t * G;
+  }
   cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, G, g,
                                           sync + (size_t)t * kClusterWords, stats,
                                           cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best, P, sync, cost,
@@ -900,9 +910,10 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table);
   const int kp = cand_lanes(k);
+  const int xcds = (P % kXcds == 0 && cus % kXcds == 0) ? kXcds : 1;
 #define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                 \
   hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, ct, kp, \
-                     k, max_iter, G, sync, stats, cost, P, best)
+                     k, max_iter, G, sync, stats, cost, P, best, xcds)
   if (per <= 2) TSPMI_LAUNCH_CLUSTER(2);
   else if (per <= 4) TSPMI_LAUNCH_CLUSTER(4);
   else TSPMI_LAUNCH_CLUSTER(8);

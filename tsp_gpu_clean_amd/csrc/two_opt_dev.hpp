@@ -171,6 +171,30 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   }
 }
 
+// Edge costs of node-indexed records {position, successor, D[node][successor], 0} that a move left
+// stale: the edges at positions lo .. hi (none when lo > hi).  Called between a pass's distance
+// gathers and its first use of the costs, so the two sets of gathers are in flight together and a
+// pass waits on global memory once.
+template <int BLOCK, class Dist>
+__device__ __forceinline__ void refresh_edge_costs(int4* s_nd, const int32_t* s_tour, int lo, int hi,
+                                                   const Dist& dist) {
+  constexpr int ZU = 4;
+  for (int x0 = lo + (int)threadIdx.x; x0 <= hi; x0 += ZU * BLOCK) {
+    int u[ZU], z[ZU];
+#pragma unroll
+    for (int q = 0; q < ZU; ++q) {
+      const int x = x0 + q * BLOCK;
+      const bool in = x <= hi;
+      u[q] = s_tour[in ? x : 0];
+      z[q] = dist(u[q], s_tour[in ? x + 1 : 1]);
+      u[q] = in ? u[q] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < ZU; ++q)
+      if (u[q] >= 0) s_nd[u[q]].z = z[q];
+  }
+}
+
 // Candidate-list steepest 2-opt for a few tours at a time (the GA's elite set), with the pairs in
 // registers and the tour in node-indexed LDS.
 //
@@ -401,19 +425,37 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
   int64_t my_deltas = 0;
   int moves = 0, passes = 0;
   bool failed = false;
+  int zlo = 1, zhi = 0;  // positions whose edge cost the last move left stale
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
+    // Gather D[succ a][succ c] of every pair, then refresh the stale edge costs: both sets of
+    // loads are in flight together; the deltas read the costs after the barrier.
+    int gd[PP], pi[PP], pj[PP];
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+      const uint32_t w = ac[m];
+      const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
+      const int2 na = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : a]);
+      const int2 nc = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : c]);
+      pi[m] = na.x;
+      pj[m] = nc.x;
+      gd[m] = dist(na.y, nc.y);
+    }
+    if (zlo <= zhi) {
+      refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo, zhi, dist);
+      __syncthreads();
+      zlo = 1;
+      zhi = 0;
+    }
     uint64_t best = kNoMove;
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
       const uint32_t w = ac[m];
       const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
-      const int4 na = s_nd[w == 0xffffffffu ? 0 : a];
-      const int4 nc = s_nd[w == 0xffffffffu ? 0 : c];
-      const int i = na.x, j = nc.x;
+      const int i = pi[m], j = pj[m];
       const int lim = (i == 0) ? n - 1 : n;
       const bool ok = (w != 0xffffffffu) & ((unsigned)i < (unsigned)rows) & (j > i + 1) & (j < lim);
-      const int delta = dac[m] + dist(na.y, nc.y) - na.z - nc.z;
+      const int delta = dac[m] + gd[m] - s_nd[w == 0xffffffffu ? 0 : a].z - s_nd[w == 0xffffffffu ? 0 : c].z;
       my_deltas += ok;
       if (ok & (delta < 0)) {
         const uint32_t r = (uint32_t)(base + m * stride - a * k);
@@ -472,10 +514,16 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
     __syncthreads();
     for (int x = i + tid; x <= j; x += BLOCK) {
       const int u = s_tour[x], v = s_tour[x + 1];
-      s_nd[u] = make_int4(x, v, dist(u, v), 0);
+      *reinterpret_cast<int2*>(&s_nd[u]) = make_int2(x, v);  // its cost is refreshed next pass
     }
     __syncthreads();
+    zlo = i;
+    zhi = j;
     ++moves;
+  }
+  if (!failed && g == 0 && cost != nullptr && zlo <= zhi) {
+    refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo, zhi, dist);
+    __syncthreads();
   }
   if (!failed && g == 0) {
     for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
