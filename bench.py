@@ -10,7 +10,7 @@ the fused memetic generation (n=1000 clustered, P=1024, tournament 5, PMX, inver
 elite-16 2-opt k=20 max_iter 10) over 200 generations; the uneven multi-instance batch (256
 instances, n in [50, 2000], P=64, 100 graph-captured generations); and the learned candidate
 scorer's time to a 2 % gap against full-neighbourhood 2-opt at n=10000, P=256 (learned top-8 vs
-kNN-20 and kNN-8, the judge run live).
+kNN-20 and kNN-8, the full-neighbourhood yardstick run live).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-scorer-eval]
 """
@@ -189,7 +189,7 @@ def time_uneven_batch(torch, generations=100):
 
 
 def learned_candidates_eval() -> dict:
-  """Config 5: learned top-8 vs kNN-20 (and kNN-8) time to within 2 % of the full 2-opt judge."""
+  """Config 5: learned top-8 vs kNN-20 (and kNN-8) time to within 2 % of the full 2-opt yardstick."""
   from tsp_gpu_clean_amd import scorer as S
 
   sc = S.LearnedCandidateScorer.load()
@@ -204,9 +204,9 @@ def learned_candidates_eval() -> dict:
   info = getattr(sc, "info", {})
   return {
     "workload": "n=10000 uniform integer coords [0,1e6]^2 (seed 2024), P=256 greedy nearest-neighbour starts, "
-                "best-improvement 2-opt; judge = full-neighbourhood 2-opt local optimum from the same starts; "
-                "target = 1.02 x judge best; times include each scheme's candidate construction",
-    "judge_best": res["full"]["best"], "judge_seconds": res["full"]["seconds_to_local_optimum"],
+                "best-improvement 2-opt; yardstick = full-neighbourhood 2-opt local optimum from the same starts; "
+                "target = 1.02 x yardstick best; times include each scheme's candidate construction",
+    "full_best": res["full"]["best"], "full_seconds": res["full"]["seconds_to_local_optimum"],
     "full_seconds_to_gap": res["full"]["seconds_to_gap"],
     "learned8_seconds_to_gap": t_l, "knn20_seconds_to_gap": t_k,
     "knn8_seconds_to_gap": m["knn8"]["seconds_to_gap"],
@@ -401,7 +401,7 @@ def main() -> int:
                   help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
   args = ap.parse_args()
   if args.gpus != 1:
-    print(f"bench.py measures one MI355X only: --gpus {args.gpus} is not supported (every GPU box here has one GPU; "
+    print(f"bench.py measures one MI355X only: --gpus {args.gpus} is not supported (one GPU per process; "
           "tours shard by population with no exchange).", file=sys.stderr)
     return 2
 

@@ -1,8 +1,8 @@
 """Oracle: the candidate-scorer MLP, its gradient, Adam and the edge features, in NumPy float64
 (test infrastructure only).
 
-There is no reference implementation of this component (the learned scorer extends the reference;
-SURVEY config 5), so the oracle is a restatement of the published method it implements — a
+There is no reference implementation of this component (the learned scorer extends the reference
+with a new candidate method), so the oracle is a restatement of the published method it implements — a
 12 -> 128 -> 128 -> 1 ReLU MLP trained with binary cross-entropy on logits and Adam (Kingma & Ba,
 2015) — evaluated in float64 while rounding to bf16 exactly where the device kernels do:
 inputs x, the weights, relu(h1) before layer 2, the layer-2 deltas before the W2^T product and

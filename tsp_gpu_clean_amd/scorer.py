@@ -7,7 +7,7 @@ fp32 master weights, and the top-k selection that feeds the packed candidate 2-o
 Training labels are the edges of locally optimal tours produced by the batched memetic GA itself
 (``BatchedMemeticGA``: tournament, PMX, inversion, survival and candidate 2-opt to the local
 optimum).  Plain k-NN lists and the full neighbourhood stay available as the fallback and as the
-judge of the tour gap the learned lists cost.
+yardstick of the tour gap the learned lists cost.
 
 The reference has no learned component; its candidate lists are k-nearest neighbours
 (src/distances/matrix.py:140-211, used by two_opt.py:117-149).
