@@ -68,7 +68,26 @@ def device_dist(nat, torch, xy):
 # ---- distances and candidate tables -------------------------------------------------------------
 
 
-@pytest.mark.parametrize("kind", ["uniform1000", "clustered1000", "berlin52", "three_point", "float2100"])
+def near_half_integer_coords():
+  """Points whose distances sit on, or a few ulps / a hair either side of, half-integers (the
+  rounding boundary of nint), over magnitudes 1 .. 2^30, plus integer points (t^2, t) whose
+  distance from the origin sqrt(t^4 + t^2) lies just below t^2 + 1/2."""
+  pts = [(0.0, 0.0)]
+  for k in sorted({int(v) for v in np.geomspace(1, 2**30, 60)} | {3, 1000, 999_999, 2**20 + 7}):
+    h = k + 0.5
+    for d in (0.0, 1e-12, -1e-12, 1e-9, -1e-9, 1e-7, -1e-7, 1e-5, -1e-5):
+      pts.append((h + d * h, 0.0))
+    lo = hi = h
+    for _ in range(3):
+      lo, hi = np.nextafter(lo, 0.0), np.nextafter(hi, np.inf)
+      pts += [(lo, 0.0), (hi, 0.0)]
+  for t in [1, 2, 3, 10, 99, 1000, 3162, 2**15 - 1]:
+    pts += [(float(t * t), float(t)), (float(t * t + 1), float(t))]
+  return np.array(pts, dtype=np.float64)
+
+
+@pytest.mark.parametrize("kind", ["uniform1000", "clustered1000", "berlin52", "three_point", "float2100",
+                                  "near_half_integers"])
 def test_dist_euc2d_matches_oracle(torch_mod, nat, berlin52_coords, kind):
   say(f"dist_euc2d {kind}")
   xy = {
@@ -77,6 +96,7 @@ def test_dist_euc2d_matches_oracle(torch_mod, nat, berlin52_coords, kind):
     "berlin52": lambda: berlin52_coords,
     "three_point": lambda: np.array([[0.0, 0.0], [3.1, 4.0], [1.5, 2.5]]),
     "float2100": lambda: np.random.default_rng(3).random((2100, 2)) * 1000.0,
+    "near_half_integers": near_half_integer_coords,
   }[kind]()
   D, mx = device_dist(nat, torch_mod, xy)
   want = compute_distance_matrix(xy)
