@@ -19,6 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
+from .engine import capture_graph
 from .rng import draw_row_seeds, make_rng
 
 FIELDS = 9  # TSPMI_BATCH_FIELDS
@@ -200,13 +201,11 @@ class BatchedMemeticGA:
 
   def capture(self, d: BatchDraws, generations: int, best=None, start: int = 0):
     """One HIP graph for ``generations`` (even) generations; a replay continues from the current state."""
-    torch = _native.require_device()
+    _native.require_device()
     if generations % 2:
       raise ValueError("capture needs an even number of generations")
-    graph = torch.cuda.CUDAGraph()
     cur0 = self.cur
-    with torch.cuda.graph(graph):
-      self.run(d, generations, best, start)
+    graph = capture_graph(lambda: self.run(d, generations, best, start))
     assert self.cur == cur0
     return graph
 

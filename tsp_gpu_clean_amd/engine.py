@@ -42,6 +42,27 @@ class GenerationDraws:
   pos2: object
 
 
+def capture_graph(fn):
+  """Capture the device work ``fn()`` enqueues into one HIP graph (replayed on the current stream).
+
+  Captures with ``capture_begin`` on a side stream rather than the ``torch.cuda.graph`` context,
+  whose entry runs a Python garbage collection and empties torch's allocator cache (tens of ms in
+  a large process); ``fn`` must not allocate device memory.
+  """
+  torch = _native.require_device()
+  graph = torch.cuda.CUDAGraph()
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  with torch.cuda.stream(side):
+    graph.capture_begin()
+    try:
+      fn()
+    finally:
+      graph.capture_end()
+  torch.cuda.current_stream().wait_stream(side)
+  return graph
+
+
 def draw_generations(rng: np.random.Generator, generations: int, P: int, n: int, k: int,
                      states: list | None = None) -> GenerationDraws:
   """Draw G generations in the reference order from ``rng`` (advances it exactly as G steps would);
@@ -200,13 +221,11 @@ class DeviceMemeticLoop:
     replay consumes draw slots start .. start + generations - 1 again: refill them (or capture a
     whole run) to keep the random stream advancing.
     """
-    torch = _native.require_device()
+    _native.require_device()
     if generations % 2:
       raise ValueError("capture needs an even number of generations")
-    graph = torch.cuda.CUDAGraph()
     cur0 = self.cur
-    with torch.cuda.graph(graph):
-      self.run(draws, generations, best_idx, start, best_tours, best_costs)
+    graph = capture_graph(lambda: self.run(draws, generations, best_idx, start, best_tours, best_costs))
     assert self.cur == cur0
     return graph
 
