@@ -198,6 +198,21 @@ int tspmi_swap(int32_t* routes, int P, int n, const double* probs, const int32_t
                double rate, void* stream);
 
 /*
+ * Host (no device work): G generations of the draws tspmi_generation consumes, in the order the
+ * reference strategies take them from NumPy's Generator(PCG64), bit-identical to those calls:
+ * per generation cand (P, k) = integers(0, P); cut pairs: P/2 first points integers(0, n), then
+ * P/2 second points integers(0, n - 1) shifted past the first, stored as (min, max) in cut_lo /
+ * cut_hi (rows of max(1, P/2)); probs (P) = random(); pos1 (P) = integers(0, n); pos2 (P) =
+ * integers(0, n - 1) shifted past pos1.  `state` is {state_hi, state_lo, inc_hi, inc_lo,
+ * has_uint32, uinteger} of the generator (NumPy's bit_generator.state) and is advanced in place;
+ * `states_out` (nullable, G x 6) receives it after each generation.
+ * Replaces: the per-generation draws of tournament_selection.py:113, pmx_crossover.py:155 with
+ *           route_helpers.py:74-108, and inversion_mutation.py:84-92.
+ */
+int tspmi_draw_generations(uint64_t* state, int G, int P, int n, int k, int32_t* cand, int32_t* cut_lo,
+                           int32_t* cut_hi, double* probs, int32_t* pos1, int32_t* pos2, uint64_t* states_out);
+
+/*
  * Fused generation: tournament -> PMX -> inversion -> evaluate, one workgroup per child row.
  * Inputs are the host-drawn buffers of one generation: tour_cand (P, k), cut_lo/cut_hi (P/2),
  * probs/pos1/pos2 (P).  Writes offspring (P, n) and their costs off_fit (P).

@@ -144,3 +144,25 @@ def test_device_route_plan_accepts_only_the_device_loop_pipeline():
   with pytest.raises(ValueError):
     t.GeneticAlgorithm(initialization=None, selection=None, crossover=None, mutation=None, evaluation=None,
                        device_loop=True, device_loop_chunk=7)
+
+
+@pytest.mark.parametrize("P,n,k,G,pre", [(1024, 1000, 5, 3, 0), (64, 120, 5, 4, 1), (7, 5, 3, 5, 3), (2, 2, 1, 3, 0),
+                                         (1, 4, 2, 2, 1), (33, 2000, 20, 2, 2), (16, 4500, 5, 1, 5)])
+def test_native_generation_draws_equal_numpy_calls(P, n, k, G, pre):
+  """tspmi_draw_generations (host C++) gives the NumPy Generator(PCG64) calls' values, end state
+  and per-generation states, from fresh states and from ones holding a buffered 32-bit half."""
+  from tsp_gpu_clean_amd.engine import draw_generations, draw_generations_numpy
+
+  for seed in (0, 42, 2**63 + 5):
+    a, b = np.random.default_rng(seed), np.random.default_rng(seed)
+    for r in (a, b):
+      r.integers(0, 1000, size=pre, dtype=np.int32)  # odd counts leave has_uint32 = 1
+      r.random(pre)
+    sa, sb = [], []
+    da = draw_generations(a, G, P, n, k, sa)
+    db = draw_generations_numpy(b, G, P, n, k, sb)
+    for name in ("tour_cand", "cut_lo", "cut_hi", "probs", "pos1", "pos2"):
+      np.testing.assert_array_equal(getattr(da, name), getattr(db, name), err_msg=name)
+    assert a.bit_generator.state == b.bit_generator.state
+    assert sa == sb
+    assert a.integers(0, 2**31 - 1, 8).tolist() == b.integers(0, 2**31 - 1, 8).tolist()

@@ -13,6 +13,8 @@ import os
 import threading
 from typing import Any
 
+import numpy as np
+
 from .errors import GPUUnavailableError
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -41,6 +43,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
   "tspmi_generation": [_P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P],
+  "tspmi_draw_generations": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_rank_select": [_P, _I, _I, _P, _P],
   "tspmi_gather_rows": [_P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P],
   "tspmi_scatter_rows": [_P, _P, _P, _I, _I, _P, _P, _P],
@@ -392,6 +395,37 @@ def generation(pop, fitness, D, tour_cand, cut_lo, cut_hi, probs, pos1, pos2, ra
   _check(_lib.tspmi_generation(_ptr(pop), _ptr(fitness), P, n, _ptr(D), D.shape[1], _ptr(tour_cand),
                                tour_cand.shape[1], _ptr(cut_lo), _ptr(cut_hi), _ptr(probs), _ptr(pos1), _ptr(pos2),
                                float(rate), _ptr(offspring), _ptr(off_fit), _stream(torch)), "tspmi_generation")
+
+
+_M64 = (1 << 64) - 1
+
+
+def draw_generations_pcg64(state: dict, G: int, P: int, n: int, k: int, record_states: bool = False):
+  """Host draws of ``G`` generations from a PCG64 generator state (``bit_generator.state``), as
+  tspmi_draw_generations makes them; no device work.  Returns (new state, cand (G, P, k),
+  cut_lo, cut_hi (G, max(1, P // 2)), probs, pos1, pos2 (G, P), states (G, 6) or None)."""
+  load_library()
+  inner = state["state"]
+  words = np.array([inner["state"] >> 64, inner["state"] & _M64, inner["inc"] >> 64, inner["inc"] & _M64,
+                    int(state["has_uint32"]), int(state["uinteger"])], dtype=np.uint64)
+  half = max(1, P // 2)
+  cand = np.empty((G, P, k), np.int32)
+  lo, hi = np.empty((G, half), np.int32), np.empty((G, half), np.int32)
+  probs = np.empty((G, P), np.float64)
+  p1, p2 = np.empty((G, P), np.int32), np.empty((G, P), np.int32)
+  states = np.empty((G, 6), np.uint64) if record_states else None
+  _check(_lib.tspmi_draw_generations(words.ctypes.data, G, P, n, k, cand.ctypes.data, lo.ctypes.data, hi.ctypes.data,
+                                     probs.ctypes.data, p1.ctypes.data, p2.ctypes.data,
+                                     None if states is None else states.ctypes.data), "tspmi_draw_generations")
+  return pcg64_state(words, state), cand, lo, hi, probs, p1, p2, states
+
+
+def pcg64_state(words, like: dict) -> dict:
+  """The ``bit_generator.state`` dict of the six words {state_hi, state_lo, inc_hi, inc_lo,
+  has_uint32, uinteger}."""
+  w = [int(x) for x in words]
+  return {"bit_generator": like["bit_generator"], "state": {"state": (w[0] << 64) | w[1], "inc": (w[2] << 64) | w[3]},
+          "has_uint32": w[4], "uinteger": w[5]}
 
 
 def rank_select(keys, keep: int, order) -> None:

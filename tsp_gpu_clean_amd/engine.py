@@ -66,7 +66,26 @@ def capture_graph(fn):
 def draw_generations(rng: np.random.Generator, generations: int, P: int, n: int, k: int,
                      states: list | None = None) -> GenerationDraws:
   """Draw G generations in the reference order from ``rng`` (advances it exactly as G steps would);
-  ``states`` receives the generator state after each generation's draws."""
+  ``states`` receives the generator state after each generation's draws.
+
+  A PCG64 generator (NumPy's default) is drawn by ``tspmi_draw_generations`` in one call, with
+  the same values and end state as the per-call NumPy loop below, which serves other bit
+  generators.
+  """
+  state = rng.bit_generator.state
+  if state.get("bit_generator") == "PCG64" and generations > 0:
+    new, cand, lo, hi, probs, p1, p2, words = _native.draw_generations_pcg64(state, generations, P, n, k,
+                                                                             states is not None)
+    rng.bit_generator.state = new
+    if states is not None:
+      states.extend(_native.pcg64_state(w, state) for w in words)
+    return GenerationDraws(cand, lo, hi, probs, p1, p2)
+  return draw_generations_numpy(rng, generations, P, n, k, states)
+
+
+def draw_generations_numpy(rng: np.random.Generator, generations: int, P: int, n: int, k: int,
+                           states: list | None = None) -> GenerationDraws:
+  """``draw_generations`` by one NumPy call per reference draw (any bit generator)."""
   half = P // 2
   cand = np.empty((generations, P, k), np.int32)
   lo = np.zeros((generations, max(1, half)), np.int32)
