@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .engine import capture_graph
+from .engine import capture_graph, draw_generations
 from .rng import draw_row_seeds, make_rng
 
 FIELDS = 9  # TSPMI_BATCH_FIELDS
@@ -152,22 +152,15 @@ class BatchedMemeticGA:
     probs = np.empty((G, B, P), np.float64)
     p1 = np.empty((G, B, P), np.int32)
     p2 = np.empty((G, B, P), np.int32)
-    for i, rng in enumerate(self.rngs):
-      n = int(self.ns[i])
-      for g in range(G):
-        cand[g, i] = rng.integers(0, P, size=(P, k), dtype=np.int32).reshape(-1)
-        if half > 0:
-          first = rng.integers(0, n, size=half, dtype=np.int32)
-          sh = rng.integers(0, n - 1, size=half, dtype=np.int32)
-          second = sh + (sh >= first).astype(np.int32)
-          lo[g, i, :half] = np.minimum(first, second)
-          hi[g, i, :half] = np.maximum(first, second)
-        probs[g, i] = rng.random(P)
-        first = rng.integers(0, n, size=P, dtype=np.int32)
-        sh = rng.integers(0, n - 1, size=P, dtype=np.int32)
-        p1[g, i] = first
-        p2[g, i] = sh + (sh >= first).astype(np.int32)
     hp = max(1, half)
+    for i, rng in enumerate(self.rngs):
+      one = draw_generations(rng, G, P, int(self.ns[i]), k)  # the instance's generator, generation order
+      cand[:, i] = one.tour_cand.reshape(G, -1)
+      lo[:, i, :hp] = one.cut_lo
+      hi[:, i, :hp] = one.cut_hi
+      probs[:, i] = one.probs
+      p1[:, i] = one.pos1
+      p2[:, i] = one.pos2
     return BatchDraws(_cu(cand.reshape(G, -1)), _cu(lo[:, :, :hp].reshape(G, -1)), _cu(hi[:, :, :hp].reshape(G, -1)),
                       _cu(probs.reshape(G, -1)), _cu(p1.reshape(G, -1)), _cu(p2.reshape(G, -1)))
 
