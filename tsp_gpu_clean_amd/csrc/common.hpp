@@ -110,6 +110,13 @@ __device__ __forceinline__ uint32_t key_index(uint64_t key) { return (uint32_t)k
 
 constexpr uint64_t kNoMove = ~0ull;
 
+// Unsigned bits of a double that order as the double does (-0.0 and +0.0 equal, NaN after +inf).
+__device__ __forceinline__ uint64_t orderable_bits(double v) {
+  if (v == 0.0) v = 0.0;
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
 // Node ids outside [0, limit) are an invalid input; clamping keeps every access in bounds.
 __device__ __forceinline__ int clamp_id(int v, int limit) { return (unsigned)v < (unsigned)limit ? v : 0; }
 

@@ -26,6 +26,31 @@ __device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ can
   return w;
 }
 
+// tournament_winner by one wave: lane c loads candidate c and its fitness, so the k candidate
+// loads and the k fitness loads are each issued together instead of as k dependent pairs; the
+// winner is the lowest-indexed candidate of least fitness (the sequential scan's first strict
+// minimum, for finite fitness).  Every lane returns it.
+__device__ __forceinline__ int wave_tournament(const int32_t* __restrict__ cand_row, int k,
+                                               const double* __restrict__ fitness, int P) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t best = ~0ull;
+  int w = 0;
+  for (int c0 = 0; c0 < k; c0 += kWave) {
+    const int c = c0 + lane;
+    const int x = c < k ? clamp_id(cand_row[c], P) : 0;
+    const uint64_t f = c < k ? orderable_bits(fitness[x]) : ~0ull;
+    const uint64_t m = wave_min_u64(f);
+    const uint64_t hit = __ballot(c < k && f == m);
+    const int first = __builtin_ctzll(hit);
+    const int xw = __shfl(x, first);
+    if (c0 == 0 || m < best) {
+      best = m;
+      w = xw;
+    }
+  }
+  return w;
+}
+
 // PMX child of (p1, p2) with cuts [c1, c2] into `child` (pmx_crossover.py:49-105, open
 // permutation).  inv1/inv2 map node -> position in p1/p2.  Caller synchronises afterwards.
 //
@@ -106,6 +131,33 @@ __device__ __forceinline__ void load_parent(const int32_t* __restrict__ src, int
   }
 }
 
+// Both parents of a pair (rows a and b) into LDS with their inverses; four elements of each row per
+// thread are loaded before any is stored, so a round waits on global memory once.
+__device__ __forceinline__ void load_parents(const int32_t* __restrict__ a, const int32_t* __restrict__ b, int n,
+                                             int32_t* p1, int32_t* inv1, int32_t* p2, int32_t* inv2) {
+  constexpr int U = 4;
+  for (int x0 = threadIdx.x; x0 < n; x0 += U * blockDim.x) {
+    int va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * blockDim.x;
+      va[u] = x < n ? a[x] : 0;
+      vb[u] = x < n ? b[x] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * blockDim.x;
+      if (x < n) {
+        const int v1 = clamp_id(va[u], n), v2 = clamp_id(vb[u], n);
+        p1[x] = v1;
+        inv1[v1] = x;
+        p2[x] = v2;
+        inv2[v2] = x;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void reverse_range(int32_t* t, int lo, int hi) {
   const int half = (hi - lo + 1) >> 1;
   for (int q = threadIdx.x; q < half; q += blockDim.x) {
@@ -117,10 +169,22 @@ __device__ __forceinline__ void reverse_range(int32_t* t, int lo, int hi) {
 
 template <class Dist>
 __device__ __forceinline__ double block_cost(const int32_t* t, int n, const Dist& dist, int64_t* red) {
+  // four edges per thread per round, their distance gathers issued together
+  constexpr int U = 4;
   int64_t s = 0;
-  for (int x = threadIdx.x; x < n; x += blockDim.x) {
-    const int nx = (x + 1 == n) ? 0 : x + 1;
-    if (n > 1) s += dist(t[x], t[nx]);
+  if (n > 1) {
+    for (int x0 = threadIdx.x; x0 < n; x0 += U * blockDim.x) {
+      int d[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = x0 + u * blockDim.x;
+        const int xs = x < n ? x : 0;
+        const int nx = (xs + 1 == n) ? 0 : xs + 1;
+        d[u] = dist(t[xs], t[nx]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) s += (x0 + u * (int)blockDim.x < n) ? d[u] : 0;
+    }
   }
   return (double)block_sum_i64(s, red);
 }
@@ -173,8 +237,11 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
   int32_t* ch = lds + 4 * n;
   int32_t* jump = lds + 5 * n;
 
-  if (threadIdx.x < 2 && (threadIdx.x == 0 || !single))
-    s_w[threadIdx.x] = tournament_winner(tour_cand + (size_t)(r0 + threadIdx.x) * k, k, fitness, P);
+  const int wid = threadIdx.x / kWave;
+  if (wid < 2 && (wid == 0 || !single)) {
+    const int w = wave_tournament(tour_cand + (size_t)(r0 + wid) * k, k, fitness, P);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[wid] = w;
+  }
   __syncthreads();
   if (single) {
     const int32_t* src = pop + (size_t)s_w[0] * n;
@@ -183,8 +250,7 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
     finish_child(r, ch, n, dist, probs, pos1, pos2, rate, offspring, off_fit, red);
     return;
   }
-  load_parent(pop + (size_t)s_w[0] * n, n, p1, inv1);
-  load_parent(pop + (size_t)s_w[1] * n, n, p2, inv2);
+  load_parents(pop + (size_t)s_w[0] * n, pop + (size_t)s_w[1] * n, n, p1, inv1, p2, inv2);
   __syncthreads();
   int c1 = cut_lo[q], c2 = cut_hi[q];
   if (c1 > c2) {
@@ -200,12 +266,6 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
   finish_child(r, ch, n, dist, probs, pos1, pos2, rate, offspring, off_fit, red);
 }
 
-
-__device__ __forceinline__ uint64_t orderable_bits(double v) {
-  if (v == 0.0) v = 0.0;  // -0.0 and +0.0 compare equal, as in the rank kernel
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
 
 // Bitonic sort of (orderable bits, index) for M <= M2 <= kSortMax keys by one workgroup of kSortBlock
 // threads; order[r] = index of rank r for r < keep.

@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t*
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[kClusterBlock / kWave];
   __shared__ int64_t red64[kClusterBlock / kWave];
-  __shared__ uint64_t s_best[1];
+  __shared__ uint64_t s_best[2];  // the pass's move; the last-tour flag
   // Workgroups are handed to the XCDs in turn (workgroup b to XCD b % xcds); with xcds > 1 the G
   // workgroups of a tour are taken from one XCD's share, so a cluster's barrier traffic stays on
   // the L2 of one XCD.  Either mapping is a bijection of blocks onto (tour, member).

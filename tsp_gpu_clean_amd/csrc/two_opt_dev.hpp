@@ -45,6 +45,30 @@ __device__ __forceinline__ int row_at(const int32_t* __restrict__ row, int node)
   return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(row) + ((uint32_t)node << 2));
 }
 
+// Tour (n node ids) into LDS, s_tour[n] mirroring s_tour[0]; four ids per thread are loaded before
+// any is stored, so a round waits on global memory once.
+template <int BLOCK>
+__device__ __forceinline__ void load_tour_lds(const int32_t* __restrict__ tour, int n, int32_t* s_tour) {
+  constexpr int U = 4;
+  for (int x0 = (int)threadIdx.x; x0 < n; x0 += U * BLOCK) {
+    int v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * BLOCK;
+      v[u] = x < n ? tour[x] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * BLOCK;
+      if (x < n) {
+        const int c = clamp_id(v[u], n);
+        s_tour[x] = c;
+        if (x == 0) s_tour[n] = c;
+      }
+    }
+  }
+}
+
 // Lanes per table row of cand_two_opt_tour for k candidates: the smallest instantiated width >= k.
 inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ? 12 : k <= 16 ? 16 : k <= 20 ? 20 : 32; }
 
@@ -86,16 +110,28 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   const int wid = tid / kWave;
   const int sub = lane / KL;       // row offset inside the wave step (RPW for a spare lane)
   const int r = lane - sub * KL;   // candidate rank handled by this lane
-  for (int x = tid; x < n; x += BLOCK) {
-    const int v = clamp_id(tour[x], n);
-    s_tour[x] = v;
-    if (x == 0) s_tour[n] = v;
-  }
+  load_tour_lds<BLOCK>(tour, n, s_tour);
   __syncthreads();
-  for (int x = tid; x < n; x += BLOCK) {
-    const int a = s_tour[x];
-    s_edge[x] = dist(a, s_tour[x + 1]);
-    s_pos[a] = x;
+  {
+    // four positions per thread per round, their distance gathers issued together
+    constexpr int U = 4;
+    for (int x0 = tid; x0 < n; x0 += U * BLOCK) {
+      int a[U], d[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int xs = x0 + u * BLOCK < n ? x0 + u * BLOCK : 0;
+        a[u] = s_tour[xs];
+        d[u] = dist(a[u], s_tour[xs + 1]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = x0 + u * BLOCK;
+        if (x < n) {
+          s_edge[x] = d[u];
+          s_pos[a[u]] = x;
+        }
+      }
+    }
   }
   __syncthreads();
 
@@ -171,6 +207,29 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   }
 }
 
+// Node-indexed records {position, successor, D[node][successor], 0} of the tour in s_tour, four
+// positions per thread per round with their distance gathers issued together.
+template <int BLOCK, class Dist>
+__device__ __forceinline__ void build_node_records(int4* s_nd, const int32_t* s_tour, int n, const Dist& dist) {
+  constexpr int U = 4;
+  for (int x0 = (int)threadIdx.x; x0 < n; x0 += U * BLOCK) {
+    int a[U], b[U], d[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * BLOCK;
+      const int xs = x < n ? x : 0;
+      a[u] = s_tour[xs];
+      b[u] = s_tour[xs + 1];
+      d[u] = dist(a[u], b[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = x0 + u * BLOCK;
+      if (x < n) s_nd[a[u]] = make_int4(x, b[u], d[u], 0);
+    }
+  }
+}
+
 // Edge costs of node-indexed records {position, successor, D[node][successor], 0} that a move left
 // stale: the edges at positions lo .. hi (none when lo > hi).  Called between a pass's distance
 // gathers and its first use of the costs, so the two sets of gathers are in flight together and a
@@ -220,11 +279,7 @@ __device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist d
   int4* s_nd = reinterpret_cast<int4*>(lds);  // n
   int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
   const int tid = threadIdx.x;
-  for (int x = tid; x < n; x += BLOCK) {
-    const int v = clamp_id(tour[x], n);
-    s_tour[x] = v;
-    if (x == 0) s_tour[n] = v;
-  }
+  load_tour_lds<BLOCK>(tour, n, s_tour);
   uint32_t ac[PP];
   int dac[PP];
   const int total = n * k;
@@ -257,10 +312,7 @@ __device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist d
     dac[m] = ok ? e[m].y : 0;
   }
   __syncthreads();
-  for (int x = tid; x < n; x += BLOCK) {
-    const int a = s_tour[x], b = s_tour[x + 1];
-    s_nd[a] = make_int4(x, b, dist(a, b), 0);
-  }
+  build_node_records<BLOCK>(s_nd, s_tour, n, dist);
   __syncthreads();
 
   const int rows = n - 2;
@@ -379,17 +431,17 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
   int4* s_nd = reinterpret_cast<int4*>(lds);  // n
   int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
   const int tid = threadIdx.x;
-  for (int x = tid; x < n; x += BLOCK) {
-    const int v = clamp_id(tour[x], n);
-    s_tour[x] = v;
-    if (x == 0) s_tour[n] = v;
-  }
   const int total = n * k;
   const int base = g * BLOCK + tid;
   const int stride = G * BLOCK;
+  // first pair index of this thread's wave: pair slots m with wave_base + m * stride >= total hold
+  // no pair in any lane of the wave and are skipped with a wave-uniform branch
+  const int wave_base = __builtin_amdgcn_readfirstlane(g * BLOCK + (tid & ~(kWave - 1)));
   uint32_t ac[PP];
   int dac[PP];
   {
+    // The table loads, then the tour loads, go out before anything waits: the setup pays one
+    // global round trip for both, and one more for the edge costs below.
     int2 e[PP];
     int pa[PP];
     const int qb = stride / k, rb = stride - qb * k;
@@ -406,6 +458,7 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
         ++a;
       }
     }
+    load_tour_lds<BLOCK>(tour, n, s_tour);
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
       const int p = base + m * stride;
@@ -415,10 +468,7 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
     }
   }
   __syncthreads();
-  for (int x = tid; x < n; x += BLOCK) {
-    const int a = s_tour[x], b = s_tour[x + 1];
-    s_nd[a] = make_int4(x, b, dist(a, b), 0);
-  }
+  build_node_records<BLOCK>(s_nd, s_tour, n, dist);
   __syncthreads();
 
   const int rows = n - 2;
@@ -428,11 +478,29 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
   int zlo = 1, zhi = 0;  // positions whose edge cost the last move left stale
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
-    // Gather D[succ a][succ c] of every pair, then refresh the stale edge costs: both sets of
-    // loads are in flight together; the deltas read the costs after the barrier.
+    // The stale edge costs' first refresh round (positions zlo + tid + q * BLOCK, q < ZU) is
+    // loaded first, then D[succ a][succ c] of every pair, and only then are the refreshed costs
+    // stored: the store waits for the refresh loads alone, so both sets of loads are in flight
+    // together and a pass waits on global memory once.  The deltas read the costs after the
+    // barrier.  Segments longer than ZU * BLOCK finish in refresh_edge_costs.
+    constexpr int ZU = 4;
+    const bool stale = zlo <= zhi;
+    int zu[ZU], zd[ZU];
+    if (stale) {
+#pragma unroll
+      for (int q = 0; q < ZU; ++q) {
+        const int x = zlo + tid + q * BLOCK;
+        const bool in = x <= zhi;
+        const int u = s_tour[in ? x : 0];
+        zd[q] = dist(u, s_tour[in ? x + 1 : 1]);
+        zu[q] = in ? u : -1;
+      }
+    }
     int gd[PP], pi[PP], pj[PP];
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
+      pi[m] = pj[m] = gd[m] = 0;
+      if (wave_base + m * stride >= total) continue;  // no lane of this wave holds pair m
       const uint32_t w = ac[m];
       const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
       const int2 na = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : a]);
@@ -441,8 +509,11 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
       pj[m] = nc.x;
       gd[m] = dist(na.y, nc.y);
     }
-    if (zlo <= zhi) {
-      refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo, zhi, dist);
+    if (stale) {
+#pragma unroll
+      for (int q = 0; q < ZU; ++q)
+        if (zu[q] >= 0) s_nd[zu[q]].z = zd[q];
+      if (zhi - zlo >= ZU * BLOCK) refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo + ZU * BLOCK, zhi, dist);
       __syncthreads();
       zlo = 1;
       zhi = 0;
@@ -450,6 +521,7 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
     uint64_t best = kNoMove;
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
+      if (wave_base + m * stride >= total) continue;
       const uint32_t w = ac[m];
       const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
       const int i = pi[m], j = pj[m];
@@ -534,22 +606,34 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
       if (tid == 0) store_relaxed(reinterpret_cast<uint64_t*>(cost), (uint64_t)__double_as_longlong((double)tot));
     }
   }
-  if (best != nullptr && cost_all != nullptr && g == 0 && tid == 0) {
+  if (best != nullptr && cost_all != nullptr && g == 0) {
     uint64_t* count = ws0 + 2 * kClusterMaxG * kClusterLine + 2;
-    __builtin_amdgcn_s_waitcnt(0);  // this tour's cost is stored before it is counted
-    const uint64_t before = __hip_atomic_fetch_add(count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (before == (uint64_t)(P - 1)) {
+    if (tid == 0) {
+      __builtin_amdgcn_s_waitcnt(0);  // this tour's cost is stored before it is counted
+      const uint64_t before = __hip_atomic_fetch_add(count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_best[1] = before == (uint64_t)(P - 1) ? 1ull : 0ull;
+    }
+    __syncthreads();
+    if (s_best[1] != 0 && tid < kWave) {
+      // The last tour takes the first minimum of the P costs, one wave loading them together.
+      uint64_t lo = ~0ull;
       int arg = 0;
-      double lo = 0.0;
-      for (int t = 0; t < P; ++t) {
-        const double c = __longlong_as_double((long long)load_relaxed(reinterpret_cast<uint64_t*>(cost_all + t)));
-        if (t == 0 || c < lo) {
-          lo = c;
-          arg = t;
+      for (int t0 = 0; t0 < P; t0 += kWave) {
+        const int t = t0 + tid;
+        const uint64_t key =
+            t < P ? orderable_bits(__longlong_as_double((long long)load_relaxed(reinterpret_cast<uint64_t*>(cost_all + t))))
+                  : ~0ull;
+        const uint64_t m = wave_min_u64(key);
+        const int first = __builtin_ctzll(__ballot(t < P && key == m));
+        if (t0 == 0 || m < lo) {
+          lo = m;
+          arg = t0 + first;
         }
       }
-      best[0] = arg;
-      store_relaxed(count, 0ull);
+      if (tid == 0) {
+        best[0] = arg;
+        store_relaxed(count, 0ull);
+      }
     }
   }
   if (stats != nullptr) {
