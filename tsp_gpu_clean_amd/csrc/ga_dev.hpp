@@ -178,12 +178,14 @@ __device__ __forceinline__ double block_cost(const int32_t* t, int n, const Dist
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int x = x0 + u * blockDim.x;
-        const int xs = x < n ? x : 0;
-        const int nx = (xs + 1 == n) ? 0 : xs + 1;
-        d[u] = dist(t[xs], t[nx]);
+        d[u] = 0;
+        if (x < n) {  // a branch, not a select: a distance from coordinates is costly
+          const int nx = (x + 1 == n) ? 0 : x + 1;
+          d[u] = dist(t[x], t[nx]);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) s += (x0 + u * (int)blockDim.x < n) ? d[u] : 0;
+      for (int u = 0; u < U; ++u) s += d[u];
     }
   }
   return (double)block_sum_i64(s, red);

@@ -119,9 +119,12 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
       int a[U], d[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int xs = x0 + u * BLOCK < n ? x0 + u * BLOCK : 0;
-        a[u] = s_tour[xs];
-        d[u] = dist(a[u], s_tour[xs + 1]);
+        const int x = x0 + u * BLOCK;
+        a[u] = d[u] = 0;
+        if (x < n) {  // a branch, not a select: a distance from coordinates is costly
+          a[u] = s_tour[x];
+          d[u] = dist(a[u], s_tour[x + 1]);
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -217,10 +220,12 @@ __device__ __forceinline__ void build_node_records(int4* s_nd, const int32_t* s_
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int x = x0 + u * BLOCK;
-      const int xs = x < n ? x : 0;
-      a[u] = s_tour[xs];
-      b[u] = s_tour[xs + 1];
-      d[u] = dist(a[u], b[u]);
+      a[u] = b[u] = d[u] = 0;
+      if (x < n) {
+        a[u] = s_tour[x];
+        b[u] = s_tour[x + 1];
+        d[u] = dist(a[u], b[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
