@@ -321,15 +321,16 @@ class DeviceEvolution:
   ``chunk`` generations, each replayed from one captured HIP graph.
 
   A chunk's draws are made on the host, in the reference order from the context generator,
-  while the previous chunk runs.  After a chunk the best cost of each of its generations is read
-  back and handed to the caller, which applies the incumbent and stopping rules
+  while the previous chunk runs; the first chunk runs eagerly while the graph is captured.  After
+  a chunk the best cost of each of its generations is read back and handed to the caller, which
+  applies the incumbent and stopping rules
   (genetic_algorithm.py:858-905) and names the generation to stop at, if any.  A stop inside a
   chunk rewinds: the population saved at the chunk start is run eagerly to the stopping
   generation and the generator is set to its state after that generation's draws, so population,
   fitness and generator end where the step pipeline ends.
   """
 
-  def __init__(self, context, plan: dict, population, fitness, chunk: int = 64) -> None:
+  def __init__(self, context, plan: dict, population, fitness, chunk: int = 32) -> None:
     torch = _native.require_device()
     if chunk < 2 or chunk % 2:
       raise ValueError(f"chunk must be an even number >= 2, got {chunk}")
@@ -376,13 +377,14 @@ class DeviceEvolution:
       upload_into(self.draws, host)
       self.saved_pop.copy_(self.loop.population)
       self.saved_fit.copy_(self.loop.fitness)
-      if g == self.chunk:
-        if self.graph is None:
-          self.graph = self.loop.capture(self.draws, g, self.best_idx, 0, self.best_tours, self.best_costs)
+      if g == self.chunk and self.graph is not None:
         self.graph.replay()
-      else:
+      else:  # the first chunk, and a short last one, run eagerly
         self.loop.run(self.draws, g, self.best_idx[:g], 0, self.best_tours[:g], self.best_costs[:g])
       left = max_generations - done - g
+      if self.graph is None and left >= self.chunk:
+        # capturing records work without running it: it overlaps the eager chunk on the device
+        self.graph = self.loop.capture(self.draws, self.chunk, self.best_idx, 0, self.best_tours, self.best_costs)
       pending = self._draw(min(self.chunk, left)) if left > 0 else None  # overlaps the device work
       costs = self.best_costs[:g].cpu().numpy()
       self.loop.verify()

@@ -214,7 +214,7 @@ class GeneticAlgorithm:
     enable_phase_timing: bool = True,
     pipeline: list | None = None,
     device_loop: bool = False,
-    device_loop_chunk: int = 64,
+    device_loop_chunk: int = 32,
   ) -> None:
     """The reference constructor (genetic_algorithm.py:381-457) plus ``device_loop``: when the
     pipeline is exactly the device memetic loop's generation (``engine.device_route_plan``),
