@@ -6,7 +6,7 @@ that the HIP kernels of ``tsp_gpu_clean_amd`` must reproduce bit for bit.  Every
 function cites the reference ``file:line`` it follows.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
-may import it, and only as the checker or the timed CPU baseline.  The product
+may import it, and only as the parity reference or the timed CPU baseline.  The product
 package never imports it and has no CPU fallback.
 
 Pinning (what checks each part independently of its own formula):

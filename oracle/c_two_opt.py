@@ -2,7 +2,7 @@
 
 The C restatement follows the same reference lines as ``oracle/two_opt.py`` for the
 cyclic TSP topology; tests cross-check the two on random inputs before either is
-trusted as the checker of the HIP kernels.
+trusted as the parity reference of the HIP kernels.
 """
 
 from __future__ import annotations
