@@ -408,9 +408,10 @@ def _memetic_ga(t, P, elite, *, device_loop, chunk=8, cpu_named=False, post_init
       t.MutationGPUPermInversion()
   steps = ([t.PostInitLocalSearchStep(ls, ev)] if post_init else []) + \
     ([t.EliteOnlyLocalSearchStep(ls, ev, elite_count=elite)] if elite else [])
+  extra = {} if chunk is None else {"device_loop_chunk": chunk}
   return t.GeneticAlgorithm(population_size=P, mutation_rate=rate, seed=42, initialization=init, selection=sel,
                             crossover=cx, mutation=mut, evaluation=ev, local_search=ls, local_search_stage="none",
-                            pipeline=steps, device_loop=device_loop, device_loop_chunk=chunk)
+                            pipeline=steps, device_loop=device_loop, **extra)
 
 
 def _host(x):
@@ -418,7 +419,7 @@ def _host(x):
 
 
 @pytest.mark.parametrize("case", ["patience_stop", "optimal_stop", "remainder_no_stop", "no_elite", "cpu_named",
-                                  "coordinates"])
+                                  "coordinates", "default_chunk"])
 def test_evolve_device_loop_equals_step_pipeline(t, case):
   """evolve(device_loop=True) runs graph-captured chunks of generations and must end exactly where
   the step pipeline ends: history, stop, best tour, final population, fitness and generator.  The
@@ -435,6 +436,8 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
     kw = {"cpu_named": True}
   elif case == "coordinates":
     n, P, gens, patience = 4500, 16, 6, 10**6
+  elif case == "default_chunk":  # the constructor's chunk: eager first chunk, captured replays, partial last
+    n, P, gens, patience = 150, 64, 90, 10**6
   prob = clustered_problem(t, n, seed=7)
   if case == "optimal_stop":
     ga = _memetic_ga(t, P, elite, device_loop=False)
@@ -446,7 +449,9 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
   c0 = t.ProblemContext(prob, seed=3)
   b0, s0 = ga0.evolve(c0, list(range(n)), gens, optimal_cost=opt, patience=patience)
   done = s0["generations_completed"]
-  if s0["stopped_early"]:
+  if case == "default_chunk":
+    chunk = None
+  elif s0["stopped_early"]:
     chunk = next(c for c in (6, 8, 10, 14) if done % c)
   else:
     chunk = next(c for c in (4, 8, 10) if done % c)
