@@ -418,8 +418,17 @@ def _host(x):
   return np.asarray(x.cpu().numpy() if hasattr(x, "cpu") else x)
 
 
+def _same_state(a, b) -> bool:
+  """Generator states compared field by field (some bit generators hold arrays)."""
+  if isinstance(a, dict):
+    return isinstance(b, dict) and a.keys() == b.keys() and all(_same_state(a[k], b[k]) for k in a)
+  if isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+    return bool(np.array_equal(a, b))
+  return a == b
+
+
 @pytest.mark.parametrize("case", ["patience_stop", "optimal_stop", "remainder_no_stop", "no_elite", "cpu_named",
-                                  "coordinates", "default_chunk"])
+                                  "coordinates", "default_chunk", "philox_rng"])
 def test_evolve_device_loop_equals_step_pipeline(t, case):
   """evolve(device_loop=True) runs graph-captured chunks of generations and must end exactly where
   the step pipeline ends: history, stop, best tour, final population, fitness and generator.  The
@@ -438,6 +447,8 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
     n, P, gens, patience = 4500, 16, 6, 10**6
   elif case == "default_chunk":  # the constructor's chunk: eager first chunk, captured replays, partial last
     n, P, gens, patience = 150, 64, 90, 10**6
+  elif case == "philox_rng":  # a non-PCG64 context generator takes the per-call NumPy draws
+    gens = 30
   prob = clustered_problem(t, n, seed=7)
   if case == "optimal_stop":
     ga = _memetic_ga(t, P, elite, device_loop=False)
@@ -445,8 +456,14 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
     opt = s["convergence_history"][len(s["convergence_history"]) // 2] / 1.005  # stops within 1% mid-run
     patience = 10**6
   say(f"evolve device loop vs step pipeline: {case} n={n} P={P} gens={gens}")
+  def context():
+    c = t.ProblemContext(prob, seed=3)
+    if case == "philox_rng":
+      c.rng = np.random.Generator(np.random.Philox(3))
+    return c
+
   ga0 = _memetic_ga(t, P, elite, device_loop=False, **kw)
-  c0 = t.ProblemContext(prob, seed=3)
+  c0 = context()
   b0, s0 = ga0.evolve(c0, list(range(n)), gens, optimal_cost=opt, patience=patience)
   done = s0["generations_completed"]
   if case == "default_chunk":
@@ -457,7 +474,7 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
     chunk = next(c for c in (4, 8, 10) if done % c)
   say(f"  step pipeline: {done} generations, {s0['stop_reason']}; device chunk {chunk}")
   ga1 = _memetic_ga(t, P, elite, device_loop=True, chunk=chunk, **kw)
-  c1 = t.ProblemContext(prob, seed=3)
+  c1 = context()
   assert device_route_plan(ga1, c1, np.zeros((P, n), np.int32)) is not None
   b1, s1 = ga1.evolve(c1, list(range(n)), gens, optimal_cost=opt, patience=patience)
   assert "device_loop" in s1["phase_timings"] and "device_loop" not in s0["phase_timings"]
@@ -471,4 +488,4 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
   assert type(ga1.final_fitness) is type(ga0.final_fitness)
   np.testing.assert_array_equal(_host(ga1.final_population), _host(ga0.final_population))
   np.testing.assert_array_equal(_host(ga1.final_fitness), _host(ga0.final_fitness))
-  assert c1.rng.bit_generator.state == c0.rng.bit_generator.state
+  assert _same_state(c1.rng.bit_generator.state, c0.rng.bit_generator.state)
