@@ -142,8 +142,9 @@ def time_evolve(generations: int, step_generations: int = 40) -> dict:
                               enable_phase_timing=False, device_loop=device_loop)
 
   out = {}
-  for name, dl, gens in (("device_loop", True, generations), ("step_pipeline", False, step_generations)):
-    make(dl).evolve(ctx, list(range(N_CITIES)), 4, patience=10**9)  # warm caches and kernels
+  legs = (("device_loop", True, generations), ("device_loop_1000", True, 1000), ("step_pipeline", False, step_generations))
+  for name, dl, gens in legs:
+    make(dl).evolve(ctx, list(range(N_CITIES)), 40, patience=10**9)  # warm caches, kernels and capture
     ga = make(dl)
     t0 = time.perf_counter()
     _, stats = ga.evolve(ctx, list(range(N_CITIES)), gens, patience=10**9)
