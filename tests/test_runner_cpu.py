@@ -166,3 +166,18 @@ def test_native_generation_draws_equal_numpy_calls(P, n, k, G, pre):
     assert a.bit_generator.state == b.bit_generator.state
     assert sa == sb
     assert a.integers(0, 2**31 - 1, 8).tolist() == b.integers(0, 2**31 - 1, 8).tolist()
+
+
+def test_native_generation_draws_reject_bad_shapes():
+  """tspmi_draw_generations refuses shapes the reference calls could not make (no population,
+  fewer than two cities, no tournament candidates) and leaves the generator untouched."""
+  from tsp_gpu_clean_amd import _native
+
+  rng = np.random.default_rng(9)
+  before = rng.bit_generator.state
+  for G, P, n, k in ((1, 0, 10, 5), (1, 8, 1, 5), (1, 8, 10, 0), (-1, 8, 10, 5)):
+    with pytest.raises(ValueError):
+      _native.draw_generations_pcg64(before, G, P, n, k)
+  assert rng.bit_generator.state == before
+  new, cand, *_ = _native.draw_generations_pcg64(before, 0, 8, 10, 5)
+  assert new == before and cand.shape == (0, 8, 5)
