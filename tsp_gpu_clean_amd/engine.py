@@ -324,7 +324,7 @@ class DeviceEvolution:
   while the previous chunk runs; the first chunk runs eagerly while the graph is captured.  After
   a chunk the best cost of each of its generations is read back and handed to the caller, which
   applies the incumbent and stopping rules
-  (genetic_algorithm.py:858-905) and names the generation to stop at, if any.  A stop inside a
+  (genetic_algorithm.py:876-910) and names the generation to stop at, if any.  A stop inside a
   chunk rewinds: the population saved at the chunk start is run eagerly to the stopping
   generation and the generator is set to its state after that generation's draws, so population,
   fitness and generator end where the step pipeline ends.

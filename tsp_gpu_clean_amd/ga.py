@@ -380,7 +380,7 @@ class GeneticAlgorithm:
 
   @staticmethod
   def _stop_reason(best_cost: float, best_feas: bool, optimal_cost, stale: int, patience: int) -> str | None:
-    """Early-stopping rules checked after every generation (:887-905)."""
+    """Early-stopping rules checked after every generation (:894-910)."""
     if best_feas and optimal_cost is not None and optimal_cost > 0:
       if best_cost <= optimal_cost + 1e-6:
         return "hit_optimal"
