@@ -229,6 +229,25 @@ def test_two_opt_n1000_to_local_optimum_matches_c_oracle(torch_mod, nat, k, mode
   np.testing.assert_array_equal(ops.tour_costs(got, d), ops.tour_costs(want, d))
 
 
+@pytest.mark.parametrize("n,iters", [(63, -1), (66, -1), (257, -1), (258, 60), (700, 40), (1025, 30), (1026, 30),
+                                     (1027, 30), (1283, 20), (1500, 20), (2050, 12)])
+def test_two_opt_full_best_slot_phases_match_c_oracle(torch_mod, nat, n, iters):
+  """Full-neighbourhood steepest scan around its slot-phase boundaries (n - 2 crossing multiples
+  of 64 and 256, both register widths) and on a tie-heavy lattice: tours and counters exact."""
+  say(f"two_opt full best phases n={n} iters={iters}")
+  d = compute_distance_matrix(uniform_coords(n, seed=n + 3))
+  tours = oinit.random_init(np.arange(n), 3, oinit.draw_row_seeds(np.random.default_rng(n), 3))
+  got, stats = _run_two_opt(nat, torch_mod, tours, d, None, "best", iters)
+  want, wstats = c_two_opt.improve_batch(tours, d, None, "best", iters)
+  np.testing.assert_array_equal(got, want)
+  assert list(stats) == list(wstats)
+  lat = lattice_matrix(n)
+  got, stats = _run_two_opt(nat, torch_mod, tours, lat, None, "best", iters)
+  want, wstats = c_two_opt.improve_batch(tours, lat, None, "best", iters)
+  np.testing.assert_array_equal(got, want)
+  assert list(stats) == list(wstats)
+
+
 def test_two_opt_full_population_is_locally_optimal(torch_mod, nat):
   """P=1024, n=1000, k=20 to local optimum: permutations, no improving candidate pair left."""
   say("two_opt P=1024 n=1000 k=20 to optimum")

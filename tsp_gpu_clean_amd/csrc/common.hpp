@@ -38,6 +38,10 @@ int device_cu_count();
 // Largest dynamic LDS a single workgroup may request on CDNA4 (160 KiB per CU).
 constexpr int kMaxLds = 160 * 1024;
 
+// Index of the calling wave in its workgroup, in a scalar register: every lane of a wave holds
+// the same value, and saying so lets loops bounded by it run on scalar counters.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); }
+
 // ---- wave64 reductions -------------------------------------------------------
 
 // One DPP step of a 64-bit minimum: lanes of the rows in `row_mask` take min(v, v of the lane that

@@ -22,70 +22,109 @@ namespace tspmi {
 
 // Steepest scan of the full neighbourhood with a diagonal lane mapping.
 //
-// Lane (wave w, lane L) owns the j offsets j = i + 2 + 64*(w + 4q) + L of every row i.  Moving from
-// row i to row i+1 shifts each owned j by one, so the lane's second gather of row i,
-// D[t_{i+1}, t_{j+1}], is exactly its first gather of row i+1, D[t_{i+1}, t_{j'}] with j' = j+1: it is
-// carried in a register and every delta costs one distance gather instead of two.  The gathers of
-// row i+1 are issued before row i is evaluated (one-row software pipeline).  The loop body is
-// branch-free so every load is unconditional and the pipeline really overlaps: s_tour and s_edge
-// carry `pad` zero entries past their ends (node 0 is a valid gather target), lanes past the row
-// end compute a harmless delta that the select discards.  The distance-row base is wave-uniform and
-// the gather uses a 32-bit unsigned byte offset.  s_tour[n] mirrors s_tour[0], so the successor of
-// position n-1 needs no wrap test.  Each lane visits its pairs in increasing scan order, so a
-// strict `<` keeps the first minimum per lane and the packed-key block minimum then restores the
-// global lowest-(i, j) tie-break.
+// Lane L of wave w owns, in slot q, the j offset j - i = 2 + off(w, q) + L of every row i, where
+// the 64-lane groups off(w, q) cover 0 .. 256*QMAX - 1 (diag_slot_offset).  Moving from row i to
+// row i+1 shifts each owned j by one, so the lane's second gather of row i, D[t_{i+1}, t_{j+1}], is
+// exactly its first gather of row i+1, D[t_{i+1}, t_{j'}] with j' = j+1: it is carried in a register
+// and every delta costs one distance gather instead of two.  The gathers of row i+1 are issued
+// before row i is evaluated (one-row software pipeline).  The loop body is branch-free so every
+// load is unconditional and the pipeline really overlaps: s_tour and s_edge carry `pad` zero
+// entries past their ends (node 0 is a valid gather target), lanes past the row end compute a
+// harmless delta that the select discards.  The distance-row base is wave-uniform and the gather
+// uses a 32-bit unsigned byte offset.  s_tour[n] mirrors s_tour[0], so the successor of position
+// n-1 needs no wrap test.
+//
+// Row i has pairs only up to j = n-1, so slot q of wave w holds a pair only while
+// i < (n-2) - off(w, q).  The rows are walked in QMAX phases: phase QL evaluates the QL lowest
+// slots of the wave, up to the row where slot QL-1 runs out, so no slot is evaluated past the
+// last row in which one of its lanes is admissible (at n = 1000: 530,432 slot evaluations per
+// pass for 498,500 pairs, against 923,648 when every slot runs to the wave's last row).  The
+// snake order of off() over the waves gives every wave the same number of live slot rows.
+//
+// Each lane visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per
+// lane and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
+__device__ __forceinline__ int diag_slot_offset(int w, int q) {
+  return kWave * (kTwoOptWaves * q + ((q & 1) ? (kTwoOptWaves - 1 - w) : w));
+}
+
 template <int QMAX>
-__device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
-                                                        const int32_t* __restrict__ D, int ld, int wid, int lane) {
-  const int rows = n - 2;
-  const int jq0 = 2 + wid * kWave + lane;  // lane's j at row 0 for q = 0; q adds 256
-  int carry[QMAX], gcur[QMAX], gnext[QMAX], bd[QMAX], bi[QMAX];
-  {
-    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
-    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
-#pragma unroll
-    for (int q = 0; q < QMAX; ++q) {
-      const int j = jq0 + q * kTwoOptBlock;
-      carry[q] = row_at(r0, s_tour[j]);
-      gcur[q] = row_at(r1, s_tour[j + 1]);
-      bd[q] = 0;
-      bi[q] = 0;
-    }
-  }
-  // Row i, lane slot q evaluates pair (i, j = i + jq0 + 256 q).  Slots past the row end read the
-  // sentinel edge padding, which makes their delta positive, so only row 0's excluded pair
-  // (0, n-1) needs an explicit test.  Per slot the first minimum (strict <) and its row are kept.
-  auto row_step = [&](int i, bool first_row) {
+struct DiagScan {
+  const int32_t* s_tour;
+  const int32_t* s_edge;
+  const int32_t* __restrict__ D;
+  int ld, n, wid;
+  int jo_even, jo_odd;  // j - i of the lane in even / odd slots, without the 256 q term
+  int carry[QMAX], gcur[QMAX], bd[QMAX], bi[QMAX];
+
+  __device__ __forceinline__ int jo(int q) const { return ((q & 1) ? jo_odd : jo_even) + q * kTwoOptBlock; }
+
+  // Row i over the QL lowest slots; FIRST marks row 0, whose pair (0, n-1) is excluded.
+  template <int QL, bool FIRST>
+  __device__ __forceinline__ void row(int i) {
     const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
-    const int* tq = s_tour + i + jq0 + 2;
+    int gnext[QL];
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) gnext[q] = row_at(rn, tq[q * kTwoOptBlock]);
+    for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, s_tour[i + jo(q) + 2]);
     const int neg_eab = -s_edge[i];
-    const int* eq = s_edge + i + jq0;
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) {
-      int delta = carry[q] + gcur[q] + neg_eab - eq[q * kTwoOptBlock];
-      if (first_row) delta = (jq0 + q * kTwoOptBlock < n - 1) ? delta : 1;
+    for (int q = 0; q < QL; ++q) {
+      int delta = carry[q] + gcur[q] + neg_eab - s_edge[i + jo(q)];
+      if (FIRST) delta = (jo(q) < n - 1) ? delta : 1;
       const bool take = delta < bd[q];
       bd[q] = take ? delta : bd[q];
       bi[q] = take ? i : bi[q];
       carry[q] = gcur[q];
       gcur[q] = gnext[q];
     }
-  };
-  row_step(0, true);
-  int i = 1;
-#pragma unroll 2
-  for (; i < rows; ++i) {
-    if (i + 2 + kWave * wid >= n) break;  // this wave owns no pair of row i or later
-    row_step(i, false);
   }
+
+  // Phases QL, QL-1, ..., 1 from row i on.
+  template <int QL>
+  __device__ __forceinline__ void phases(int i) {
+    if constexpr (QL > 0) {
+      const int end = (n - 2) - diag_slot_offset(wid, QL - 1);
+      for (; i + 1 < end; i += 2) {  // two rows per trip: the carry moves become register renames
+        row<QL, false>(i);
+        row<QL, false>(i + 1);
+      }
+      if (i < end) row<QL, false>(i++);
+      phases<QL - 1>(i);
+    }
+  }
+};
+
+template <int QMAX>
+__device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
+                                                        const int32_t* __restrict__ D, int ld, int wid, int lane) {
+  DiagScan<QMAX> sc;
+  sc.s_tour = s_tour;
+  sc.s_edge = s_edge;
+  sc.D = D;
+  sc.ld = ld;
+  sc.n = n;
+  sc.wid = wid;
+  sc.jo_even = 2 + diag_slot_offset(wid, 0) + lane;
+  sc.jo_odd = 2 + diag_slot_offset(wid, 1) - kTwoOptBlock + lane;
+  {
+    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
+    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int j = sc.jo(q);
+      sc.carry[q] = row_at(r0, s_tour[j]);
+      sc.gcur[q] = row_at(r1, s_tour[j + 1]);
+      sc.bd[q] = 0;
+      sc.bi[q] = 0;
+    }
+  }
+  sc.template row<QMAX, true>(0);
+  sc.template phases<QMAX>(1);
   uint64_t best = kNoMove;
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) {
-    if (bd[q] < 0) {
-      const uint32_t r = (uint32_t)bi[q];
-      const uint64_t key = pack_key(bd[q], r * (uint32_t)n + r + (uint32_t)(jq0 + q * kTwoOptBlock));
+    if (sc.bd[q] < 0) {
+      const uint32_t r = (uint32_t)sc.bi[q];
+      const uint64_t key = pack_key(sc.bd[q], r * (uint32_t)n + r + (uint32_t)sc.jo(q));
       best = key < best ? key : best;
     }
   }
@@ -155,7 +194,7 @@ __global__ __launch_bounds__(kWideBlock) void two_opt_wide_kernel(int32_t* __res
   int32_t* s_edge = lds + n + 1;  // n
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
+  const int wid = wave_id();
   int32_t* tour = tours + (size_t)blockIdx.x * n;
   for (int x = tid; x < n; x += kWideBlock) {
     const int v = clamp_id(tour[x], ld);
@@ -285,7 +324,7 @@ __global__ __launch_bounds__(kWideBlock) void two_opt_staged_kernel(int32_t* __r
   int32_t* s_row = s_edge + n;    // n
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
+  const int wid = wave_id();
   int32_t* tour = tours + (size_t)blockIdx.x * n;
   for (int x = tid; x < n; x += kWideBlock) {
     const int v = clamp_id(tour[x], n);
@@ -331,7 +370,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
+  const int wid = wave_id();
   int32_t* tour = tours + (size_t)blockIdx.x * n;
 
   for (int x = tid; x < n; x += blockDim.x) {

@@ -107,7 +107,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   int32_t* s_pos = s_edge + n;     // n
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
+  const int wid = wave_id();
   const int sub = lane / KL;       // row offset inside the wave step (RPW for a spare lane)
   const int r = lane - sub * KL;   // candidate rank handled by this lane
   load_tour_lds<BLOCK>(tour, n, s_tour);

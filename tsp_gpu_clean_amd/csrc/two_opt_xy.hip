@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
   int32_t* s_edge = lds + n + 1;  // n
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wid = tid / kWave;
+  const int wid = wave_id();
   const CoordDist dist{xy};
   int32_t* tour = tours + (size_t)blockIdx.x * n;
   for (int x = tid; x < n; x += BLOCK) {
