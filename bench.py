@@ -32,6 +32,13 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_PER_DELTA = 8  # two int32 distance gathers per 2-opt delta
+CUS = 256  # MI355X compute units
+PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+# Random 4-byte lanes per clock per CU that the texture addresser (TA) takes in when the 64 lanes of
+# a gather fall at random inside one 4 KB row: one such wave64 load occupies the TA for 32 cycles
+# (TA_TA_BUSY / TA_FLAT_READ_WAVEFRONTS of the scan, and the tspmi_probe_gather micro-benchmark).
+GATHER_LANES_PER_CLK_CU = 2.0
+GATHER_PEAK = GATHER_LANES_PER_CLK_CU * CUS * PEAK_CLOCK_HZ  # lanes/s
 N_CITIES = 1000
 POP = 1024
 
@@ -49,6 +56,47 @@ def c2_inputs(torch, native):
   return xy, D, tours
 
 
+def diag_gather_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4) -> int:
+  """Distance-gather lanes one pass of the full-neighbourhood steepest scan issues for one tour
+  (two_opt_kernel with 256 threads): the carry set-up (two loads per slot lane), row 0 over every
+  slot, then per wave the slot phases -- slot q of wave w runs while i < (n-2) - off(w, q), with
+  off(w, q) = 64 (4q + w) on even q and 64 (4q + 3 - w) on odd q."""
+  rows = n - 2
+  slot_rows = 0
+  for w in range(waves):
+    off = [64 * (waves * q + ((waves - 1 - w) if q & 1 else w)) for q in range(qmax)]
+    slot_rows += qmax
+    i = 1
+    for ql in range(qmax, 0, -1):
+      end = rows - off[ql - 1]
+      if end > i:
+        slot_rows += ql * (end - i)
+        i = end
+  return 64 * slot_rows + 2 * qmax * 64 * waves
+
+
+def gather_probe(torch, native, reps: int = 3) -> dict:
+  """Live rate of random 4-byte gathers inside 4 KB rows (tspmi_probe_gather), 4 workgroups per CU."""
+  rows = torch.randint(0, 1 << 20, (64, 1024), dtype=torch.int32, device="cuda")
+  out = torch.zeros(1, dtype=torch.int32, device="cuda")
+  blocks, iters = 4 * CUS, 2048
+  native.probe_gather(rows, blocks, 8, out)
+  stream = torch.cuda.current_stream()
+  best = None
+  for _ in range(reps):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    native.probe_gather(rows, blocks, iters, out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    best = ms if best is None else min(best, ms)
+  lanes = blocks * 256 * iters * 8
+  rate = lanes / (best * 1e-3)
+  return {"lanes_per_s": rate, "lanes_per_clk_per_cu_at_peak_clock": rate / (CUS * PEAK_CLOCK_HZ), "ms": best,
+          "shape": f"{blocks} workgroups x 256 threads x {iters * 8} random 4-byte loads in 4 KB rows"}
+
+
 def c3_coords():
   rng = np.random.default_rng(42)
   centres = rng.uniform(0, 1e6, (20, 2))
@@ -58,7 +106,8 @@ def c3_coords():
 
 
 def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
-  """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch).  With
+  """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch, refined tours,
+  passes per step).  With
   candidates and device coordinates `xy`, the scan computes distances from the coordinates."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
@@ -88,7 +137,7 @@ def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
   wall = time.perf_counter() - t0
   kernel_ms = sum(a.elapsed_time(b) for a, b in ev) / steps
   s = stats.cpu().tolist()
-  return wall / steps, s[0] / steps, s[1] / steps, kernel_ms, work
+  return wall / steps, s[0] / steps, s[1] / steps, kernel_ms, work, s[2] / steps
 
 
 def time_generations(torch, native, generations=200, warmup=5):
@@ -421,10 +470,13 @@ def main() -> int:
     return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
   xy, D, start = c2_inputs(torch, _native)
-  sec, deltas, moves, kms, refined = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
+  sec, deltas, moves, kms, refined, passes = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
   print(f"bench: full neighbourhood {deltas / sec:.3e} deltas/s, {kms:.1f} ms/launch", file=sys.stderr, flush=True)
   value = deltas / sec
   achieved = deltas * BYTES_PER_DELTA / (kms * 1e-3) / 1e9
+  traffic = load_traffic()
+  lanes = passes * diag_gather_lanes_per_pass(N_CITIES)
+  probe = gather_probe(torch, _native)
   line = {
     "metric": "2-opt deltas/s & gens/s (n=1000,pop=1024); time-to-2% gap, learned vs plain kNN",
     "value": value,
@@ -446,9 +498,24 @@ def main() -> int:
     },
     "roofline": {
       "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-      "traffic": load_traffic(),
-      "kernel": "tspmi::two_opt_kernel", "kernel_ms_per_launch": kms,
+      "traffic": traffic,
+      "traffic_source": "stored calibration, not measured in this run: profiles/two_opt_c2_traffic.json "
+                        "(rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch, scaled by a 256 MiB copy)",
+      "measured_hbm_GBps": (traffic / (kms * 1e-3) / 1e9) if traffic else None,
+      "measured_hbm_frac": (traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+      "note": "achieved counts the algorithmic 8 B per delta; the 4 MB distance matrix stays in the caches, so the "
+              "HBM moves only `traffic` bytes per launch and the binding unit is on chip (roofline_onchip)",
+      "kernel": "tspmi::two_opt_kernel<4>", "kernel_ms_per_launch": kms,
       "bytes_per_unit": BYTES_PER_DELTA, "units_per_launch": deltas,
+    },
+    "roofline_onchip": {
+      "bound": "texture-address gather issue (TA)", "achieved": lanes / (kms * 1e-3), "peak": GATHER_PEAK,
+      "unit": "gather-lanes/s", "frac": lanes / (kms * 1e-3) / GATHER_PEAK,
+      "peak_basis": f"{GATHER_LANES_PER_CLK_CU:g} random 4-byte lanes per clock per CU (a wave64 gather at random "
+                    f"offsets of one 4 KB row occupies the TA 32 cycles) x {CUS} CUs x {PEAK_CLOCK_HZ / 1e9:g} GHz",
+      "lanes_per_launch": lanes, "lanes_per_pass_per_tour": diag_gather_lanes_per_pass(N_CITIES),
+      "passes_per_launch": passes, "probe": probe,
+      "counters": "profiles/two_opt_c2_counters.md (TA busy fraction, L1 hit rate, VALU and LDS issue)",
     },
   }
   if not args.no_extras:
@@ -459,7 +526,7 @@ def main() -> int:
     _native.knn(D, 20, cand)
     # the distance source the local-search strategy picks for this shape (coordinates here)
     xy_dev = torch.from_numpy(xy).cuda() if coordinate_candidates_pay(N_CITIES, POP) else None
-    s20, d20, _, k20ms, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, xy_dev)
+    s20, d20, _, k20ms, _, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, xy_dev)
     line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
                              "deltas_per_step": d20, "distances": "coordinates" if xy_dev is not None else "matrix"}
     print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)

@@ -39,6 +39,16 @@ const char* tspmi_last_error(void);
 int tspmi_device_count(int* count);
 
 /*
+ * Calibration probe of random 4-byte gathers inside 4 KB rows (the distance-gather shape of the
+ * full-neighbourhood 2-opt scan): `blocks` workgroups of 256 threads, each thread issuing
+ * 8 * iters loads at pseudo-random offsets of row (block % nrows) of `rows` (nrows x 1024 int32).
+ * `out` (one int32) is written only if an impossible checksum appears.  Used to price the
+ * on-chip gather roofline of the 2-opt scan.
+ * No reference counterpart (the reference measures no hardware ceilings).
+ */
+int tspmi_probe_gather(const int32_t* rows, int nrows, int blocks, int iters, int32_t* out, void* stream);
+
+/*
  * EUC_2D TSPLIB nint distances: D[u,v] = (int)(sqrt(dx^2 + dy^2) + 0.5) in FP64 with a
  * correctly rounded sqrt, diagonal 0.  xy is (n, 2) float64.  Writes max(D) to *max_out
  * (device int32, may be NULL).

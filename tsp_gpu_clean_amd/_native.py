@@ -31,6 +31,7 @@ _D = ctypes.c_double
 _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_version": [],
   "tspmi_device_count": [_P],
+  "tspmi_probe_gather": [_P, _I, _I, _I, _P, _P],
   "tspmi_dist_euc2d": [_P, _I, _P, _I, _P, _P],
   "tspmi_knn": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
@@ -172,6 +173,17 @@ def dist_euc2d(xy, D, max_out=None) -> None:
   if max_out is not None:
     _need(max_out, torch.int32, "max_out")
   _check(_lib.tspmi_dist_euc2d(_ptr(xy), n, _ptr(D), D.shape[1], _ptr(max_out), _stream(torch)), "tspmi_dist_euc2d")
+
+
+def probe_gather(rows, blocks: int, iters: int, out) -> None:
+  """Random 4-byte gathers inside the 4 KB rows of `rows` ((nrows, 1024) int32); calibration only."""
+  torch = require_device()
+  _need(rows, torch.int32, "rows", 2)
+  _need(out, torch.int32, "out")
+  if rows.shape[1] != 1024:
+    raise ValueError(f"probe_gather: rows must be (nrows, 1024), got {tuple(rows.shape)}")
+  _check(_lib.tspmi_probe_gather(_ptr(rows), rows.shape[0], blocks, iters, _ptr(out), _stream(torch)),
+         "tspmi_probe_gather")
 
 
 METRICS = {"EUC_2D": 0, "CEIL_2D": 1, "MAN_2D": 2, "MAX_2D": 3, "ATT": 4, "GEO": 5}
