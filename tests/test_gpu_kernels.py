@@ -418,6 +418,18 @@ def test_nn_init_matches_oracle(torch_mod, nat, berlin52_coords, k):
   np.testing.assert_array_equal(out.cpu().numpy(), oinit.nn_init(np.arange(n), P, cand, seeds))
 
 
+def test_nn_init_hand_worked_rows(torch_mod, nat):
+  """The two hand-worked GRASP rows of the oracle tests (seed 7, k = 2 and seed 2024, k = 1)."""
+  say("nn_init hand-worked rows")
+  for seed, cand, want in ((7, [[1, 2], [0, 2], [1, 3], [2, 4], [3, 5], [4, 0]], [2, 1, 0, 4, 3, 5]),
+                           (2024, [[1], [0], [0], [0], [0], [0]], [3, 0, 1, 2, 4, 5])):
+    out = torch_mod.empty((1, 6), dtype=torch_mod.int32, device="cuda")
+    work = torch_mod.empty(12, dtype=torch_mod.int32, device="cuda")
+    nat.nn_init(cu(np.array([seed], dtype=np.int64)), cu(np.arange(6, dtype=np.int32)),
+                cu(np.array(cand, dtype=np.int32)), work, out)
+    assert out.cpu().numpy()[0].tolist() == want
+
+
 @pytest.mark.parametrize("n,lattice", [(300, False), (400, True), (1, False), (2, False)])
 def test_nn_greedy_init_matches_oracle(torch_mod, nat, n, lattice):
   say(f"nn_greedy_init n={n} lattice={lattice}")

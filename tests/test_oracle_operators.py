@@ -114,6 +114,97 @@ def test_nn_init_rows_are_permutations_and_follow_neighbours(berlin52_coords):
   assert hits > 0.7 * 8 * 51
 
 
+# Hand-worked GRASP nearest-neighbour rows (nearest_neighbor_init.py:31-99).  Each SplitMix64 draw
+# is listed as u = (z >> 32) / 2^32, so bounded(b) = floor(u * b); the z values come from the
+# published algorithm (Steele, Lea & Flood 2014), restated independently below and checked against
+# its published first output for state 0.
+_M64 = (1 << 64) - 1
+
+
+def _sm64_draws(seed, count):
+  out, s = [], seed
+  for _ in range(count):
+    s = (s + 0x9E3779B97F4A7C15) & _M64
+    z = ((s ^ (s >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    out.append((z ^ (z >> 31)) & _M64)
+  return out
+
+
+def test_nn_row_hand_worked_shuffle_and_fallback():
+  """Seed 7, n = 6, k = 2.  Draws (u): 0.3898 0.0168 0.9008 0.5829 0.4524 0.2494 0.4680 0.3281
+  0.1343 0.4131 0.1036.
+    start   d0 bounded(6) = 2 -> city 2; unvisited [0 1 5 3 4 | 2]
+    step 1  row 2 = [1 3]: i=0 d1 bounded(2) = 0 -> order [0 1], city 1 unvisited -> 1
+            unvisited [0 4 5 3 | 1 2]
+    step 2  row 1 = [0 2]: i=0 d2 bounded(2) = 1 -> order [1 0], city 2 visited;
+            i=1 d3 bounded(1) = 0 -> city 0 unvisited -> 0; unvisited [3 4 5 | 0 1 2]
+    step 3  row 0 = [1 2]: i=0 d4 bounded(2) = 0 -> city 1 visited; i=1 d5 bounded(1) = 0 ->
+            city 2 visited; fallback d6 bounded(3) = 1 -> unvisited[1] = 4; unvisited [3 5 | 4 ...]
+    step 4  row 4 = [3 5]: i=0 d7 bounded(2) = 0 -> city 3 unvisited -> 3; unvisited [5 | 3 ...]
+    step 5  row 3 = [2 4]: d8, d9 both visited; fallback d10 bounded(1) = 0 -> 5
+  A bound of k instead of k - i at step 2, i = 1 would give j = 2 (outside the row)."""
+  draws = _sm64_draws(7, 11)
+  assert [round((z >> 32) / 2**32, 4) for z in draws] == [0.3898, 0.0168, 0.9008, 0.5829, 0.4524, 0.2494, 0.468,
+                                                          0.3281, 0.1343, 0.4131, 0.1036]
+  assert _sm64_draws(0, 1) == [0xE220A8397B1DCDAF]
+  cand = np.array([[1, 2], [0, 2], [1, 3], [2, 4], [3, 5], [4, 0]], dtype=np.int32)
+  assert oinit.nn_row(np.arange(6, dtype=np.int32), cand, 7).tolist() == [2, 1, 0, 4, 3, 5]
+
+
+def test_nn_row_hand_worked_swap_and_pop_fallback():
+  """Seed 2024, n = 6, k = 1 (every candidate draw is bounded(1) = 0 but still consumed).  Draws (u):
+  0.6228 0.0972 0.2986 0.1162 0.8306 0.5524 0.1368 0.5963 0.1689.
+    start   d0 bounded(6) = 3 -> city 3; unvisited [0 1 2 5 4 | 3]
+    step 1  row 3 = [0]: d1 -> city 0 -> 0; unvisited [4 1 2 5 | 0 3]
+    step 2  row 0 = [1]: d2 -> city 1 -> 1; unvisited [4 5 2 | 1 0 3]
+    step 3  row 1 = [0]: d3 -> city 0 visited; fallback d4 bounded(3) = 2 -> unvisited[2] = 2
+            (in sorted order the pick would be 5: the swap-and-pop order is what is pinned)
+    step 4  row 2 = [0]: d5 visited; fallback d6 bounded(2) = 0 -> unvisited[0] = 4
+    step 5  row 4 = [0]: d7 visited; fallback d8 bounded(1) = 0 -> 5"""
+  draws = _sm64_draws(2024, 9)
+  assert [round((z >> 32) / 2**32, 4) for z in draws] == [0.6228, 0.0972, 0.2986, 0.1162, 0.8306, 0.5524, 0.1368,
+                                                          0.5963, 0.1689]
+  cand = np.array([[1], [0], [0], [0], [0], [0]], dtype=np.int32)
+  assert oinit.nn_row(np.arange(6, dtype=np.int32), cand, 2024).tolist() == [3, 0, 1, 2, 4, 5]
+
+
+def test_nn_row_plausible_mistakes_fail_the_hand_worked_rows(monkeypatch):
+  """The two hand-worked rows reject a start draw taken after the first step's shuffle draw and a
+  fallback that picks from the sorted unvisited set."""
+  cand1 = np.array([[1, 2], [0, 2], [1, 3], [2, 4], [3, 5], [4, 0]], dtype=np.int32)
+  cand2 = np.array([[1], [0], [0], [0], [0], [0]], dtype=np.int32)
+  good = (oinit.nn_row(np.arange(6, dtype=np.int32), cand1, 7).tolist(),
+          oinit.nn_row(np.arange(6, dtype=np.int32), cand2, 2024).tolist())
+  assert good == ([2, 1, 0, 4, 3, 5], [3, 0, 1, 2, 4, 5])
+
+  def sorted_fallback_row(customers, candidates, seed):
+    # same rules, but the fallback draws an index into the sorted unvisited cities
+    n = customers.size
+    state, start = oinit.splitmix64_bounded_scalar(seed, n)
+    seen = {int(customers[start])}
+    out = [int(customers[start])]
+    for _ in range(1, n):
+      order = list(range(candidates.shape[1]))
+      chosen = -1
+      for i in range(len(order)):
+        state, off = oinit.splitmix64_bounded_scalar(state, len(order) - i)
+        order[i], order[i + off] = order[i + off], order[i]
+        nb = int(candidates[out[-1], order[i]])
+        if nb not in seen:
+          chosen = nb
+          break
+      if chosen < 0:
+        left = sorted(set(range(n)) - seen)
+        state, u = oinit.splitmix64_bounded_scalar(state, len(left))
+        chosen = left[u]
+      out.append(chosen)
+      seen.add(chosen)
+    return out
+
+  assert sorted_fallback_row(np.arange(6), cand2, 2024) != good[1]
+
+
 def test_greedy_nn_hand_worked():
   from oracle.init import greedy_nn
 
