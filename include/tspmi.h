@@ -75,6 +75,16 @@ int tspmi_dist_metric(const double* xy, int n, int metric, int32_t* D, int ld, i
 int tspmi_knn(const int32_t* D, int n, int ld, int k, int32_t* out, int32_t* ties_out, void* stream);
 
 /*
+ * tspmi_knn with a per-row flag instead of the count: tie_flags[u] (device int32, n entries) is
+ * 1 where row u's k-th and (k+1)-th distances tie, else 0.  Those are the rows on which the
+ * reference's np.argpartition picks among equal distances by its own rule; the host rebuilds
+ * exactly those rows with that rule (ProblemContext.get_gpu_candidate_list).
+ * Replaces: src/distances/matrix.py:140-211 (compute_candidate_list), with the tie rows of
+ *           :205-209 left to the caller.
+ */
+int tspmi_knn_flagged(const int32_t* D, int n, int ld, int k, int32_t* out, int32_t* tie_flags, void* stream);
+
+/*
  * Closed-tour cost: out[r] = (double) sum_x D[t_x, t_{x+1}] (+ D[t_{n-1}, t_0] if closed).
  * Replaces: src/algorithms/strategies/evaluation/tour_cost.py:17-65 and
  *           evaluation/kernels/tour_cost.cu:21 (tour_cost_kernel).

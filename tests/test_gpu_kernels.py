@@ -310,6 +310,32 @@ def test_tournament_pmx_inversion_match_oracle(torch_mod, nat, P, n):
   np.testing.assert_array_equal(kids.cpu().numpy(), ops.inversion_apply(want, probs, p1, p2, 0.3))
 
 
+def test_tournaments_follow_argmin_on_nan_fitness(torch_mod, nat):
+  """A NaN fitness wins its tournament as in np.argmin (the first NaN), in the tournament kernel
+  and in the fused generation's wave tournament."""
+  say("tournament NaN fitness")
+  P, n = 64, 30
+  pop = _population(n, P, seed=9)
+  d = compute_distance_matrix(uniform_coords(n, seed=4))
+  fit = ops.tour_costs(pop, d)
+  fit[[3, 17, 40]] = [np.nan, -np.nan, np.array([0x7FF8000000000005], dtype=np.uint64).view(np.float64)[0]]
+  cand = ops.tournament_draw(np.random.default_rng(2), P, P, 5)
+  cand[0, 2], cand[1, 4] = 3, 17
+  want = ops.tournament_winners(cand, fit)
+  sel = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  win = torch_mod.empty(P, dtype=torch_mod.int32, device="cuda")
+  nat.tournament(cu(pop), cu(fit), cu(cand), sel, win)
+  np.testing.assert_array_equal(win.cpu().numpy(), want)
+  assert want[0] == 3 and want[1] == 17
+  lo, hi = ops.pmx_draw(np.random.default_rng(3), P, n)
+  probs, p1, p2 = ops.inversion_draw(np.random.default_rng(4), P, n)
+  off = torch_mod.empty((P, n), dtype=torch_mod.int32, device="cuda")
+  off_fit = torch_mod.empty(P, dtype=torch_mod.float64, device="cuda")
+  nat.generation(cu(pop), cu(fit), cu(d), cu(cand), cu(lo), cu(hi), cu(probs), cu(p1), cu(p2), 0.2, off, off_fit)
+  kids = ops.inversion_apply(ops.pmx_apply(pop[want], lo, hi), probs, p1, p2, 0.2)
+  np.testing.assert_array_equal(off.cpu().numpy(), kids)
+
+
 @pytest.mark.parametrize("P,n", [(128, 52), (1024, 1000), (9, 40), (1, 10)])
 def test_fused_generation_matches_oracle(torch_mod, nat, P, n):
   say(f"fused generation P={P} n={n}")
@@ -348,11 +374,13 @@ def test_rank_select_is_stable_survival(torch_mod, nat):
 
 @pytest.mark.parametrize("M,keep", [(1, 1), (7, 7), (64, 10), (65, 65), (2048, 1024), (5000, 4999), (20000, 300)])
 def test_rank_select_matches_stable_argsort(torch_mod, nat, M, keep):
-  """Ties, signed zeros, infinities and NaN: the order is numpy's stable argsort (NaN last)."""
+  """Ties, signed zeros, infinities and NaNs of either sign and several payloads: the order is
+  numpy's stable argsort (every NaN last, in index order)."""
   say(f"rank_select M={M} keep={keep}")
   rng = np.random.default_rng(M)
   keys = rng.integers(-20, 20, M).astype(np.float64)
-  special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e300, -1e-300])
+  payload_nan = np.array([0x7FF8000000000001, 0xFFF8000000000002, 0x7FF0000000000003], dtype=np.uint64).view(np.float64)
+  special = np.concatenate([[0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1e300, -1e-300], payload_nan])
   hit = rng.random(M) < 0.2
   keys[hit] = special[rng.integers(0, len(special), int(hit.sum()))]
   order = torch_mod.full((keep,), -1, dtype=torch_mod.int32, device="cuda")

@@ -14,6 +14,7 @@ from oracle import ga as oga
 from oracle import init as oinit
 from oracle import operators as ops
 from oracle import two_opt as o2
+from oracle.distances import candidate_boundary_ties, compute_candidate_list
 
 pytestmark = pytest.mark.gpu
 
@@ -139,6 +140,20 @@ def _c1_ga(t, ctx, generations=50, patience=None):
   return ga, ga.evolve(ctx, list(range(ctx.dimension)), generations, patience=patience)
 
 
+@pytest.mark.parametrize("n,k", [(49, 6), (400, 20), (1000, 20), (52, 5)])
+def test_context_candidate_list_is_the_reference_table_on_tie_rows(t, n, k):
+  """Lattice cities (k-th boundary ties on most rows): the context's device table, with its
+  flagged rows rebuilt by the reference's argpartition rule, equals the oracle's table."""
+  say(f"context k-NN on lattice n={n} k={k}")
+  side = int(np.ceil(np.sqrt(n)))
+  xy = np.array([[x, y] for x in range(side) for y in range(side)][:n], dtype=np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("lattice", xy), seed=1)
+  d = ctx.get_cpu_distances()
+  assert ctx.candidate_boundary_ties(k) == candidate_boundary_ties(d, k) > 0
+  np.testing.assert_array_equal(ctx.get_cpu_candidate_list(k), compute_candidate_list(d, k))
+  np.testing.assert_array_equal(ctx.get_gpu_candidate_list(k).cpu().numpy(), compute_candidate_list(d, k))
+
+
 def test_config1_ga_end_to_end_matches_oracle(t):
   """Config 1: 52 uniform integer cities in [0,1000]^2 (seed 42), README memetic pipeline, 50 generations."""
   say("GA config 1 end to end")
@@ -146,11 +161,16 @@ def test_config1_ga_end_to_end_matches_oracle(t):
   ctx = t.ProblemContext(prob, seed=42)
   ga, (best, stats) = _c1_ga(t, ctx)
   d = ctx.get_cpu_distances()
+  # the oracle builds its own k-NN tables with the reference's argpartition rule; config 1 has
+  # k-th boundary ties at k = 20, which the context rebuilds by the same rule
+  c5, c20 = compute_candidate_list(d, 5), compute_candidate_list(d, 20)
+  assert candidate_boundary_ties(d, 20) > 0
+  np.testing.assert_array_equal(ctx.get_cpu_candidate_list(20), c20)
+  np.testing.assert_array_equal(ctx.get_cpu_candidate_list(5), c5)
   rng = np.random.default_rng(42)
-  pop0 = oinit.nn_init(np.arange(52), 128, ctx.get_cpu_candidate_list(5), oinit.draw_row_seeds(rng, 128))
+  pop0 = oinit.nn_init(np.arange(52), 128, c5, oinit.draw_row_seeds(rng, 128))
   res = oga.evolve(rng, d, pop0, max_generations=50, mutation_rate=0.05, tournament_k=5,
-                   local_search=oga.LocalSearchSpec("best", 10, ctx.get_cpu_candidate_list(20)), post_init=True,
-                   elite_count=16)
+                   local_search=oga.LocalSearchSpec("best", 10, c20), post_init=True, elite_count=16)
   assert stats["convergence_history"] == res.stats["convergence_history"]
   assert stats["generations_completed"] == res.stats["generations_completed"]
   assert stats["stop_reason"] == res.stats["stop_reason"]

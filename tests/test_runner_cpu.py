@@ -181,3 +181,56 @@ def test_native_generation_draws_reject_bad_shapes():
   assert rng.bit_generator.state == before
   new, cand, *_ = _native.draw_generations_pcg64(before, 0, 8, 10, 5)
   assert new == before and cand.shape == (0, 8, 5)
+
+
+def test_reference_knn_rows_equals_the_reference_rule_on_tie_rows():
+  """The host rebuild of flagged k-NN rows (context.reference_knn_rows) equals the oracle's
+  full-table restatement of matrix.py:140-211 row for row, on tie-heavy lattices."""
+  from oracle.distances import candidate_boundary_ties, compute_candidate_list
+  from tests.conftest import lattice_matrix
+  from tsp_gpu_clean_amd.context import reference_knn_rows
+
+  for n, k in ((49, 6), (400, 20), (1000, 20), (52, 5)):
+    d = lattice_matrix(n)
+    assert candidate_boundary_ties(d, k) > 0
+    rows = np.sort(np.random.default_rng(n).choice(n, size=min(n, 37), replace=False))
+    np.testing.assert_array_equal(reference_knn_rows(d[rows], rows, min(k, n - 1)), compute_candidate_list(d, k)[rows])
+
+
+def test_evaluation_gpu_cost_takes_the_reference_keywords():
+  """cost_evaluation.py:168-176: a config entry with the VRP keywords at their TSP defaults builds;
+  a VRP decoder or penalty policy raises NotImplementedError; a bad split mode is a ValueError."""
+  import tsp_gpu_clean_amd as t
+  from tsp_gpu_clean_amd.runner import _strategy
+
+  ev = _strategy({"class": "EvaluationGPUCost",
+                  "params": {"return_gpu": True, "threads_per_block": 200, "decoder": None, "split_mode": "greedy",
+                             "penalty_policy": None}}, 3)
+  assert isinstance(ev, t.EvaluationGPUCost) and ev.return_gpu and ev.split_mode == "greedy"
+  assert ev.threads_per_block == 128
+  assert t.EvaluationGPUCost(False, 256, None, None, "prins", None).threads_per_block == 256
+  with pytest.raises(NotImplementedError):
+    t.EvaluationGPUCost(decoder=object())
+  with pytest.raises(NotImplementedError):
+    t.EvaluationGPUCost(penalty_policy=object())
+  with pytest.raises(ValueError):
+    t.EvaluationGPUCost(split_mode="bogus")
+  with pytest.raises(ValueError):
+    t.EvaluationCPUCost(split_mode="bogus")
+
+
+def test_explicit_distances_are_checked_before_the_int32_cast():
+  """Values that an int32 cast would wrap or truncate are rejected before the cast."""
+  from tsp_gpu_clean_amd.context import MAX_DISTANCE, explicit_int32_distances
+
+  ok = np.array([[0, 5], [5, 0]], dtype=np.int64)
+  assert explicit_int32_distances(ok).dtype == np.int32
+  assert explicit_int32_distances(ok.astype(np.float64)).tolist() == [[0, 5], [5, 0]]
+  for bad in (np.array([[0, 2**31], [2**31, 0]], dtype=np.int64),       # would wrap negative
+              np.array([[0, 2**32 + 3], [2**32 + 3, 0]], dtype=np.int64),  # would wrap to 3
+              np.array([[0, MAX_DISTANCE + 1], [MAX_DISTANCE + 1, 0]]),
+              np.array([[0.0, 1.5], [1.5, 0.0]]),                          # would truncate to 1
+              np.array([[0.0, np.nan], [np.nan, 0.0]]),
+              np.array([[0, -1], [-1, 0]])):
+    with pytest.raises(ValueError):
+      explicit_int32_distances(bad)

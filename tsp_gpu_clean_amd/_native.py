@@ -34,6 +34,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_probe_gather": [_P, _I, _I, _I, _P, _P],
   "tspmi_dist_euc2d": [_P, _I, _P, _I, _P, _P],
   "tspmi_knn": [_P, _I, _I, _I, _P, _P, _P],
+  "tspmi_knn_flagged": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
   "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
@@ -215,6 +216,19 @@ def knn(D, k: int, out, ties_out=None) -> None:
   if ties_out is not None:
     _need(ties_out, torch.int32, "ties_out")
   _check(_lib.tspmi_knn(_ptr(D), n, D.shape[1], k, _ptr(out), _ptr(ties_out), _stream(torch)), "tspmi_knn")
+
+
+def knn_flagged(D, k: int, out, tie_flags) -> None:
+  """k-NN table in (distance, index) order plus a per-row flag of k-th/(k+1)-th distance ties."""
+  torch = require_device()
+  _need(D, torch.int32, "D", 2)
+  _need(out, torch.int32, "out", 2)
+  _need(tie_flags, torch.int32, "tie_flags", 1)
+  n = D.shape[0]
+  if out.shape != (n, k) or tie_flags.shape[0] != n:
+    raise ValueError(f"knn_flagged: out {tuple(out.shape)} / flags {tuple(tie_flags.shape)} for n={n}, k={k}")
+  _check(_lib.tspmi_knn_flagged(_ptr(D), n, D.shape[1], k, _ptr(out), _ptr(tie_flags), _stream(torch)),
+         "tspmi_knn_flagged")
 
 
 def tour_cost(tours, D, out, closed: bool = True) -> None:

@@ -46,6 +46,43 @@ def _cu_count() -> int:
     return 1 << 30
 
 
+def explicit_int32_distances(dist) -> np.ndarray:
+  """An explicit matrix as int32, checked before the cast: integral values in [0, MAX_DISTANCE],
+  so that neither a wrap of a large integer nor the truncation of a fraction can slip under the
+  guard that keeps every int32 2-opt delta exact."""
+  a = np.asarray(dist)
+  if a.size:
+    if a.dtype.kind == "b" or a.dtype.kind not in "iuf":
+      raise ValueError(f"explicit distances must be numeric, got dtype {a.dtype}")
+    if a.dtype.kind == "f" and not np.all(np.isfinite(a)):
+      raise ValueError("explicit distances must be finite")
+    lo, hi = a.min(), a.max()
+    if lo < 0:
+      raise ValueError("explicit distances must be non-negative")
+    if hi > MAX_DISTANCE:
+      raise ValueError(f"explicit distance {hi} exceeds {MAX_DISTANCE}: int32 2-opt deltas would not be exact")
+    if a.dtype.kind == "f" and not np.array_equal(a, np.floor(a)):
+      raise ValueError("explicit distances must be integers")
+  return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def reference_knn_rows(rows: np.ndarray, row_ids: np.ndarray, k: int) -> np.ndarray:
+  """The reference's k-NN rule on distance rows `rows` ((m, n), row r is city row_ids[r]): the
+  city itself masked with the int64 maximum, ``np.argpartition(block, k, axis=1)[:, :k]``, then
+  ``np.lexsort((cols, dist))`` (src/distances/matrix.py:189, 204-209).  On a row whose k-th and
+  (k+1)-th distances tie, argpartition decides which tied city enters the list; the device table
+  keeps (distance, index) order, which equals this rule on every other row, so only the flagged
+  rows are rebuilt here, with the same NumPy call the reference makes on the same host."""
+  block = np.asarray(rows, dtype=np.int64).copy()
+  m = block.shape[0]
+  local = np.arange(m)
+  block[local, np.asarray(row_ids, dtype=np.int64)] = np.iinfo(np.int64).max
+  cols = np.argpartition(block, k, axis=1)[:, :k]
+  cdist = block[local[:, None], cols]
+  order = np.lexsort((cols, cdist), axis=1)
+  return np.ascontiguousarray(cols[local[:, None], order], dtype=np.int32)
+
+
 class ProblemContext:
   def __init__(
     self,
@@ -63,6 +100,7 @@ class ProblemContext:
     self.rng: np.random.Generator = make_rng(seed)
     self.seed = seed
     self.h2d_bytes = 0
+    self.d2h_bytes = 0
     self._gpu_distances = None
     self._cpu_distances: np.ndarray | None = None
     self._gpu_coordinates = None
@@ -108,13 +146,10 @@ class ProblemContext:
       n = self.dimension
       self._check_vram(4 * n * n, f"dense distance matrix (n={n})")
       if self.problem.distances is not None:
-        host = np.ascontiguousarray(self.problem.distances, dtype=np.int32)
+        host = explicit_int32_distances(self.problem.distances)
         d = torch.from_numpy(host).to("cuda")
         self.h2d_bytes += int(host.nbytes)
         mx = int(host.max()) if host.size else 0
-        mn = int(host.min()) if host.size else 0
-        if mn < 0:
-          raise ValueError("explicit distances must be non-negative")
       else:
         xy = self.get_gpu_coordinates()
         d = torch.empty((n, n), dtype=torch.int32, device="cuda")
@@ -150,9 +185,17 @@ class ProblemContext:
       else:
         d = self.get_gpu_distances()
         table = torch.empty((n, k_eff), dtype=torch.int32, device="cuda")
-        ties = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ties = 0
         if k_eff > 0:
-          _native.knn(d, k_eff, table, ties)
+          flags = torch.empty(n, dtype=torch.int32, device="cuda")
+          _native.knn_flagged(d, k_eff, table, flags)
+          rows = torch.nonzero(flags).flatten()  # host sync: once per (context, k)
+          ties = int(rows.numel())
+          if ties:
+            fixed = reference_knn_rows(d.index_select(0, rows)[:, :n].cpu().numpy(), rows.cpu().numpy(), k_eff)
+            self.d2h_bytes += ties * n * 4
+            table.index_copy_(0, rows, torch.from_numpy(fixed).to("cuda"))
+            self.h2d_bytes += int(fixed.nbytes)
         self._candidate_ties[k] = ties
       self._gpu_candidates[k] = table
     return table
@@ -200,7 +243,7 @@ class ProblemContext:
     return cached
 
   def candidate_boundary_ties(self, k: int = 20) -> int:
-    """Rows whose k-th and (k+1)-th distances tie (where the reference's argpartition is arbitrary)."""
+    """Rows whose k-th and (k+1)-th distances tie: the rows rebuilt on the host by the
+    reference's argpartition rule (reference_knn_rows)."""
     self.get_gpu_candidate_list(k)
-    t = self._candidate_ties.get(k)
-    return 0 if t is None else int(t.item())
+    return int(self._candidate_ties.get(k, 0))

@@ -114,12 +114,19 @@ __device__ __forceinline__ uint32_t key_index(uint64_t key) { return (uint32_t)k
 
 constexpr uint64_t kNoMove = ~0ull;
 
-// Unsigned bits of a double that order as the double does (-0.0 and +0.0 equal, NaN after +inf).
+// Unsigned bits of a double that order as NumPy's sorts order doubles: -0.0 and +0.0 equal, and
+// every NaN (either sign, any payload) after +inf and equal to every other NaN, so that equal
+// keys broken by index give np.argsort(kind="stable") (NaNs last, in index order).
 __device__ __forceinline__ uint64_t orderable_bits(double v) {
+  if (v != v) return ~0ull;
   if (v == 0.0) v = 0.0;
   const uint64_t b = (uint64_t)__double_as_longlong(v);
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
+
+// Key of np.argmin over doubles: a NaN propagates, so the first NaN is the minimum; other values
+// order as orderable_bits (which never returns 0 for a number: -inf maps to 2^52 - 1).
+__device__ __forceinline__ uint64_t argmin_bits(double v) { return v != v ? 0ull : orderable_bits(v); }
 
 // Node ids outside [0, limit) are an invalid input; clamping keeps every access in bounds.
 __device__ __forceinline__ int clamp_id(int v, int limit) { return (unsigned)v < (unsigned)limit ? v : 0; }

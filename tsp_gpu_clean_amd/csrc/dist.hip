@@ -101,11 +101,14 @@ __global__ __launch_bounds__(kDistTile) void dist_metric_kernel(const double* __
 
 // One workgroup per row u: k rounds of a block argmin over 64-bit keys (D[u,v] << 32 | v),
 // each round taking the smallest key above the previous one.  Keys are unique, so the result
-// is the (distance, node index) order of src/distances/matrix.py:205-209.
+// is the (distance, node index) order of src/distances/matrix.py:205-209.  A (k+1)-th round
+// detects a row whose k-th and (k+1)-th distances tie: it is counted in ties_out and flagged
+// in tie_flags[u] (both nullable).
 constexpr int kKnnBlock = 256;
 
 __global__ __launch_bounds__(kKnnBlock) void knn_kernel(const int32_t* __restrict__ D, int n, int ld, int k,
-                                                        int32_t* __restrict__ out, int32_t* __restrict__ ties_out) {
+                                                        int32_t* __restrict__ out, int32_t* __restrict__ ties_out,
+                                                        int32_t* __restrict__ tie_flags) {
   __shared__ uint64_t red[kKnnBlock / kWave];
   const int u = blockIdx.x;
   const int32_t* row = D + (size_t)u * ld;
@@ -123,8 +126,10 @@ __global__ __launch_bounds__(kKnnBlock) void knn_kernel(const int32_t* __restric
     if (r < k) {
       if (threadIdx.x == 0) out[(size_t)u * k + r] = (int32_t)(uint32_t)best;
       kth_dist = (uint32_t)(best >> 32);
-    } else if (ties_out != nullptr && best != kNoMove && threadIdx.x == 0 && (uint32_t)(best >> 32) == kth_dist) {
-      atomicAdd(ties_out, 1);
+    } else if (threadIdx.x == 0) {
+      const bool tie = best != kNoMove && (uint32_t)(best >> 32) == kth_dist;
+      if (tie && ties_out != nullptr) atomicAdd(ties_out, 1);
+      if (tie_flags != nullptr) tie_flags[u] = tie ? 1 : 0;
     }
     prev = best;
     have_prev = true;
@@ -168,8 +173,23 @@ extern "C" int tspmi_knn(const int32_t* D, int n, int ld, int k, int32_t* out, i
   TSPMI_CHECK_ARG(k_eff == k, "tspmi_knn: k=%d must already be clamped to min(k, n-1)=%d", k, k_eff);
   if (n == 0 || k == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(D != nullptr && out != nullptr, "tspmi_knn: null pointer");
-  hipLaunchKernelGGL(knn_kernel, dim3(n), dim3(kKnnBlock), 0, (hipStream_t)stream, D, n, ld, k, out, ties_out);
+  hipLaunchKernelGGL(knn_kernel, dim3(n), dim3(kKnnBlock), 0, (hipStream_t)stream, D, n, ld, k, out, ties_out,
+                     nullptr);
   TSPMI_CHECK_LAUNCH("tspmi_knn launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_knn_flagged(const int32_t* D, int n, int ld, int k, int32_t* out, int32_t* tie_flags,
+                                 void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(n >= 0 && ld >= n && k >= 0, "tspmi_knn_flagged: bad shape n=%d ld=%d k=%d", n, ld, k);
+  const int k_eff = n > 1 ? (k < n - 1 ? k : n - 1) : 0;
+  TSPMI_CHECK_ARG(k_eff == k, "tspmi_knn_flagged: k=%d must already be clamped to min(k, n-1)=%d", k, k_eff);
+  if (n == 0 || k == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(D != nullptr && out != nullptr && tie_flags != nullptr, "tspmi_knn_flagged: null pointer");
+  hipLaunchKernelGGL(knn_kernel, dim3(n), dim3(kKnnBlock), 0, (hipStream_t)stream, D, n, ld, k, out, nullptr,
+                     tie_flags);
+  TSPMI_CHECK_LAUNCH("tspmi_knn_flagged launch");
   return TSPMI_OK;
 }
 

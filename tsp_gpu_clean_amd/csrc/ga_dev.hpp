@@ -10,14 +10,15 @@ constexpr int kRowBlock = 256;
 constexpr int kSortBlock = 1024;
 constexpr int kSortMax = 4096;
 
-// First minimum of fitness over one tournament row (tournament_selection.py:125-127).
+// First minimum of fitness over one tournament row, np.argmin's rule (tournament_selection.py:125-127):
+// the first NaN if any, else the first least value.
 __device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ cand_row, int k,
                                                  const double* __restrict__ fitness, int P) {
   int w = clamp_id(cand_row[0], P);
-  double best = fitness[w];
+  uint64_t best = argmin_bits(fitness[w]);
   for (int c = 1; c < k; ++c) {
     const int x = clamp_id(cand_row[c], P);
-    const double f = fitness[x];
+    const uint64_t f = argmin_bits(fitness[x]);
     if (f < best) {
       best = f;
       w = x;
@@ -29,7 +30,7 @@ __device__ __forceinline__ int tournament_winner(const int32_t* __restrict__ can
 // tournament_winner by one wave: lane c loads candidate c and its fitness, so the k candidate
 // loads and the k fitness loads are each issued together instead of as k dependent pairs; the
 // winner is the lowest-indexed candidate of least fitness (the sequential scan's first strict
-// minimum, for finite fitness).  Every lane returns it.
+// minimum; a NaN fitness wins as in np.argmin).  Every lane returns it.
 __device__ __forceinline__ int wave_tournament(const int32_t* __restrict__ cand_row, int k,
                                                const double* __restrict__ fitness, int P) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -38,7 +39,7 @@ __device__ __forceinline__ int wave_tournament(const int32_t* __restrict__ cand_
   for (int c0 = 0; c0 < k; c0 += kWave) {
     const int c = c0 + lane;
     const int x = c < k ? clamp_id(cand_row[c], P) : 0;
-    const uint64_t f = c < k ? orderable_bits(fitness[x]) : ~0ull;
+    const uint64_t f = c < k ? argmin_bits(fitness[x]) : ~0ull;
     const uint64_t m = wave_min_u64(f);
     const uint64_t hit = __ballot(c < k && f == m);
     const int first = __builtin_ctzll(hit);

@@ -626,7 +626,7 @@ __device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Di
       for (int t0 = 0; t0 < P; t0 += kWave) {
         const int t = t0 + tid;
         const uint64_t key =
-            t < P ? orderable_bits(__longlong_as_double((long long)load_relaxed(reinterpret_cast<uint64_t*>(cost_all + t))))
+            t < P ? argmin_bits(__longlong_as_double((long long)load_relaxed(reinterpret_cast<uint64_t*>(cost_all + t))))
                   : ~0ull;
         const uint64_t m = wave_min_u64(key);
         const int first = __builtin_ctzll(__ballot(t < P && key == m));

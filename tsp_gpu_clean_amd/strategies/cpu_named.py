@@ -68,9 +68,7 @@ class EvaluationCPUCost(EvaluationGPUCost):
                max_matrix_dimension: int | None = None) -> None:
     if decoder is not None or penalty_policy is not None:
       raise NotImplementedError("EvaluationCPUCost: VRP decoders and penalty policies are not implemented")
-    if split_mode not in ("prins", "greedy"):
-      raise ValueError(f"split_mode must be 'prins' or 'greedy', got {split_mode!r}")
-    super().__init__(return_gpu=False, max_matrix_dimension=max_matrix_dimension)
+    super().__init__(return_gpu=False, max_matrix_dimension=max_matrix_dimension, split_mode=split_mode)
 
 
 class LocalSearchCPUTourTwoOpt(LocalSearchGPUTourTwoOpt):

@@ -14,13 +14,31 @@ from ..timing import time_strategy
 from .base import StrategyBase
 
 
+def _prev_power_of_2(v: int, lo: int = 32, hi: int = 256) -> int:
+  """Largest power of two <= v, clamped to [lo, hi] (the reference's prev_power_of_2)."""
+  v = max(lo, min(hi, int(v)))
+  return 1 << (v.bit_length() - 1)
+
+
 class EvaluationGPUCost(StrategyBase):
+  """Constructor of cost_evaluation.py:168-188: `decoder`, `split_mode` and `penalty_policy` are
+  the VRP components; for TSP tours they stay at their defaults (no decoder, no penalty), and a
+  VRP decoder or penalty policy raises NotImplementedError, as EvaluationCPUCost does."""
+
   def __init__(self, return_gpu: bool = False, threads_per_block: int = 256, max_matrix_dimension: int | None = None,
+               decoder: Any = None, split_mode: str = "prins", penalty_policy: Any = None, *,
                seed: Any = None) -> None:
+    if decoder is not None or penalty_policy is not None:
+      raise NotImplementedError("EvaluationGPUCost: VRP decoders and penalty policies are not implemented")
+    if split_mode not in ("prins", "greedy"):
+      raise ValueError(f"split_mode must be 'prins' or 'greedy', got {split_mode!r}")
     super().__init__(seed)
     self.return_gpu = return_gpu
-    self.threads_per_block = threads_per_block
+    self.threads_per_block = _prev_power_of_2(threads_per_block)
     self.max_matrix_dimension = max_matrix_dimension
+    self.decoder = None
+    self.split_mode = split_mode
+    self.penalty_policy = None
 
   def _warmup_impl(self, context) -> None:
     self.evaluate_batch(self._warmup_batch(context, rows=1), context, return_gpu=True)
