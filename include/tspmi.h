@@ -158,6 +158,24 @@ int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, 
                         int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync, void* stream);
 
 /*
+ * tspmi_two_opt_elite without a caller workspace, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the cluster barrier
+ * words live in a static device array that the library holds on every GPU it runs on until the
+ * process ends (TSPMI_ELITE_STATIC_TOURS x 528 int64 = 540,672 bytes per GPU), so P may not
+ * exceed TSPMI_ELITE_STATIC_TOURS (TSPMI_EINVAL otherwise).  The call allocates nothing.  Each
+ * call first clears the P tours' words with a kernel, so nothing of an earlier call carries over;
+ * failed (device int32[1], may be NULL) receives the number of tours whose cluster barrier timed
+ * out (those tours and their costs are left unchanged, as in tspmi_two_opt_elite).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race.  Same tours, moves,
+ * counters, costs and best index as tspmi_two_opt_elite.
+ * No reference counterpart: the reference allocates its workspaces per call.
+ */
+#define TSPMI_ELITE_STATIC_TOURS 128
+int tspmi_two_opt_elite_noalloc(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                                int max_iter, int64_t* stats, double* cost, int32_t* best, int32_t* failed,
+                                void* stream);
+
+/*
  * EUC_2D variants that compute every distance from the coordinates xy (n, 2) float64 (16-byte
  * aligned) with the FP64 nint of tspmi_dist_euc2d instead of reading a matrix: same tours, moves
  * and counters as tspmi_two_opt(mode = BEST, full neighbourhood) and tspmi_two_opt_packed on the

@@ -41,6 +41,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
   "tspmi_two_opt_elite": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P],
+  "tspmi_two_opt_elite_noalloc": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P],
   "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
@@ -332,6 +333,29 @@ def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync, bes
   _check(_lib.tspmi_two_opt_elite(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
                                   _ptr(stats), _ptr(cost), _ptr(best), _ptr(sync), _stream(torch)),
          "tspmi_two_opt_elite")
+
+
+ELITE_STATIC_TOURS = 128  # TSPMI_ELITE_STATIC_TOURS: capacity of the library's static elite workspace
+
+
+def two_opt_elite_noalloc(tours, D, table, k: int, max_iter: int, stats, cost, best=None, failed=None) -> None:
+  """:func:`two_opt_elite` on the library's static workspace (no allocation, P <= 128); ``failed``
+  (int32 (1,), optional) receives the number of tours whose cluster barrier timed out."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(table, torch.int32, "table", 3)
+  P, n = tours.shape
+  if table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+    raise ValueError("two_opt_elite_noalloc: table does not match n / k")
+  for t, name in ((stats, "stats"), (cost, "cost"), (best, "best"), (failed, "failed")):
+    if t is not None:
+      _need(t, torch.float64 if name == "cost" else (torch.int64 if name == "stats" else torch.int32), name)
+  if cost is not None and cost.shape[0] < P:
+    raise ValueError("two_opt_elite_noalloc: cost shorter than the number of tours")
+  _check(_lib.tspmi_two_opt_elite_noalloc(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
+                                          _ptr(stats), _ptr(cost), _ptr(best), _ptr(failed), _stream(torch)),
+         "tspmi_two_opt_elite_noalloc")
 
 
 def elite_failed(sync) -> bool:

@@ -16,6 +16,8 @@
 //   move               : reverse positions i+1 .. j                        (two_opt.py:438)
 // Exact tie-breaking comes from packing (delta, scan index) into one 64-bit key whose unsigned
 // minimum is the reference's choice.
+#include <mutex>
+
 #include "two_opt_dev.hpp"
 
 namespace tspmi {
@@ -920,10 +922,30 @@ extern "C" int tspmi_two_opt_packed(int32_t* tours, int P, int n, const int32_t*
   return tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, nullptr, stream);
 }
 
-extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
-                                   int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync,
-                                   void* stream) {
-  using namespace tspmi;
+namespace tspmi {
+
+// Zeroes `words` 64-bit words (a kernel rather than hipMemsetAsync, so a captured graph replays it
+// correctly).
+__global__ void zero_words_kernel(uint64_t* __restrict__ w, int words) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) w[i] = 0ull;
+}
+
+// failed[0] = number of tours whose cluster barrier timed out (error word of each tour's workspace).
+__global__ void count_elite_errors_kernel(const uint64_t* __restrict__ sync, int P, int32_t* __restrict__ failed) {
+  int c = 0;
+  for (int t = threadIdx.x; t < P; t += blockDim.x)
+    c += sync[(size_t)t * kClusterWords + 2 * kClusterMaxG * kClusterLine] != 0ull ? 1 : 0;
+  for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, kWave);
+  if (threadIdx.x == 0) failed[0] = c;
+}
+
+// The elite search of tspmi_two_opt_elite on workspace `sync`.  With `fresh`, the workspace of the
+// P tours is zeroed first (so nothing of an earlier call, finished or timed out, carries over),
+// and `failed` (nullable) receives the count of tours whose barrier timed out.
+int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k, int max_iter,
+              int64_t* stats, double* cost, int32_t* best, uint64_t* sync, bool fresh, int32_t* failed,
+              hipStream_t s) {
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_elite: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_elite: k=%d outside [1, 32]", k);
   const int cus = device_cu_count();
@@ -938,18 +960,26 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
                        pairs < (1L << (32 - kRegJBits)) && lds + 1024 <= (size_t)kMaxLds;
   TSPMI_CHECK_ARG(best == nullptr || cost != nullptr, "tspmi_two_opt_elite: best needs cost");
   if (!cluster) {
-    const int rc = tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, stream);
-    if (rc != TSPMI_OK || best == nullptr || P == 0) return rc;
-    return tspmi_rank_select(cost, P, 1, best, stream);
+    int rc = tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, s);
+    if (rc == TSPMI_OK && best != nullptr && P > 0) rc = tspmi_rank_select(cost, P, 1, best, s);
+    if (rc == TSPMI_OK && fresh && failed != nullptr) {  // no barrier ran: writes failed[0] = 0
+      hipLaunchKernelGGL(count_elite_errors_kernel, dim3(1), dim3(kWave), 0, s, sync, 0, failed);
+      TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite error count launch");
+    }
+    return rc;
   }
   TSPMI_CHECK_ARG(tours && D && table && sync, "tspmi_two_opt_elite: null pointer");
   TSPMI_CHECK_ARG(ld >= n && (long)n * ld < kMaxMatrixElems, "tspmi_two_opt_elite: bad ld=%d for n=%d", ld, n);
   TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0,
                   "tspmi_two_opt_elite: table and sync must be 8-byte aligned");
-  hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table);
   const int kp = cand_lanes(k);
   const int xcds = (P % kXcds == 0 && cus % kXcds == 0) ? kXcds : 1;
+  if (fresh) {
+    const int words = P * kClusterWords;
+    hipLaunchKernelGGL(zero_words_kernel, dim3((words + 255) / 256), dim3(256), 0, s, sync, words);
+    TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite workspace clear launch");
+  }
 #define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                 \
   hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, ct, kp, \
                      k, max_iter, G, sync, stats, cost, P, best, xcds)
@@ -958,5 +988,55 @@ extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* 
   else TSPMI_LAUNCH_CLUSTER(8);
 #undef TSPMI_LAUNCH_CLUSTER
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite launch");
+  if (fresh && failed != nullptr) {
+    hipLaunchKernelGGL(count_elite_errors_kernel, dim3(1), dim3(kWave), 0, s, sync, P, failed);
+    TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite error count launch");
+  }
   return TSPMI_OK;
+}
+
+// The elite workspace of tspmi_two_opt_elite_noalloc: kEliteStaticTours tours of kClusterWords
+// words in device memory that the library holds for the process's lifetime (540,672 bytes per GPU).
+constexpr int kEliteStaticTours = TSPMI_ELITE_STATIC_TOURS;
+static __device__ __align__(256) uint64_t scratch[kEliteStaticTours * kClusterWords];
+
+// Device address of `scratch` on the current device, looked up once per device.
+int elite_scratch(uint64_t** out) {
+  static std::mutex mu;
+  static uint64_t* addr[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "tspmi_two_opt_elite_noalloc: hipGetDevice");
+  TSPMI_CHECK_ARG(dev >= 0 && dev < 64, "tspmi_two_opt_elite_noalloc: device %d outside [0, 64)", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "tspmi_two_opt_elite_noalloc: hipGetSymbolAddress(scratch)");
+    addr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = addr[dev];
+  return TSPMI_OK;
+}
+
+}  // namespace tspmi
+
+extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
+                                   int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync,
+                                   void* stream) {
+  return tspmi::elite_run(tours, P, n, D, ld, table, k, max_iter, stats, cost, best, sync, false, nullptr,
+                          (hipStream_t)stream);
+}
+
+extern "C" int tspmi_two_opt_elite_noalloc(int32_t* tours, int P, int n, const int32_t* D, int ld,
+                                           const int32_t* table, int k, int max_iter, int64_t* stats, double* cost,
+                                           int32_t* best, int32_t* failed, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && P <= kEliteStaticTours,
+                  "tspmi_two_opt_elite_noalloc: P=%d outside [0, %d], the static workspace's capacity", P,
+                  kEliteStaticTours);
+  uint64_t* ws = nullptr;
+  const int rc = elite_scratch(&ws);
+  if (rc != TSPMI_OK) return rc;
+  return elite_run(tours, P, n, D, ld, table, k, max_iter, stats, cost, best, ws, true, failed, (hipStream_t)stream);
 }
