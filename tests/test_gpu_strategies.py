@@ -448,13 +448,32 @@ def _same_state(a, b) -> bool:
 
 
 @pytest.mark.parametrize("case", ["patience_stop", "optimal_stop", "remainder_no_stop", "no_elite", "cpu_named",
-                                  "coordinates", "default_chunk", "philox_rng"])
-def test_evolve_device_loop_equals_step_pipeline(t, case):
+                                  "coordinates", "default_chunk", "philox_rng", "cluster_timeout"])
+def test_evolve_device_loop_equals_step_pipeline(t, case, monkeypatch):
   """evolve(device_loop=True) runs graph-captured chunks of generations and must end exactly where
   the step pipeline ends: history, stop, best tour, final population, fitness and generator.  The
   chunk is chosen so that a stop falls inside a chunk (the rewind path) and a run without a stop
-  ends in a partial chunk (the eager remainder)."""
+  ends in a partial chunk (the eager remainder).  In "cluster_timeout" the host raises the error
+  word of an elite cluster before some eager elite searches (the cluster then leaves at once, as
+  after a barrier timeout): the chunks concerned are run again and the result is unchanged."""
+  import torch
+
+  from tsp_gpu_clean_amd import _native
   from tsp_gpu_clean_amd.engine import device_route_plan
+
+  forced = {"calls": 0, "raised": 0}
+  if case == "cluster_timeout":
+    real = _native.two_opt_elite
+
+    def raising(tours, D, table, k, iters, stats, cost, sync, best=None):
+      # every 4th elite search, eager or captured (a captured one raises the word in every replay)
+      forced["calls"] += 1
+      if forced["calls"] % 4 == 3:
+        e = _native.ELITE_ERROR_WORD
+        sync.view(-1, _native.ELITE_SYNC_WORDS)[0, e : e + 1].fill_(1)  # a fill kernel: capturable
+        forced["raised"] += 1
+        forced["captured"] = forced.get("captured", 0) + int(torch.cuda.is_current_stream_capturing())
+      return real(tours, D, table, k, iters, stats, cost, sync, best)
 
   n, P, gens, elite, kw, opt, patience = 120, 64, 40, 8, {}, None, 3
   if case == "remainder_no_stop":
@@ -469,6 +488,8 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
     n, P, gens, patience = 150, 64, 90, 10**6
   elif case == "philox_rng":  # a non-PCG64 context generator takes the per-call NumPy draws
     gens = 30
+  elif case == "cluster_timeout":  # eager first chunk, captured replays, eager partial last chunk
+    gens, patience = 27, 10**6
   prob = clustered_problem(t, n, seed=7)
   if case == "optimal_stop":
     ga = _memetic_ga(t, P, elite, device_loop=False)
@@ -496,7 +517,12 @@ def test_evolve_device_loop_equals_step_pipeline(t, case):
   ga1 = _memetic_ga(t, P, elite, device_loop=True, chunk=chunk, **kw)
   c1 = context()
   assert device_route_plan(ga1, c1, np.zeros((P, n), np.int32)) is not None
+  if case == "cluster_timeout":
+    monkeypatch.setattr(_native, "two_opt_elite", raising)
   b1, s1 = ga1.evolve(c1, list(range(n)), gens, optimal_cost=opt, patience=patience)
+  if case == "cluster_timeout":
+    assert forced.get("captured", 0) >= 1 and forced["raised"] >= 3, forced
+    assert ga1.device_loop_recoveries >= 3, (forced, ga1.device_loop_recoveries)
   assert "device_loop" in s1["phase_timings"] and "device_loop" not in s0["phase_timings"]
   for key in ("convergence_history", "generations_completed", "stop_reason", "stopped_early", "best_fitness",
               "initial_fitness"):

@@ -156,6 +156,9 @@ class DeviceMemeticLoop:
     self.stats = torch.zeros(3, dtype=torch.int64, device=dev)
     # Barrier words of the elite search's workgroup clusters (left zeroed by every launch).
     self.sync = torch.zeros(_native.ELITE_SYNC_WORDS * max(1, self.elite or 1), dtype=torch.int64, device=dev)
+    # False: the elite search runs one workgroup per tour (same results, no cluster barrier); the
+    # recovery path after a cluster could not meet.
+    self.clusters = True
     self.cur = 0
 
   def load(self, population, fitness=None) -> None:
@@ -198,10 +201,12 @@ class DeviceMemeticLoop:
       fused = self.table is not None and self.n >= 4 and self.ls_iters != 0
       if fused and self.xy is not None:
         _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
-      elif fused:
+      elif fused and self.clusters:
         _native.two_opt_elite(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E], self.sync,
                               best_out)
         best_done = True
+      elif fused:
+        _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
       else:
         if self.table is not None:
           _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats)
@@ -214,11 +219,32 @@ class DeviceMemeticLoop:
       _native.gather_rows(self.pop[b], kb[:P], None, None, best_out, best_tour, best_cost)
     self.cur = b
 
+  def elite_failed(self) -> bool:
+    """Whether an elite search's workgroup cluster could not meet since the last clear (that tour
+    was left unrefined, so the generations from it on differ from the reference); reads the
+    device, so call it after the work has been issued and outside graph capture."""
+    return bool(self.elite) and _native.elite_failed(self.sync)
+
+  def clear_elite_failure(self) -> None:
+    self.sync.zero_()
+
   def verify(self) -> None:
-    """Raise if an elite search's workgroup cluster could not meet (its result was not written);
-    call after the work has been issued, outside graph capture (reads the device)."""
-    if _native.elite_failed(self.sync):
+    """Raise if an elite search's workgroup cluster could not meet (its result was not written)."""
+    if self.elite_failed():
       raise RuntimeError("elite 2-opt: a workgroup cluster was not co-resident; results are invalid")
+
+  def rerun_without_clusters(self, population, fitness, draws: GenerationDraws, generations: int, best_idx=None,
+                             best_tours=None, best_costs=None) -> None:
+    """Run ``generations`` steps again from ``population`` / ``fitness`` with the elite search on
+    one workgroup per tour (no cluster barrier), after clearing the failure; same results as a run
+    whose clusters all met."""
+    self.clear_elite_failure()
+    self.load(population, fitness)
+    self.clusters = False
+    try:
+      self.run(draws, generations, best_idx, 0, best_tours, best_costs)
+    finally:
+      self.clusters = True
 
   def run(self, draws: GenerationDraws, generations: int, best_idx=None, start: int = 0, best_tours=None,
           best_costs=None) -> None:
@@ -327,7 +353,9 @@ class DeviceEvolution:
   (genetic_algorithm.py:876-910) and names the generation to stop at, if any.  A stop inside a
   chunk rewinds: the population saved at the chunk start is run eagerly to the stopping
   generation and the generator is set to its state after that generation's draws, so population,
-  fitness and generator end where the step pipeline ends.
+  fitness and generator end where the step pipeline ends.  A chunk in which a cluster of the elite
+  search could not meet (its bounded barrier timed out, leaving a tour unrefined) is run again
+  from its start with the elite search on one workgroup per tour, which gives the same results.
   """
 
   def __init__(self, context, plan: dict, population, fitness, chunk: int = 32) -> None:
@@ -355,6 +383,7 @@ class DeviceEvolution:
     self.saved_fit = torch.empty(P, dtype=torch.float64, device=dev)
     self.graph = None
     self.k = k
+    self.recoveries = 0  # chunks re-run after an elite cluster could not meet
 
   def _draw(self, generations: int):
     states: list = []
@@ -387,12 +416,21 @@ class DeviceEvolution:
         self.graph = self.loop.capture(self.draws, self.chunk, self.best_idx, 0, self.best_tours, self.best_costs)
       pending = self._draw(min(self.chunk, left)) if left > 0 else None  # overlaps the device work
       costs = self.best_costs[:g].cpu().numpy()
-      self.loop.verify()
+      if self.loop.elite_failed():
+        # a cluster of the elite search could not meet: the chunk from that generation on is not
+        # the reference's; run it again from its start with one workgroup per elite tour
+        self.recoveries += 1
+        self.loop.rerun_without_clusters(self.saved_pop, self.saved_fit, self.draws, g, self.best_idx[:g],
+                                         self.best_tours[:g], self.best_costs[:g])
+        costs = self.best_costs[:g].cpu().numpy()
       stop = walk(done, costs)
       if stop is not None:
         if stop + 1 < g:
           self.loop.load(self.saved_pop, self.saved_fit)
           self.loop.run(self.draws, stop + 1)
+          if self.loop.elite_failed():
+            self.recoveries += 1
+            self.loop.rerun_without_clusters(self.saved_pop, self.saved_fit, self.draws, stop + 1)
         self.rng.bit_generator.state = states[stop]
         return done + stop + 1
       done += g

@@ -421,6 +421,7 @@ class GeneticAlgorithm:
       return stop
 
     self.generation = self.timer.measure("device_loop", runner.run, max_generations, walk)
+    self.device_loop_recoveries = runner.recoveries
     state.generation = self.generation
     pop = runner.population.clone()
     state.population = pop if is_gpu_array(pop_like) else pop.cpu().numpy()
