@@ -192,6 +192,25 @@ int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* 
                                     int max_iter, int64_t* stats, double* cost, void* stream);
 
 /*
+ * tspmi_two_opt_packed_euc2d_cost that also logs each tour's progress: pass_log (device int64,
+ * P x (log_moves + 1) x 2) receives per tour {cost, wall clock} at the start and after each of its
+ * first log_moves moves (entries past a tour's last logged move are left as they were).  The cost
+ * is exact (start cost plus the applied deltas); the clock is the device's constant-rate counter,
+ * in ticks of tspmi_wall_clock_khz() kHz, common to all CUs.  With it the population's best cost
+ * is known at the resolution of single passes from one launch, without host synchronisation.
+ * Same tours, counters and costs as tspmi_two_opt_packed_euc2d_cost.
+ * Replaces: two_opt.py:831-899 in candidate mode (the per-pass best-cost trace has no counterpart:
+ *           the reference's convergence history is kept per generation, genetic_algorithm.py:951-964).
+ */
+int tspmi_two_opt_packed_euc2d_log(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k,
+                                   int max_iter, int64_t* stats, double* cost, int64_t* pass_log, int log_moves,
+                                   void* stream);
+
+/* Rate of the device's constant wall clock (the clock of tspmi_two_opt_packed_euc2d_log), in kHz.
+ * No reference counterpart. */
+int tspmi_wall_clock_khz(int* khz);
+
+/*
  * Tournament selection: out[r] = pop[w_r] with w_r = cand[r, argmin_c fitness[cand[r, c]]]
  * (first minimum).  cand is (num_parents, k) int32 drawn on the host.
  * Replaces: src/algorithms/strategies/selection/tournament_selection.py:78-128 and

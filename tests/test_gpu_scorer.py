@@ -263,3 +263,44 @@ def test_local_search_strategy_with_learned_candidates(env):
     np.testing.assert_array_equal(got, want)
   knn = ctx.get_cpu_candidate_list(8)
   assert not np.array_equal(np.sort(cand, axis=1), np.sort(knn, axis=1))  # the lists really differ
+
+
+def test_pass_log_records_exact_costs_after_each_move(env):
+  """The logged coordinate scan: same tours as the plain launch; entry m of a tour's log is the
+  cost of the tour the plain kernel leaves after m passes (one move per pass until the optimum);
+  the clock never runs backwards; entries past a tour's last move stay untouched."""
+  torch, nat = env
+  n, P, k = 3000, 24, 8
+  xy_h = np.random.default_rng(5).integers(0, 1_000_001, (n, 2)).astype(np.float64)
+  xy = torch.from_numpy(xy_h).cuda()
+  D = torch.empty((n, n), dtype=torch.int32, device="cuda")
+  nat.dist_euc2d(xy, D)
+  cand = torch.empty((n, k), dtype=torch.int32, device="cuda")
+  nat.knn(D, k, cand)
+  tab = torch.empty((n, nat.candidate_lanes(k), 2), dtype=torch.int32, device="cuda")
+  nat.candidate_table(cand, D, tab)
+  start = torch.from_numpy(np.stack([np.random.default_rng(i).permutation(n) for i in range(P)]).astype(np.int32)).cuda()
+  M = 6000
+  log = torch.full((P, M + 1, 2), -1, dtype=torch.int64, device="cuda")
+  stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+  got = start.clone()
+  nat.two_opt_packed_euc2d_log(got, xy, tab, k, -1, log, stats)
+  plain = start.clone()
+  nat.two_opt_packed_euc2d(plain, xy, tab, k, -1)
+  np.testing.assert_array_equal(got.cpu().numpy(), plain.cpu().numpy())
+  h = log.cpu().numpy()
+  moves = (h[:, 1:, 0] >= 0).sum(axis=1)
+  assert moves.sum() == int(stats[1].item()) and moves.max() < M
+  cost = torch.empty(P, dtype=torch.float64, device="cuda")
+  nat.tour_cost(start, D, cost)
+  np.testing.assert_array_equal(h[:, 0, 0], cost.cpu().numpy().astype(np.int64))
+  for m in (1, 7, 40, int(moves.min())):
+    t = start.clone()
+    nat.two_opt_packed_euc2d(t, xy, tab, k, m)
+    nat.tour_cost(t, D, cost)
+    np.testing.assert_array_equal(h[:, m, 0], cost.cpu().numpy().astype(np.int64), err_msg=f"after {m} moves")
+  for p in range(P):
+    c, clk = h[p, : moves[p] + 1, 0], h[p, : moves[p] + 1, 1]
+    assert (np.diff(c) < 0).all() and (np.diff(clk) >= 0).all()
+    assert (h[p, moves[p] + 1 :] == -1).all()
+  assert nat.wall_clock_khz() > 0

@@ -69,6 +69,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_euc2d": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d_cost": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
+  "tspmi_two_opt_packed_euc2d_log": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
+  "tspmi_wall_clock_khz": [_P],
   "tspmi_scorer_param_count": [],
   "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
   "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
@@ -560,6 +562,36 @@ def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=Non
       raise ValueError("two_opt_packed_euc2d: cost shorter than the number of tours")
   _check(_lib.tspmi_two_opt_packed_euc2d_cost(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
                                               _ptr(stats), _ptr(cost), _stream(torch)), "tspmi_two_opt_packed_euc2d_cost")
+
+
+def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, stats=None, cost=None) -> None:
+  """:func:`two_opt_packed_euc2d` with a per-move progress log: ``pass_log`` int64 (P, M + 1, 2)
+  receives each tour's {cost, wall clock} at the start and after each of its first M moves."""
+  torch = require_device()
+  _need(pass_log, torch.int64, "pass_log", 3)
+  P, n = tours.shape
+  if pass_log.shape[0] != P or pass_log.shape[2] != 2 or pass_log.shape[1] < 1:
+    raise ValueError(f"two_opt_packed_euc2d_log: pass_log {tuple(pass_log.shape)} is not (P, M + 1, 2)")
+  _need(tours, torch.int32, "tours", 2)
+  _need(xy, torch.float64, "xy", 2)
+  _need(table, torch.int32, "table", 3)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  if xy.shape != (n, 2) or table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+    raise ValueError("two_opt_packed_euc2d_log: xy / table do not match n / k")
+  if cost is not None:
+    _need(cost, torch.float64, "cost", 1)
+  _check(_lib.tspmi_two_opt_packed_euc2d_log(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
+                                             _ptr(stats), _ptr(cost), _ptr(pass_log), pass_log.shape[1] - 1,
+                                             _stream(torch)), "tspmi_two_opt_packed_euc2d_log")
+
+
+def wall_clock_khz() -> int:
+  """Rate of the device's constant wall clock in kHz."""
+  require_device()
+  v = ctypes.c_int(0)
+  _check(_lib.tspmi_wall_clock_khz(ctypes.addressof(v)), "tspmi_wall_clock_khz")
+  return int(v.value)
 
 
 def nn_greedy_init(starts, D, out) -> None:

@@ -95,10 +95,15 @@ inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ?
 // (delta, i*k + r) key restores the reference tie-break across lanes.
 //
 // `cost` (nullable) receives the closed cost of the final tour, the sum of its edge costs.
+// `pass_log` (nullable) receives {tour cost, wall clock} int64 pairs: entry 0 at the start, entry
+// m after the m-th move (m <= log_moves; later moves are not logged), the cost kept exactly as the
+// start cost plus the applied deltas and the clock read with wall_clock64() (the device's
+// constant-rate counter, the same on every CU).
 template <int BLOCK, int KP, int UNROLL, int KL = KP, class Dist>
 __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                   int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
-                                  int64_t* red64, double* __restrict__ cost = nullptr) {
+                                  int64_t* red64, double* __restrict__ cost = nullptr,
+                                  int64_t* __restrict__ pass_log = nullptr, int log_moves = 0) {
   static_assert(KL <= KP && KL <= kWave, "lanes per row exceed the table row");
   constexpr int NW = BLOCK / kWave;
   constexpr int RPW = kWave / KL;  // rows per wave step
@@ -141,6 +146,16 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   const int rows = n - 2;
   int64_t my_deltas = 0;
   int moves = 0, passes = 0;
+  int64_t run_cost = 0;
+  if (pass_log != nullptr) {
+    int64_t part = 0;
+    for (int x = tid; x < n; x += BLOCK) part += s_edge[x];
+    run_cost = block_sum_i64(part, red64);
+    if (tid == 0) {
+      pass_log[0] = run_cost;
+      pass_log[1] = (int64_t)wall_clock64();
+    }
+  }
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
     int bd = 0;
@@ -192,6 +207,13 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
     __syncthreads();
     apply_move(s_tour, s_edge, s_pos, i, j, n, dist);
     ++moves;
+    if (pass_log != nullptr) {
+      run_cost += (int32_t)((uint32_t)(best >> 32) ^ 0x80000000u);  // the applied delta
+      if (tid == 0 && moves <= log_moves) {
+        pass_log[2 * moves] = run_cost;
+        pass_log[2 * moves + 1] = (int64_t)wall_clock64();
+      }
+    }
   }
   for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
   if (stats != nullptr) {

@@ -119,12 +119,16 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restr
                                                                 const double2* __restrict__ xy,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
                                                                 int64_t* __restrict__ stats,
-                                                                double* __restrict__ cost) {
+                                                                double* __restrict__ cost,
+                                                                int64_t* __restrict__ pass_log, int log_moves) {
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, CoordDist{xy}, ct, k, max_iter, stats,
-                                           lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr);
+                                           lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr,
+                                           pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1)
+                                                               : nullptr,
+                                           log_moves);
 }
 
 }  // namespace tspmi
@@ -158,8 +162,10 @@ extern "C" int tspmi_two_opt_euc2d(int32_t* tours, int P, int n, const double* x
   return TSPMI_OK;
 }
 
-extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
-                                               int k, int max_iter, int64_t* stats, double* cost, void* stream) {
+namespace {
+
+int packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k, int max_iter,
+                 int64_t* stats, double* cost, int64_t* pass_log, int log_moves, void* stream) {
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_packed_euc2d: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_packed_euc2d: k=%d outside [1, 32]", k);
   if (P == 0) return TSPMI_OK;
@@ -181,7 +187,7 @@ extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, con
   const int block = P <= cus ? 1024 : (P <= 4 * cus ? 512 : 256);
 #define TSPMI_CXY(BL, KPV, KLV)                                                                                      \
   hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
-                     stats, cost)
+                     stats, cost, pass_log, log_moves)
   if (block == 1024) {
     TSPMI_CAND_BY_K(TSPMI_CXY, 1024, k)
   } else if (block == 512) {
@@ -191,6 +197,29 @@ extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, con
   }
 #undef TSPMI_CXY
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d launch");
+  return TSPMI_OK;
+}
+
+}  // namespace
+
+extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
+                                               int k, int max_iter, int64_t* stats, double* cost, void* stream) {
+  return packed_euc2d(tours, P, n, xy, table, k, max_iter, stats, cost, nullptr, 0, stream);
+}
+
+extern "C" int tspmi_two_opt_packed_euc2d_log(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
+                                              int k, int max_iter, int64_t* stats, double* cost, int64_t* pass_log,
+                                              int log_moves, void* stream) {
+  TSPMI_CHECK_ARG(pass_log != nullptr && log_moves >= 0, "tspmi_two_opt_packed_euc2d_log: bad pass log");
+  return packed_euc2d(tours, P, n, xy, table, k, max_iter, stats, cost, pass_log, log_moves, stream);
+}
+
+extern "C" int tspmi_wall_clock_khz(int* khz) {
+  TSPMI_CHECK_ARG(khz != nullptr, "tspmi_wall_clock_khz: null pointer");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev);
+  if (e != hipSuccess) return hip_fail(e, "tspmi_wall_clock_khz");
   return TSPMI_OK;
 }
 
