@@ -206,6 +206,21 @@ int tspmi_two_opt_packed_euc2d_log(int32_t* tours, int P, int n, const double* x
                                    int max_iter, int64_t* stats, double* cost, int64_t* pass_log, int log_moves,
                                    void* stream);
 
+/*
+ * Best-improvement 2-opt for instances too large for the LDS-resident kernels (candidate scan
+ * above n ~ 13,500, full neighbourhood above n = 16,386): each tour's positions, edge costs and
+ * (candidate mode) inverse positions live in the caller's workspace ws, device int32[P * (3n + 1)],
+ * scratch whose contents need not be initialised and are dead after the call.  table (packed, as
+ * tspmi_two_opt_packed) selects the candidate scan with k candidates, NULL the full neighbourhood.
+ * Distances come from xy (EUC_2D coordinates, float64 (n, 2), computed as tspmi_dist_euc2d does)
+ * when xy is given, else from D (n * ld < 2^30).  cost (nullable) receives each final closed cost.
+ * Same pairs, order, tie-break, moves and counters as tspmi_two_opt / tspmi_two_opt_packed.
+ * Replaces: two_opt.py:831-899 above the shared-memory tile limit (two_opt.py:75-94, 707-709;
+ *           kernels/two_opt_batch.cu:192-268, the windowed scan).
+ */
+int tspmi_two_opt_large(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy, const int32_t* table,
+                        int k, int max_iter, int64_t* stats, double* cost, int32_t* ws, void* stream);
+
 /* Rate of the device's constant wall clock (the clock of tspmi_two_opt_packed_euc2d_log), in kHz.
  * No reference counterpart. */
 int tspmi_wall_clock_khz(int* khz);

@@ -682,6 +682,46 @@ def test_two_opt_large_n_lattice_ties_match_c_oracle(torch_mod, nat):
   assert list(stats) == list(wstats)
 
 
+@pytest.fixture(scope="module")
+def large_instance():
+  n = 25000
+  xy = uniform_coords(n, seed=25)
+  d = compute_distance_matrix(xy)
+  return xy, d, compute_candidate_list(d, 8)
+
+
+@pytest.mark.parametrize("source", ["coordinates", "matrix"])
+def test_two_opt_large_n_matches_c_oracle(torch_mod, nat, large_instance, source):
+  """n = 25,000 (beyond the LDS-resident kernels): the workspace kernels' candidate scan (k = 8,
+  1,500 passes) and full scan (2 passes) equal the C oracle on one row, tours and counters."""
+  import time
+
+  xy, d, cand = large_instance
+  n = d.shape[0]
+  say(f"two_opt_large n={n} source={source}")
+  tours = oinit.random_init(np.arange(n), 3, oinit.draw_row_seeds(np.random.default_rng(25), 3))
+  D = cu(d) if source == "matrix" else None
+  XY = cu(xy) if source == "coordinates" else None
+  tab = torch_mod.empty((n, nat.candidate_lanes(8), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cu(cand), D if D is not None else cu(d), tab)
+  for table, iters in ((tab, 1500), (None, 2)):
+    t = cu(tours)
+    stats = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+    cost = torch_mod.empty(3, dtype=torch_mod.float64, device="cuda")
+    torch_mod.cuda.synchronize()
+    t0 = time.perf_counter()
+    nat.two_opt_large(t, D, XY, table, 8 if table is not None else 0, iters, stats, cost)
+    torch_mod.cuda.synchronize()
+    say(f"  {'k=8' if table is not None else 'full'}: {time.perf_counter() - t0:.3f} s, stats {stats.tolist()}")
+    got = t.cpu().numpy()
+    want, wstats = c_two_opt.improve_batch(tours[:1], d, cand if table is not None else None, "best", iters)
+    np.testing.assert_array_equal(got[:1], want)
+    assert (np.sort(got, axis=1) == np.arange(n)).all()
+    np.testing.assert_array_equal(cost.cpu().numpy(), ops.tour_costs(got, d))
+    if table is None:
+      assert int(stats[2].item()) == 3 * iters and int(stats[1].item()) == 3 * iters
+
+
 # ---- EUC_2D coordinate-mode 2-opt ---------------------------------------------------------------
 
 

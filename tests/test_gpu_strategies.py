@@ -154,6 +154,24 @@ def test_context_candidate_list_is_the_reference_table_on_tie_rows(t, n, k):
   np.testing.assert_array_equal(ctx.get_gpu_candidate_list(k).cpu().numpy(), compute_candidate_list(d, k))
 
 
+def test_local_search_strategy_beyond_the_lds_limits(t):
+  """LocalSearchGPUTourTwoOpt at n = 20,000 (above the LDS-resident candidate and full scans)
+  takes the workspace kernels: candidate lists k = 8 and the full neighbourhood, against the C
+  oracle on one row; other modes are refused there."""
+  n = 20000
+  say(f"strategy beyond the LDS limits n={n}")
+  xy = np.random.default_rng(20).integers(0, 1_000_001, (n, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates("u20000", xy), seed=1)
+  routes = np.stack([np.random.default_rng(i).permutation(n) for i in range(2)]).astype(np.int32)
+  d = ctx.get_cpu_distances()
+  for k, iters in ((8, 400), (None, 2)):
+    got = t.LocalSearchGPUTourTwoOpt(max_iterations=iters, candidate_k=k).improve_batch(routes, ctx)
+    want, _ = c_two_opt.improve_batch(routes[:1], d, None if k is None else compute_candidate_list(d, k), "best", iters)
+    np.testing.assert_array_equal(got[:1], want)
+  with pytest.raises(ValueError, match="steepest"):
+    t.LocalSearchGPUTourTwoOpt(max_iterations=2, mode="first").improve_batch(routes, ctx)
+
+
 def test_config1_ga_end_to_end_matches_oracle(t):
   """Config 1: 52 uniform integer cities in [0,1000]^2 (seed 42), README memetic pipeline, 50 generations."""
   say("GA config 1 end to end")

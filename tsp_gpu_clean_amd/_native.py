@@ -71,6 +71,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_packed_euc2d_cost": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_two_opt_packed_euc2d_log": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
   "tspmi_wall_clock_khz": [_P],
+  "tspmi_two_opt_large": [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_scorer_param_count": [],
   "tspmi_scorer_features": [_P, _I, _P, _P, _I, _P, _P],
   "tspmi_scorer_labels": [_P, _I, _P, _P, _P, _P],
@@ -584,6 +585,51 @@ def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, 
   _check(_lib.tspmi_two_opt_packed_euc2d_log(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
                                              _ptr(stats), _ptr(cost), _ptr(pass_log), pass_log.shape[1] - 1,
                                              _stream(torch)), "tspmi_two_opt_packed_euc2d_log")
+
+
+#: Largest n of the LDS-resident kernels: candidate scans keep (3n + 1) int32 in LDS next to 1 KiB
+#: of reductions; the full-neighbourhood scans reach n = 16,386 (16 slots of 1024 lanes).
+CAND_LDS_MAX_N = (160 * 1024 - 1024 - 4) // 12
+FULL_LDS_MAX_N = 16 * 1024 + 2
+
+
+def two_opt_large(tours, D=None, xy=None, table=None, k: int = 0, max_iter: int = 10, stats=None, cost=None,
+                  ws=None) -> None:
+  """Best-improvement 2-opt with the tour arrays in a global workspace (any n whose scan indices
+  fit 32 bits): candidate scan on a packed ``table`` with ``k`` candidates, or the full
+  neighbourhood when ``table`` is None; distances from ``xy`` (EUC_2D) if given, else from ``D``.
+  ``ws`` (int32, P * (3n + 1)) is allocated when not given."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  P, n = tours.shape
+  if xy is None and D is None:
+    raise ValueError("two_opt_large: needs D or xy")
+  if xy is not None:
+    _need(xy, torch.float64, "xy", 2)
+    if xy.shape != (n, 2):
+      raise ValueError(f"two_opt_large: xy {tuple(xy.shape)} != ({n}, 2)")
+  if D is not None:
+    _need(D, torch.int32, "D", 2)
+  if table is not None:
+    _need(table, torch.int32, "table", 3)
+    if table.shape[0] != n or table.shape[1] != candidate_lanes(k):
+      raise ValueError("two_opt_large: table does not match n / k")
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  if cost is not None:
+    _need(cost, torch.float64, "cost", 1)
+    if cost.shape[0] < P:
+      raise ValueError("two_opt_large: cost shorter than the number of tours")
+  need = P * (3 * n + 1)
+  if ws is None:
+    ws = torch.empty(max(1, need), dtype=torch.int32, device="cuda")
+  else:
+    _need(ws, torch.int32, "ws")
+    if ws.numel() < need:
+      raise ValueError(f"two_opt_large: ws needs {need} int32")
+  _check(_lib.tspmi_two_opt_large(_ptr(tours), P, n, _ptr(D), 0 if D is None else D.shape[1], _ptr(xy), _ptr(table),
+                                  int(k), int(max_iter), _ptr(stats), _ptr(cost), _ptr(ws), _stream(torch)),
+         "tspmi_two_opt_large")
 
 
 def wall_clock_khz() -> int:

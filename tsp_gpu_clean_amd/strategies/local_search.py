@@ -98,14 +98,22 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       self._stats = torch.zeros(3, dtype=torch.int64, device="cuda")
     else:
       self._stats.zero_()
+    large = n > (_native.CAND_LDS_MAX_N if cand is not None else _native.FULL_LDS_MAX_N)
+    if large and _MODES[self.mode] != TSPMI_MODE_BEST:
+      raise ValueError(f"2-opt mode {self.mode!r} at n={n}: above the shared-memory limits only the steepest "
+                       "('batch', 'naive', 'best') scan is implemented")
     if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
       if table is None:
         table = context.get_gpu_candidate_table(int(self.candidate_k))
       xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(n, int(tours.shape[0])) else None
-      if xy is not None:  # same tours; distances from coordinates once the matrix leaves the caches
+      if large:  # the tour arrays in a global workspace (two_opt.py:75-94)
+        _native.two_opt_large(tours, dist, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
+      elif xy is not None:  # same tours; distances from coordinates once the matrix leaves the caches
         _native.two_opt_packed_euc2d(tours, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
       else:
         _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
+    elif large and cand is None:
+      _native.two_opt_large(tours, dist, context.get_gpu_euc2d_coordinates(), None, 0, self.max_iterations, self._stats)
     else:
       _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
     self.kernel_launches += 1
