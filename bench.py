@@ -469,6 +469,8 @@ def main() -> int:
   ap.add_argument("--no-extras", action="store_true")
   ap.add_argument("--no-scorer-eval", action="store_true")
   ap.add_argument("--generations", type=int, default=200)
+  ap.add_argument("--mlp-only", action="store_true",
+                  help="only time the scorer MLP (forward over 320,000 candidates, training steps on 65,536)")
   ap.add_argument("--copy-probe-mib", type=int, default=0,
                   help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
   args = ap.parse_args()
@@ -489,6 +491,11 @@ def main() -> int:
       dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20, "copies": 3}))
+    return 0
+  if args.mlp_only:
+    from tsp_gpu_clean_amd import scorer as S
+
+    print(json.dumps({"mlp": mlp_throughput(S.LearnedCandidateScorer.load())}))
     return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
   xy, D, start = c2_inputs(torch, _native)

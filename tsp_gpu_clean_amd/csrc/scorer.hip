@@ -67,23 +67,56 @@ struct alignas(16) ScorerLds : FwdLds {
   uint16_t w2t[kH * kH];
 };
 
-__device__ void load_fwd(FwdLds& L, const uint16_t* __restrict__ wbf, uint16_t* w2t) {
-  for (int x = threadIdx.x; x < kH * kF; x += blockDim.x) L.w1[x] = wbf[kOffW1 + x];
-  for (int x = threadIdx.x; x < kH * kH; x += blockDim.x) {
-    const uint16_t v = wbf[kOffW2 + x];
-    L.w2[x] = v;
-    if (w2t != nullptr) w2t[(x % kH) * kH + x / kH] = v;
+// The weights into LDS with 16-byte loads, every load of a thread issued before it stores any:
+// W1 (256 x 16 B), W2 (2048 x 16 B) and, for the backward pass, W2^T from its copy in global
+// memory (w2t_g, written by w2t_kernel), so that no workgroup transposes in LDS.  wbf and w2t_g are
+// 16-byte aligned (the entry points check it; the W1 and W2 offsets are multiples of 8 elements).
+constexpr int kMlpBlock = 256;  // 4 waves, 32 samples each per step
+
+__device__ void load_fwd(FwdLds& L, const uint16_t* __restrict__ wbf, uint16_t* w2t, const uint16_t* __restrict__ w2t_g) {
+  constexpr int V1 = kH * kF / 8, V2 = kH * kH / 8, E2 = V2 / kMlpBlock;
+  static_assert(V1 == kMlpBlock && V2 % kMlpBlock == 0, "weight staging assumes 256 threads");
+  const int t = threadIdx.x;
+  const uint4 r1 = reinterpret_cast<const uint4*>(wbf + kOffW1)[t];
+  uint4 r2[E2], rt[E2];
+#pragma unroll
+  for (int e = 0; e < E2; ++e) r2[e] = reinterpret_cast<const uint4*>(wbf + kOffW2)[t + e * kMlpBlock];
+  if (w2t != nullptr) {
+#pragma unroll
+    for (int e = 0; e < E2; ++e) rt[e] = reinterpret_cast<const uint4*>(w2t_g)[t + e * kMlpBlock];
   }
-  for (int x = threadIdx.x; x < kH; x += blockDim.x) {
-    L.b1[x] = bf2f(wbf[kOffB1 + x]);
-    L.b2[x] = bf2f(wbf[kOffB2 + x]);
-    L.w3[x] = bf2f(wbf[kOffW3 + x]);
+  float b1 = 0.0f, b2 = 0.0f, w3 = 0.0f, b3 = 0.0f;
+  if (t < kH) {
+    b1 = bf2f(wbf[kOffB1 + t]);
+    b2 = bf2f(wbf[kOffB2 + t]);
+    w3 = bf2f(wbf[kOffW3 + t]);
   }
-  if (threadIdx.x == 0) L.b3 = bf2f(wbf[kOffB3]);
+  if (t == 0) b3 = bf2f(wbf[kOffB3]);
+  reinterpret_cast<uint4*>(L.w1)[t] = r1;
+#pragma unroll
+  for (int e = 0; e < E2; ++e) reinterpret_cast<uint4*>(L.w2)[t + e * kMlpBlock] = r2[e];
+  if (w2t != nullptr) {
+#pragma unroll
+    for (int e = 0; e < E2; ++e) reinterpret_cast<uint4*>(w2t)[t + e * kMlpBlock] = rt[e];
+  }
+  if (t < kH) {
+    L.b1[t] = b1;
+    L.b2[t] = b2;
+    L.w3[t] = w3;
+  }
+  if (t == 0) L.b3 = b3;
   __syncthreads();
 }
 
-__device__ void load_scorer(ScorerLds& L, const uint16_t* __restrict__ wbf) { load_fwd(L, wbf, L.w2t); }
+__device__ void load_scorer(ScorerLds& L, const uint16_t* __restrict__ wbf, const uint16_t* __restrict__ w2t_g) {
+  load_fwd(L, wbf, L.w2t, w2t_g);
+}
+
+// out = W2^T (bf16 [128][128]) for the backward pass: out[c][r] = W2[r][c].
+__global__ void w2t_kernel(const uint16_t* __restrict__ wbf, uint16_t* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < kH * kH) out[(x % kH) * kH + x / kH] = wbf[kOffW2 + x];
+}
 
 // Forward of 32 samples (rows base .. base+31 of X, bf16 [S][16]) by one wave.
 // Returns z for sample s (valid in every lane; both halves agree); fills pre-activations.
@@ -139,8 +172,6 @@ __device__ __forceinline__ void forward32(const FwdLds& L, const uint16_t* __res
   f.z = part + __shfl_xor(part, 32, 64) + L.b3;
 }
 
-constexpr int kMlpBlock = 256;  // 4 waves, 32 samples each per step
-
 // Inference forward of 32 samples by one wave, same arithmetic as forward32 but keeping only the
 // packed layer-1 activations (no pre-activation arrays), so the kernel runs at high occupancy.
 __device__ __forceinline__ float forward32_z(const FwdLds& L, const uint16_t* __restrict__ X, int base, int S) {
@@ -184,7 +215,7 @@ __device__ __forceinline__ float forward32_z(const FwdLds& L, const uint16_t* __
 __global__ __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(3))) void mlp_forward_kernel(const uint16_t* __restrict__ X, int S,
                                                                 const uint16_t* __restrict__ wbf, float* __restrict__ z) {
   __shared__ FwdLds L;
-  load_fwd(L, wbf, nullptr);
+  load_fwd(L, wbf, nullptr, nullptr);
   const int wave = blockIdx.x * (kMlpBlock / 64) + threadIdx.x / 64;
   const int waves = gridDim.x * (kMlpBlock / 64);
   const int lane = threadIdx.x & 63;
@@ -194,70 +225,80 @@ __global__ __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(3))) 
   }
 }
 
-// Training pass 1: forward + backward to the per-sample deltas, one wave per 32 samples.
+// Training pass 1: forward + backward to the per-sample deltas, one wave per 32 samples, each
+// workgroup taking kBwdGroups groups of 128 samples (the weights staged once for all of them).
 // Writes (unit-major, bf16) a1T = relu(h1), d2T = delta2, d1T = delta1 as [128][S]; dz[S] = delta3;
-// per-block partial sums of dw3, db3 and the loss.
+// per-block partial sums of dw3, db3 and the loss, accumulated over the block's groups in order.
+constexpr int kBwdGroups = 2;
+constexpr int kBwdSamples = kBwdGroups * (kMlpBlock / 64) * 32;  // samples per workgroup
+
 __global__ __launch_bounds__(kMlpBlock) void mlp_backward_kernel(const uint16_t* __restrict__ X,
                                                                  const float* __restrict__ y, int S,
-                                                                 const uint16_t* __restrict__ wbf, float inv_batch,
+                                                                 const uint16_t* __restrict__ wbf,
+                                                                 const uint16_t* __restrict__ w2t_g, float inv_batch,
                                                                  uint16_t* __restrict__ a1T, uint16_t* __restrict__ d2T,
                                                                  uint16_t* __restrict__ d1T, float* __restrict__ part) {
   __shared__ ScorerLds L;
   __shared__ float s_w3[kMlpBlock / 64][kH + 2];
-  load_scorer(L, wbf);
+  load_scorer(L, wbf, w2t_g);
   const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
   const int s = lane & 31, h = lane >> 5;
-  const int base = (blockIdx.x * (kMlpBlock / 64) + wid) * 32;
   float dw3[4][16];
   float db3 = 0.0f, loss = 0.0f;
 #pragma unroll
   for (int T = 0; T < 4; ++T)
 #pragma unroll
     for (int i = 0; i < 16; ++i) dw3[T][i] = 0.0f;
-  if (base < S) {
-    Fwd f;
-    forward32(L, X, base, S, f);
-    const bool valid = base + s < S;
-    const float yy = valid ? y[base + s] : 0.0f;
-    const float zz = f.z;
-    const float p = 1.0f / (1.0f + __expf(-zz));
-    const float d3 = valid ? (p - yy) * inv_batch : 0.0f;
-    if (h == 0 && valid) loss = (fmaxf(zz, 0.0f) - zz * yy + log1pf(__expf(-fabsf(zz)))) * inv_batch;
-    if (h == 0) db3 = d3;
-    // delta2 = d3 * w3 * relu'(h2); dw3 += d3 * relu(h2)
-    float d2[4][16];
-#pragma unroll
-    for (int To = 0; To < 4; ++To)
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int u = dunit(To, h, i);
-        const float a2 = fmaxf(f.h2[To][i], 0.0f);
-        dw3[To][i] = d3 * a2;
-        d2[To][i] = f.h2[To][i] > 0.0f ? d3 * L.w3[u] : 0.0f;
-      }
-    // dH1^T = W2^T . delta2 (MFMA), delta1 = dH1 * relu'(h1)
-    bf16x8 bop[4][2];
-#pragma unroll
-    for (int To = 0; To < 4; ++To)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) bop[To][c] = pack_slots(&d2[To][8 * c]);
-#pragma unroll
-    for (int Ti = 0; Ti < 4; ++Ti) {
-      f32x16 acc = {};
+  // (kept in registers over the groups: accumulating dw3 in LDS group by group avoids the 25
+  // spilled registers but measured slower, 70.5 against 64.7 us at 65,536 samples)
+#pragma unroll 1
+  for (int grp = 0; grp < kBwdGroups; ++grp) {
+    const int base = blockIdx.x * kBwdSamples + (grp * (kMlpBlock / 64) + wid) * 32;
+    if (base < S) {
+      Fwd f;
+      forward32(L, X, base, S, f);
+      const bool valid = base + s < S;
+      const float yy = valid ? y[base + s] : 0.0f;
+      const float zz = f.z;
+      const float p = 1.0f / (1.0f + __expf(-zz));
+      const float d3 = valid ? (p - yy) * inv_batch : 0.0f;
+      if (h == 0 && valid) loss += (fmaxf(zz, 0.0f) - zz * yy + log1pf(__expf(-fabsf(zz)))) * inv_batch;
+      if (h == 0) db3 += d3;
+      // delta2 = d3 * w3 * relu'(h2); dw3 += d3 * relu(h2)
+      float d2[4][16];
 #pragma unroll
       for (int To = 0; To < 4; ++To)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2t, kH, 32 * Ti + s, 32 * To + 16 * c + 4 * h),
-                                                        bop[To][c], acc, 0, 0, 0);
-      if (valid) {
-#pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const int u = dunit(Ti, h, i);
-          const float d1 = f.h1[Ti][i] > 0.0f ? acc[i] : 0.0f;
-          d1T[(size_t)u * S + base + s] = f2bf(d1);
-          a1T[(size_t)u * S + base + s] = f2bf(fmaxf(f.h1[Ti][i], 0.0f));
-          d2T[(size_t)u * S + base + s] = f2bf(d2[Ti][i]);
+          const int u = dunit(To, h, i);
+          const float a2 = fmaxf(f.h2[To][i], 0.0f);
+          dw3[To][i] += d3 * a2;
+          d2[To][i] = f.h2[To][i] > 0.0f ? d3 * L.w3[u] : 0.0f;
+        }
+      // dH1^T = W2^T . delta2 (MFMA), delta1 = dH1 * relu'(h1)
+      bf16x8 bop[4][2];
+#pragma unroll
+      for (int To = 0; To < 4; ++To)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) bop[To][c] = pack_slots(&d2[To][8 * c]);
+#pragma unroll
+      for (int Ti = 0; Ti < 4; ++Ti) {
+        f32x16 acc = {};
+#pragma unroll
+        for (int To = 0; To < 4; ++To)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2t, kH, 32 * Ti + s, 32 * To + 16 * c + 4 * h),
+                                                          bop[To][c], acc, 0, 0, 0);
+        if (valid) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int u = dunit(Ti, h, i);
+            const float d1 = f.h1[Ti][i] > 0.0f ? acc[i] : 0.0f;
+            d1T[(size_t)u * S + base + s] = f2bf(d1);
+            a1T[(size_t)u * S + base + s] = f2bf(fmaxf(f.h1[Ti][i], 0.0f));
+            d2T[(size_t)u * S + base + s] = f2bf(d2[Ti][i]);
+          }
         }
       }
     }
@@ -518,7 +559,8 @@ extern "C" int tspmi_scorer_labels(const int32_t* tour, int n, const int32_t* nn
 extern "C" int tspmi_mlp_forward(const uint16_t* X, int S, const uint16_t* wbf, float* z, void* stream) {
   TSPMI_CHECK_ARG(S >= 0, "tspmi_mlp_forward: S < 0");
   if (S == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(X && wbf && z && ((uintptr_t)X % 16) == 0, "tspmi_mlp_forward: null or misaligned pointer");
+  TSPMI_CHECK_ARG(X && wbf && z && ((uintptr_t)X % 16) == 0 && ((uintptr_t)wbf % 16) == 0,
+                  "tspmi_mlp_forward: null or misaligned pointer");
   // Persistent grid: the weights are staged into LDS once per workgroup (36 KB, four workgroups
   // per CU), and every wave then walks groups of 32 samples.
   const int waves = (S + 31) / 32;
@@ -535,15 +577,19 @@ extern "C" int tspmi_mlp_grad(const uint16_t* X, const float* y, int S, const ui
   TSPMI_CHECK_ARG(S > 0 && X && y && wbf && work && partials && grad && ((uintptr_t)X % 16) == 0 && (S % 8) == 0,
                   "tspmi_mlp_grad: bad arguments (S must be a positive multiple of 8)");
   hipStream_t s = (hipStream_t)stream;
+  TSPMI_CHECK_ARG(((uintptr_t)wbf % 16) == 0 && ((uintptr_t)work % 16) == 0,
+                  "tspmi_mlp_grad: wbf and work must be 16-byte aligned");
   uint16_t* a1T = work;
   uint16_t* d2T = work + (size_t)kH * S;
   uint16_t* d1T = work + (size_t)2 * kH * S;
-  const int n1 = (S + kMlpBlock / 2 - 1) / (kMlpBlock / 2);  // 128 samples per block
+  uint16_t* w2t = work + (size_t)3 * kH * S;  // 16-byte aligned: S is a multiple of 8
+  const int n1 = (S + kBwdSamples - 1) / kBwdSamples;
   const int nw = (S + kGradChunk - 1) / kGradChunk;
   float* p1 = partials;
   float* pw = partials + (size_t)n1 * (kH + 2);
-  hipLaunchKernelGGL(mlp_backward_kernel, dim3(n1), dim3(kMlpBlock), 0, s, X, y, S, wbf, 1.0f / (float)S, a1T, d2T, d1T,
-                     p1);
+  hipLaunchKernelGGL(w2t_kernel, dim3(kH * kH / 256), dim3(256), 0, s, wbf, w2t);
+  hipLaunchKernelGGL(mlp_backward_kernel, dim3(n1), dim3(kMlpBlock), 0, s, X, y, S, wbf, w2t, 1.0f / (float)S, a1T, d2T,
+                     d1T, p1);
   hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(nw), dim3(kMlpBlock), 0, s, X, S, a1T, d2T, d1T, pw);
   hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3((kScorerParams + 1 + kSumParams - 1) / kSumParams), dim3(256), 0, s, pw,
                      nw, p1, n1, grad);
@@ -553,9 +599,9 @@ extern "C" int tspmi_mlp_grad(const uint16_t* X, const float* y, int S, const ui
 
 extern "C" int tspmi_mlp_work_sizes(int S, long* work_elems, long* partial_floats) {
   TSPMI_CHECK_ARG(S > 0 && work_elems && partial_floats, "tspmi_mlp_work_sizes: bad arguments");
-  const long n1 = (S + kMlpBlock / 2 - 1) / (kMlpBlock / 2);
+  const long n1 = (S + kBwdSamples - 1) / kBwdSamples;
   const long nw = (S + kGradChunk - 1) / kGradChunk;
-  *work_elems = 3L * kH * S;
+  *work_elems = 3L * kH * S + (long)kH * kH;
   *partial_floats = n1 * (kH + 2) + nw * kOffW3;
   return TSPMI_OK;
 }
