@@ -56,23 +56,40 @@ def c2_inputs(torch, native):
   return xy, D, tours
 
 
-def diag_gather_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4) -> int:
-  """Distance-gather lanes one pass of the full-neighbourhood steepest scan issues for one tour
-  (two_opt_kernel with 256 threads): the carry set-up (two loads per slot lane), row 0 over every
-  slot, then per wave the slot phases -- slot q of wave w runs while i < (n-2) - off(w, q), with
-  off(w, q) = 64 (4q + w) on even q and 64 (4q + 3 - w) on odd q."""
+MIX_PERIOD, MIX_ROWS = 8, 0b00100101  # rows 0, 2, 5 of each 8 from the coordinates
+
+
+def diag_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4, mix: bool = False) -> tuple[int, int]:
+  """(distance-gather lanes, coordinate lanes) of one pass of the full-neighbourhood steepest scan
+  over one tour (256 threads): the carry set-up (two gathers per slot lane), row 0 over every slot,
+  then per wave the slot phases -- slot q of wave w runs while i < (n-2) - off(w, q), with
+  off(w, q) = 64 (4q + w) on even q and 64 (4q + 3 - w) on odd q.  With `mix`, each phase takes its
+  rows in periods of 8 whose rows 0, 2 and 5 compute the distances from LDS coordinates, and the
+  phase's last (< 8) rows gather."""
   rows = n - 2
-  slot_rows = 0
+  gather_rows = coord_rows = 0
   for w in range(waves):
     off = [64 * (waves * q + ((waves - 1 - w) if q & 1 else w)) for q in range(qmax)]
-    slot_rows += qmax
+    gather_rows += qmax
     i = 1
     for ql in range(qmax, 0, -1):
       end = rows - off[ql - 1]
       if end > i:
-        slot_rows += ql * (end - i)
+        span = end - i
+        if mix:
+          periods = span // MIX_PERIOD
+          c = periods * bin(MIX_ROWS).count("1")
+          coord_rows += ql * c
+          gather_rows += ql * (span - c)
+        else:
+          gather_rows += ql * span
         i = end
-  return 64 * slot_rows + 2 * qmax * 64 * waves
+  return 64 * gather_rows + 2 * qmax * 64 * waves, 64 * coord_rows
+
+
+def diag_gather_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4) -> int:
+  """Distance-gather lanes of one pass of the matrix-only scan (two_opt_kernel)."""
+  return diag_lanes_per_pass(n, qmax, waves)[0]
 
 
 def gather_probe(torch, native, reps: int = 3) -> dict:
@@ -107,8 +124,8 @@ def c3_coords():
 
 def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
   """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch, refined tours,
-  passes per step).  With
-  candidates and device coordinates `xy`, the scan computes distances from the coordinates."""
+  passes per step).  With device coordinates `xy`, the candidate scan computes every distance from
+  them and the full scan 3 rows in 8 (tspmi_two_opt_full_euc2d)."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
   if cand is not None:
@@ -118,6 +135,8 @@ def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
       run = lambda: native.two_opt_packed_euc2d(work, xy, table, cand.shape[1], -1, stats)  # noqa: E731
     else:
       run = lambda: native.two_opt_packed(work, D, table, cand.shape[1], -1, stats)  # noqa: E731
+  elif xy is not None:
+    run = lambda: native.two_opt_full_euc2d(work, D, xy, -1, stats)  # noqa: E731
   else:
     run = lambda: native.two_opt(work, D, None, 0, -1, stats)  # noqa: E731
   for _ in range(warmup):
@@ -499,13 +518,20 @@ def main() -> int:
     return 0
   print(f"bench: C2 setup n={N_CITIES} P={POP}", file=sys.stderr, flush=True)
   xy, D, start = c2_inputs(torch, _native)
-  sec, deltas, moves, kms, refined, passes = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup)
+  xy_dev = torch.from_numpy(xy).cuda()
+  # the kernel LocalSearchGPUTourTwoOpt runs for this shape: EUC_2D, full neighbourhood, n <= 2050
+  sec, deltas, moves, kms, refined, passes = time_two_opt(torch, _native, D, start, None, args.steps, args.warmup,
+                                                          xy_dev)
   print(f"bench: full neighbourhood {deltas / sec:.3e} deltas/s, {kms:.1f} ms/launch", file=sys.stderr, flush=True)
   value = deltas / sec
   achieved = deltas * BYTES_PER_DELTA / (kms * 1e-3) / 1e9
   traffic = load_traffic()
-  lanes = passes * diag_gather_lanes_per_pass(N_CITIES)
+  glanes, clanes = diag_lanes_per_pass(N_CITIES, mix=True)
+  lanes, coord_lanes = passes * glanes, passes * clanes
   probe = gather_probe(torch, _native)
+  msec, mdeltas, _, mkms, mtours, _ = time_two_opt(torch, _native, D, start, None, 1, 0)
+  if not torch.equal(mtours, refined):
+    raise RuntimeError("matrix-only and mixed full scans disagree on the C2 tours")
   line = {
     "metric": "2-opt deltas/s & gens/s (n=1000,pop=1024); time-to-2% gap, learned vs plain kNN",
     "value": value,
@@ -534,17 +560,26 @@ def main() -> int:
       "measured_hbm_frac": (traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
       "note": "achieved counts the algorithmic 8 B per delta; the 4 MB distance matrix stays in the caches, so the "
               "HBM moves only `traffic` bytes per launch and the binding unit is on chip (roofline_onchip)",
-      "kernel": "tspmi::two_opt_kernel<4>", "kernel_ms_per_launch": kms,
+      "kernel": "tspmi::two_opt_mix_kernel<4>", "kernel_ms_per_launch": kms,
       "bytes_per_unit": BYTES_PER_DELTA, "units_per_launch": deltas,
     },
     "roofline_onchip": {
-      "bound": "texture-address gather issue (TA)", "achieved": lanes / (kms * 1e-3), "peak": GATHER_PEAK,
-      "unit": "gather-lanes/s", "frac": lanes / (kms * 1e-3) / GATHER_PEAK,
+      "bound": "texture-address gather issue (TA) on the gathered rows", "achieved": lanes / (kms * 1e-3),
+      "peak": GATHER_PEAK, "unit": "gather-lanes/s", "frac": lanes / (kms * 1e-3) / GATHER_PEAK,
       "peak_basis": f"{GATHER_LANES_PER_CLK_CU:g} random 4-byte lanes per clock per CU (a wave64 gather at random "
                     f"offsets of one 4 KB row occupies the TA 32 cycles) x {CUS} CUs x {PEAK_CLOCK_HZ / 1e9:g} GHz",
-      "lanes_per_launch": lanes, "lanes_per_pass_per_tour": diag_gather_lanes_per_pass(N_CITIES),
+      "lanes_per_launch": lanes, "lanes_per_pass_per_tour": glanes,
+      "coordinate_lanes_per_launch": coord_lanes, "coordinate_lanes_per_pass_per_tour": clanes,
       "passes_per_launch": passes, "probe": probe,
+      "note": "3 rows in 8 take their distances from LDS coordinates (FP64 VALU) and issue no gather; the TA "
+              "fraction counts only the gathered rows",
       "counters": "profiles/two_opt_c2_counters.md (TA busy fraction, L1 hit rate, VALU and LDS issue)",
+    },
+    "matrix_only": {
+      "kernel": "tspmi::two_opt_kernel<4>", "kernel_ms_per_launch": mkms, "deltas_per_s": mdeltas / msec,
+      "gather_lanes_per_s": passes * diag_gather_lanes_per_pass(N_CITIES) / (mkms * 1e-3),
+      "gather_frac": passes * diag_gather_lanes_per_pass(N_CITIES) / (mkms * 1e-3) / GATHER_PEAK,
+      "note": "same tours and counters with every distance gathered from the matrix (checked in this run)",
     },
   }
   if not args.no_extras:
@@ -554,10 +589,10 @@ def main() -> int:
     cand = torch.empty((N_CITIES, 20), dtype=torch.int32, device="cuda")
     _native.knn(D, 20, cand)
     # the distance source the local-search strategy picks for this shape (coordinates here)
-    xy_dev = torch.from_numpy(xy).cuda() if coordinate_candidates_pay(N_CITIES, POP) else None
-    s20, d20, _, k20ms, _, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, xy_dev)
+    cxy = xy_dev if coordinate_candidates_pay(N_CITIES, POP) else None
+    s20, d20, _, k20ms, _, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, cxy)
     line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
-                             "deltas_per_step": d20, "distances": "coordinates" if xy_dev is not None else "matrix"}
+                             "deltas_per_step": d20, "distances": "coordinates" if cxy is not None else "matrix"}
     print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)
     gps, gwall, gbest, gdel = time_generations(torch, _native, args.generations)
     line["gens_per_s"] = gps

@@ -113,6 +113,16 @@ int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const 
                   int mode, int max_iter, int64_t* stats, void* stream);
 
 /*
+ * tspmi_two_opt(mode = BEST, no candidates) for EUC_2D instances with n <= 2050: D must be the
+ * matrix tspmi_dist_euc2d built from xy (float64 (n, 2)).  Three rows in eight of the scan compute
+ * their distances from the coordinates (euc2d_nint, the matrix's own function), the others gather
+ * them from D; the values are the same, so are the tours, moves and counters.
+ * Replaces: two_opt.py:831-899 (full neighbourhood, best) on EUC_2D problems.
+ */
+int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy, int max_iter,
+                        int64_t* stats, void* stream);
+
+/*
  * Packed candidate table for tspmi_two_opt_packed: out is (n, KP) pairs of int32
  * {N[u, r], D[u, N[u, r]]} (pairs {-1, 0} for k <= r < KP) with KP = 8, 16 or 32 — the smallest of
  * those >= k.  Build once per (instance, k); 1 <= k <= 32.

@@ -771,6 +771,61 @@ def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iter
   assert sb.cpu().tolist() == sa.cpu().tolist()
 
 
+@pytest.mark.parametrize("n,P,iters", [(4, 3, -1), (5, 6, -1), (10, 6, -1), (63, 4, -1), (66, 4, -1), (257, 3, -1),
+                                       (258, 3, 60), (700, 3, 40), (1025, 3, 30), (1026, 3, 30), (1283, 2, 20),
+                                       (2050, 2, 12)])
+@pytest.mark.parametrize("kind", ["uniform", "float", "lattice"])
+def test_two_opt_full_euc2d_matches_c_oracle(torch_mod, nat, n, P, iters, kind):
+  """Full-neighbourhood steepest scan with rows from the coordinates, around the slot-phase and
+  8-row mix-period boundaries, on integer, fractional and tie-heavy lattice instances: tours and
+  counters equal the C oracle's on the matrix of the same coordinates."""
+  say(f"two_opt_full_euc2d n={n} P={P} iters={iters} {kind}")
+  xy = {"uniform": lambda: uniform_coords(n, seed=n + 11),
+        "float": lambda: np.random.default_rng(n).random((n, 2)) * 1000.0,
+        "lattice": lambda: lattice_coords(n)}[kind]()
+  d = compute_distance_matrix(xy)
+  D, _ = device_dist(nat, torch_mod, xy)
+  np.testing.assert_array_equal(D.cpu().numpy(), d)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + 5), P))
+  t = cu(tours)
+  stats = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  nat.two_opt_full_euc2d(t, D, cu(xy), iters, stats)
+  want, wstats = c_two_opt.improve_batch(tours, d, None, "best", iters)
+  np.testing.assert_array_equal(t.cpu().numpy(), want)
+  assert stats.cpu().tolist() == list(wstats)
+
+
+def test_two_opt_full_euc2d_population_equals_matrix_kernel(torch_mod, nat):
+  """P=1024, n=1000 to the local optimum: the same tours and counters as the matrix-only scan."""
+  say("two_opt_full_euc2d P=1024 n=1000 to optimum")
+  n, P = 1000, 1024
+  xy = uniform_coords(n, seed=7)
+  D, _ = device_dist(nat, torch_mod, xy)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(8), P))
+  a, b = cu(tours), cu(tours)
+  sa = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  sb = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  nat.two_opt(a, D, None, 0, -1, sa)
+  nat.two_opt_full_euc2d(b, D, cu(xy), -1, sb)
+  np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy())
+  assert sb.cpu().tolist() == sa.cpu().tolist()
+  sample = [0, 1023]
+  want, _ = c_two_opt.improve_batch(tours[sample], D.cpu().numpy(), None, "best", -1)
+  np.testing.assert_array_equal(b.cpu().numpy()[sample], want)
+
+
+def test_two_opt_full_euc2d_rejects_bad_input(torch_mod, nat):
+  xy = uniform_coords(2051)
+  D, _ = device_dist(nat, torch_mod, xy)
+  with pytest.raises(ValueError):  # above the diagonal scan's 8 slots
+    nat.two_opt_full_euc2d(cu(np.arange(2051, dtype=np.int32)[None, :]), D, cu(xy), 1)
+  with pytest.raises(ValueError):  # coordinates of another instance size
+    nat.two_opt_full_euc2d(cu(np.arange(10, dtype=np.int32)[None, :]), D[:10, :10].contiguous(), cu(xy[:11]), 1)
+  t = cu(np.arange(3, dtype=np.int32)[None, :])
+  nat.two_opt_full_euc2d(t, D[:3, :3].contiguous(), cu(xy[:3]), 5)  # n < 4: the tour is kept
+  assert t.cpu().tolist() == [[0, 1, 2]]
+
+
 def test_two_opt_euc2d_rejects_bad_input(torch_mod, nat):
   t = cu(np.arange(9000, dtype=np.int32)[None, :])
   with pytest.raises(ValueError):

@@ -234,3 +234,23 @@ def test_explicit_distances_are_checked_before_the_int32_cast():
               np.array([[0, -1], [-1, 0]])):
     with pytest.raises(ValueError):
       explicit_int32_distances(bad)
+
+
+def test_bench_diag_lane_counts_match_brute_force():
+  """The bench's per-pass lane counts of the diagonal scan against a direct count over (wave, slot,
+  row): slot q of wave w is live at row i >= 1 while i < (n-2) - off(w, q); the mixed scan moves 3
+  rows in 8 of each phase to coordinates without changing the total."""
+  import bench
+
+  def off(w, q):
+    return 64 * (4 * q + ((3 - w) if q & 1 else w))
+
+  for n in (4, 63, 66, 257, 700, 1000, 1026):
+    live = sum(1 for w in range(4) for q in range(4) for i in range(n - 2)
+               if i == 0 or i < (n - 2) - off(w, q))
+    g, c = bench.diag_lanes_per_pass(n)
+    assert c == 0 and g == 64 * live + 2 * 4 * 64 * 4
+    mg, mc = bench.diag_lanes_per_pass(n, mix=True)
+    assert mg + mc == g
+  g, c = bench.diag_lanes_per_pass(1000, mix=True)
+  assert 0.35 < c / (g + c) < 0.375

@@ -37,6 +37,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_knn_flagged": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
+  "tspmi_two_opt_full_euc2d": [_P, _I, _I, _P, _I, _P, _I, _P, _P],
   "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
@@ -529,6 +530,26 @@ def nn_init(seeds, customers, cand, work, out) -> None:
     raise ValueError("nn_init: work needs 2 * P * n int32")
   _check(_lib.tspmi_nn_init(_ptr(seeds), out.shape[0], _ptr(customers), customers.shape[0], _ptr(cand), k, _ptr(work),
                             _ptr(out), _stream(torch)), "tspmi_nn_init")
+
+
+#: Largest n of tspmi_two_opt_full_euc2d (the diagonal scan's 8 slots of 256 lanes).
+FULL_EUC2D_MAX_N = 2050
+
+
+def two_opt_full_euc2d(tours, D, xy, max_iter: int = 10, stats=None) -> None:
+  """Full-neighbourhood best-improvement 2-opt on an EUC_2D instance: distances gathered from the
+  matrix ``D`` built from ``xy`` on most rows and computed from ``xy`` on the others (n <= 2050)."""
+  torch = require_device()
+  _need(tours, torch.int32, "tours", 2)
+  _need(D, torch.int32, "D", 2)
+  _need(xy, torch.float64, "xy", 2)
+  if stats is not None:
+    _need(stats, torch.int64, "stats")
+  P, n = tours.shape
+  if xy.shape != (n, 2) or D.shape[0] < n or D.shape[1] < n:
+    raise ValueError(f"two_opt_full_euc2d: xy {tuple(xy.shape)} / D {tuple(D.shape)} do not match n={n}")
+  _check(_lib.tspmi_two_opt_full_euc2d(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(xy), int(max_iter), _ptr(stats),
+                                  _stream(torch)), "tspmi_two_opt_full_euc2d")
 
 
 def two_opt_euc2d(tours, xy, max_iter: int = 10, stats=None) -> None:

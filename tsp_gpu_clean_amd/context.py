@@ -135,7 +135,7 @@ class ProblemContext:
 
   def get_gpu_euc2d_coordinates(self):
     """Device (n, 2) float64 coordinates when the distances are EUC_2D of them, else None."""
-    if self.problem.edge_type != "EUC_2D" or self.problem.coordinates is None:
+    if self.problem.edge_type != "EUC_2D" or self.problem.coordinates is None or self.problem.distances is not None:
       return None
     return self.get_gpu_coordinates()
 

@@ -45,14 +45,26 @@ namespace tspmi {
 //
 // Each lane visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per
 // lane and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
+//
+// With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), rows 0, 2 and 5 of every 8
+// compute their new distances from the coordinates with euc2d_nint, the function that built the
+// matrix, instead of gathering them: the carried values are the same either way, and those rows
+// put no work on the texture addresser, whose 64-lane random gathers (32 cycles each) bind the
+// matrix-only scan, while the FP64 arithmetic runs on otherwise idle VALU cycles.  3 rows in 8
+// was the best mix measured at n = 1000 (430 ms per C2 launch, against 574 ms without it and
+// 440 / 472 / 500 ms for 1 row in 3 / 4 / 2).
+constexpr int kMixPeriod = 8;
+constexpr int kMixRows = 0b00100101;
+
 __device__ __forceinline__ int diag_slot_offset(int w, int q) {
   return kWave * (kTwoOptWaves * q + ((q & 1) ? (kTwoOptWaves - 1 - w) : w));
 }
 
-template <int QMAX>
+template <int QMAX, bool MIX = false>
 struct DiagScan {
   const int32_t* s_tour;
   const int32_t* s_edge;
+  const double2* s_xy;  // MIX only
   const int32_t* __restrict__ D;
   int ld, n, wid;
   int jo_even, jo_odd;  // j - i of the lane in even / odd slots, without the 256 q term
@@ -60,13 +72,24 @@ struct DiagScan {
 
   __device__ __forceinline__ int jo(int q) const { return ((q & 1) ? jo_odd : jo_even) + q * kTwoOptBlock; }
 
-  // Row i over the QL lowest slots; FIRST marks row 0, whose pair (0, n-1) is excluded.
-  template <int QL, bool FIRST>
+  // Row i over the QL lowest slots; FIRST marks row 0, whose pair (0, n-1) is excluded; COORD
+  // takes the row's new distances from the coordinates.
+  template <int QL, bool FIRST, bool COORD = false>
   __device__ __forceinline__ void row(int i) {
-    const int32_t* __restrict__ rn = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[i + 2]) * ld;
+    const int a = __builtin_amdgcn_readfirstlane(s_tour[i + 2]);
     int gnext[QL];
+    if constexpr (COORD) {
+      const double2 pa = s_xy[a];
 #pragma unroll
-    for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, s_tour[i + jo(q) + 2]);
+      for (int q = 0; q < QL; ++q) {
+        const double2 pb = s_xy[s_tour[i + jo(q) + 2]];
+        gnext[q] = euc2d_nint(pa.x, pa.y, pb.x, pb.y);
+      }
+    } else {
+      const int32_t* __restrict__ rn = D + (size_t)a * ld;
+#pragma unroll
+      for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, s_tour[i + jo(q) + 2]);
+    }
     const int neg_eab = -s_edge[i];
 #pragma unroll
     for (int q = 0; q < QL; ++q) {
@@ -80,11 +103,23 @@ struct DiagScan {
     }
   }
 
+  // Rows i + R .. i + kMixPeriod - 1 of a MIX period.
+  template <int QL, int R = 0>
+  __device__ __forceinline__ void mix_rows(int i) {
+    if constexpr (R < kMixPeriod) {
+      row<QL, false, ((kMixRows >> R) & 1) != 0>(i + R);
+      mix_rows<QL, R + 1>(i);
+    }
+  }
+
   // Phases QL, QL-1, ..., 1 from row i on.
   template <int QL>
   __device__ __forceinline__ void phases(int i) {
     if constexpr (QL > 0) {
       const int end = (n - 2) - diag_slot_offset(wid, QL - 1);
+      if constexpr (MIX) {
+        for (; i + kMixPeriod - 1 < end; i += kMixPeriod) mix_rows<QL>(i);
+      }
       for (; i + 1 < end; i += 2) {  // two rows per trip: the carry moves become register renames
         row<QL, false>(i);
         row<QL, false>(i + 1);
@@ -95,12 +130,14 @@ struct DiagScan {
   }
 };
 
-template <int QMAX>
+template <int QMAX, bool MIX = false>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
-                                                        const int32_t* __restrict__ D, int ld, int wid, int lane) {
-  DiagScan<QMAX> sc;
+                                                        const int32_t* __restrict__ D, int ld, int wid, int lane,
+                                                        const double2* s_xy = nullptr) {
+  DiagScan<QMAX, MIX> sc;
   sc.s_tour = s_tour;
   sc.s_edge = s_edge;
+  sc.s_xy = s_xy;
   sc.D = D;
   sc.ld = ld;
   sc.n = n;
@@ -526,6 +563,57 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
   }
 }
 
+// Full neighbourhood, best improvement, for EUC_2D instances: the diagonal scan with its MIX rows
+// from the coordinates, staged in LDS next to the tour (16 n bytes first, then the tour and edge
+// arrays with their padding).  Same pairs, deltas, tie-break, moves and counters as two_opt_kernel.
+template <int QMAX>
+__global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __restrict__ tours, int n,
+                                                                   const int32_t* __restrict__ D, int ld,
+                                                                   const double2* __restrict__ xy, int max_iter,
+                                                                   int pad, int64_t* __restrict__ stats) {
+  extern __shared__ __align__(16) int32_t lds[];
+  __shared__ uint64_t red[kTwoOptWaves];
+  double2* s_xy = reinterpret_cast<double2*>(lds);  // n
+  int32_t* s_tour = lds + 4 * n;                    // n + 1 + pad
+  int32_t* s_edge = s_tour + n + 1 + pad;           // n + pad
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = wave_id();
+  int32_t* tour = tours + (size_t)blockIdx.x * n;
+  for (int x = tid; x < n; x += kTwoOptBlock) {
+    s_xy[x] = xy[x];
+    const int v = clamp_id(tour[x], n);
+    s_tour[x] = v;
+    if (x == 0) s_tour[n] = v;
+  }
+  for (int x = tid; x < pad; x += kTwoOptBlock) {
+    s_tour[n + 1 + x] = 0;
+    s_edge[n + x] = -(1 << 30);
+  }
+  __syncthreads();
+  for (int x = tid; x < n; x += kTwoOptBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
+  __syncthreads();
+  int moves = 0, passes = 0;
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = scan_full_best_diag<QMAX, true>(s_tour, s_edge, n, D, ld, wid, lane, s_xy);
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)n);
+    const int j = (int)(idx - (uint32_t)i * (uint32_t)n);
+    __syncthreads();
+    apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
+    ++moves;
+  }
+  for (int x = tid; x < n; x += kTwoOptBlock) tour[x] = s_tour[x];
+  if (stats != nullptr && tid == 0) {
+    atomicAdd((unsigned long long*)&stats[0], (unsigned long long)((int64_t)passes * n * (n - 3) / 2));
+    atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+    atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+  }
+}
+
 template <int BLOCK, int KP, int UNROLL, int KL>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_kernel(int32_t* __restrict__ tours, int n,
                                                              const int32_t* __restrict__ D, int ld,
@@ -835,6 +923,31 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
                        use_cand ? cand : nullptr, use_cand ? k : 0, mode, max_iter, 0, stats);
   }
   TSPMI_CHECK_LAUNCH("tspmi_two_opt launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy,
+                                   int max_iter, int64_t* stats, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_full_euc2d: negative shape P=%d n=%d", P, n);
+  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;  // reference: n < 4 or no budget is a copy
+  TSPMI_CHECK_ARG(tours != nullptr && D != nullptr && xy != nullptr, "tspmi_two_opt_full_euc2d: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy & 15) == 0, "tspmi_two_opt_full_euc2d: xy must be 16-byte aligned");
+  TSPMI_CHECK_ARG(ld >= n && (long)n * ld < kMaxMatrixElems, "tspmi_two_opt_full_euc2d: bad ld=%d for n=%d", ld, n);
+  TSPMI_CHECK_ARG(n - 2 <= kTwoOptWaves * 8 * kWave, "tspmi_two_opt_full_euc2d: n=%d above 2050", n);
+  const int q = n - 2 <= kTwoOptWaves * 4 * kWave ? 4 : 8;
+  const int pad = q * kTwoOptBlock + kWave;
+  const size_t lds = ((size_t)4 * n + 2 * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
+  TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_full_euc2d: n=%d needs %zu B of LDS", n, lds);
+  hipStream_t s = (hipStream_t)stream;
+  const double2* p = reinterpret_cast<const double2*>(xy);
+  if (q == 4)
+    hipLaunchKernelGGL(two_opt_mix_kernel<4>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, p, max_iter, pad,
+                       stats);
+  else
+    hipLaunchKernelGGL(two_opt_mix_kernel<8>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, p, max_iter, pad,
+                       stats);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d launch");
   return TSPMI_OK;
 }
 

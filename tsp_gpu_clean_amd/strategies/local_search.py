@@ -115,7 +115,13 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     elif large and cand is None:
       _native.two_opt_large(tours, dist, context.get_gpu_euc2d_coordinates(), None, 0, self.max_iterations, self._stats)
     else:
-      _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
+      xy = None
+      if cand is None and _MODES[self.mode] == TSPMI_MODE_BEST and n <= _native.FULL_EUC2D_MAX_N:
+        xy = context.get_gpu_euc2d_coordinates()
+      if xy is not None:  # same tours; 3 rows in 8 take their distances from the coordinates
+        _native.two_opt_full_euc2d(tours, dist, xy, self.max_iterations, self._stats)
+      else:
+        _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
     self.kernel_launches += 1
     if on_gpu:
       return restore_routes_shape(tours, was_1d)
