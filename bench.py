@@ -236,25 +236,48 @@ def time_uneven_batch(torch, generations=100):
   torch.cuda.synchronize()
   setup = time.perf_counter() - t0
   d = ga.draw(generations + 2)
-  shapes = ga.autotune(d, g=generations)
+  shapes = dict(ga.autotune(d, g=generations))
   ga.run(d, 2, start=generations)  # warm-up generations on the spare draw slots
-  graph = ga.capture(d, generations)
-  ga.stats.zero_()
   torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  graph.replay()
-  torch.cuda.synchronize()
-  wall = time.perf_counter() - t0
+  start = [p.clone() for p in ga.pop], [k.clone() for k in ga.keys], ga.cur
+  forms, finals = {}, {}
+  # both forms from the same state: one persistent launch per generation, and one launch per stage
+  for name, fused in (("fused", True), ("staged", False)):
+    for src, dst in zip(start[0], ga.pop):
+      dst.copy_(src)
+    for src, dst in zip(start[1], ga.keys):
+      dst.copy_(src)
+    ga.cur = start[2]
+    ga.shapes["fused"] = fused
+    graph = ga.capture(d, generations)
+    ga.stats.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ga.verify()
+    forms[name] = {"wall_s": wall, "gens_per_s": generations / wall, "elite_deltas": int(ga.stats[0].item())}
+    finals[name] = ga.pop[ga.cur].clone()
+    del graph
+  if not torch.equal(finals["fused"], finals["staged"]):
+    raise RuntimeError("the fused and staged uneven-batch runs disagree")
+  ga.shapes["fused"] = shapes["fused"]
+  chosen = forms["fused" if shapes["fused"] else "staged"]
+  wall = chosen["wall_s"]
   ns = np.array([c.shape[0] for c in coords])
-  gbytes = 16 * 64 * int(ns.sum()) + 8 * int(ga.stats[0].item()) / generations
+  gbytes = 16 * 64 * int(ns.sum()) + 8 * chosen["elite_deltas"] / generations
   return {"instances": 256, "n_range": [int(ns.min()), int(ns.max())], "population": 64, "generations": generations,
           "wall_s": wall, "gens_per_s": generations / wall, "instance_gens_per_s": 256 * generations / wall,
+          "form": "fused" if shapes["fused"] else "staged", "fused_gens_per_s": forms["fused"]["gens_per_s"],
+          "staged_gens_per_s": forms["staged"]["gens_per_s"], "autotune": getattr(ga, "tuning", None),
           "setup_s": setup, "distance_bytes": int(4 * (ns * ns).sum()), "shapes": shapes,
           "bytes_per_generation": gbytes, "achieved_GBps": gbytes * generations / wall / 1e9,
           "hbm_frac": gbytes * generations / wall / 1e9 / HBM_PEAK_GBS,
           "workload": "256 instances, n uniform-integer [50,2000], uniform coords [0,1e6]^2, P=64 each, tournament 5, "
-                      "PMX, inversion 0.05, elite-16 2-opt best k=20 max_iter 10, one launch per stage for all "
-                      "instances, 100 generations captured in one HIP graph, autotuned launch shapes"}
+                      "PMX, inversion 0.05, elite-16 2-opt best k=20 max_iter 10, 100 generations captured in one HIP "
+                      "graph; form (one persistent launch per generation, or one launch per stage) and launch shapes "
+                      "autotuned; both forms timed from the same state, identical results checked"}
 
 
 C5_SEEDS = (2024, 2025, 2026)

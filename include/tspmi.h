@@ -430,6 +430,33 @@ int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32_t* item_in
 /* First minimum among the first `count` keys of every instance.  Replaces genetic_algorithm.py:708-739. */
 int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best, void* stream);
 
+/* One whole generation of every instance in ONE persistent launch: the fused child generation,
+ * the stable (mu + lambda) survival, the survivors' copy into popB/keysB and the elite candidate
+ * 2-opt with its costs (plus, when `best` is given, the first minimum of each instance's first
+ * best_count keys), with the same results as tspmi_batch_generation + tspmi_batch_survival +
+ * tspmi_batch_gather + tspmi_batch_two_opt_packed_mixed + tspmi_batch_cost (+ tspmi_batch_best).
+ * `tasks` is a device list of ntasks uint32 task words, kind << 30 | instance << 12 | index, in
+ * this order: every child task (0, i, g) for g < ceil(P / children_per_task) (children
+ * g * children_per_task onwards; children_per_task even and >= 2), then every survive task
+ * (1, i, 0), then every elite row (2, i, d) for d < min(elite, P); instances longest first within
+ * each kind.  `counters` is a device int32 array of 4 + 3 B words, zero before the first call and
+ * owned by one batch: word 2 is set to 1 if a dependency wait of a call ran out (that call's
+ * results are then invalid), word 3 counts calls, the others are back at zero after every call,
+ * so consecutive calls on one stream and graph replays need no clearing; calls that share
+ * `counters` must not overlap.  block in {256, 512, 1024}; grid <= 0 launches as many workgroups
+ * as fit on the device at once.
+ * Replaces genetic_algorithm.py:120-296 (selection, crossover, mutation, evaluation, survival
+ * steps) and pipeline_steps.py:153-195 (EliteOnlyLocalSearchStep) per instance; the reference
+ * has no multi-instance batch. */
+int tspmi_batch_fused_generation(const int64_t* desc, int B, const uint32_t* tasks, int ntasks, int32_t* counters,
+                                 int children_per_task, int max_n, int max_p, const int32_t* popA, double* keysA,
+                                 int32_t* off, int32_t* order, int32_t* popB, double* keysB, const int32_t* D_all,
+                                 const double* xy_all, const int32_t* table_all, const int32_t* tour_cand,
+                                 int k_tour, const int32_t* cut_lo, const int32_t* cut_hi, const double* probs,
+                                 const int32_t* pos1, const int32_t* pos2, double rate, int elite, int k,
+                                 int max_iter, int coord_min_n, int block, int grid, int64_t* stats, int32_t* best,
+                                 int best_count, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Learned candidate-edge scorer.  No counterpart exists in the reference (it extends the
  * reference's k-nearest candidate lists of src/distances/matrix.py:140-211 and

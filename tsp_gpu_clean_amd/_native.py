@@ -63,6 +63,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_two_opt_packed_mixed": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
+  "tspmi_batch_fused_generation": [_P, _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P,
+                                   _P, _P, _D, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P],
   "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_ox": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_dist_metric": [_P, _I, _I, _P, _I, _P, _P],

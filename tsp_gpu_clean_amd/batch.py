@@ -1,7 +1,9 @@
 """Uneven multi-instance memetic GA: many TSP instances of different n, each with its own
-population and generator, advanced together — one launch per pipeline stage covering every
-instance, whole runs of generations captured in one HIP graph, launch shapes picked by a small
-autotuner.
+population and generator, advanced together — either one persistent launch per generation
+(``tspmi_batch_fused_generation``: a dependency-ordered task list of children, survival, survivor
+copies and elite 2-opt of every instance) or one launch per pipeline stage covering every
+instance; whole runs of generations captured in one HIP graph, the form and launch shapes picked
+by a small autotuner.
 
 Each instance follows exactly the pipeline of ``DeviceMemeticLoop`` (the reference's
 Selection -> PMX -> Inversion -> Evaluation -> Survival -> EliteOnly 2-opt, genetic_algorithm.py
@@ -24,6 +26,7 @@ from .rng import draw_row_seeds, make_rng
 
 FIELDS = 9  # TSPMI_BATCH_FIELDS
 KP = 32  # packed candidate row width used by the batch kernels
+TASK_CHILD, TASK_SURVIVE, TASK_ELITE = 0, 1, 2
 # Instances at least this large compute their distances from coordinates in the elite search (and
 # every instance does in the generation and cost kernels).  With 256 instances the matrices (1.5 GB
 # at n <= 2000) thrash the caches while each instance's coordinates are at most 32 KB: measured on
@@ -101,6 +104,8 @@ class BatchedMemeticGA:
     self.w_dist = items(ns)
     self.w_row = items(np.full(B, P))
     self.w_elite = items(np.full(B, self.elite))
+    self.set_children_per_task(4)
+    self.counters = torch.zeros(4 + 3 * B, dtype=torch.int32, device="cuda")
     xy = np.concatenate([np.ascontiguousarray(c, dtype=np.float64).reshape(-1) for c in coords])
     self.xy = _cu(xy)
     total_pop = int((P * ns).sum())
@@ -114,9 +119,18 @@ class BatchedMemeticGA:
     self.order = torch.empty(self.rows, dtype=torch.int32, device=dev)
     self.stats = torch.zeros(3, dtype=torch.int64, device=dev)
     self.cur = 0
-    self.shapes = {"gen_block": 256, "gen_grid": 0, "ls_block": 1024, "ls_grid": 0}
+    self.shapes = {"fused": False, "fused_block": 256, "fused_grid": 0, "children_per_task": 4, "gen_block": 256,
+                   "gen_grid": 0, "ls_block": 1024, "ls_grid": 0}
     _native.batch_call("tspmi_batch_dist_euc2d", self.desc, *self.w_dist, self.xy, self.D)
     _native.batch_call("tspmi_batch_knn", self.desc, *self.w_dist, self.kc, self.D, self.cand, self.table)
+
+  def set_children_per_task(self, c: int) -> None:
+    """Children per child task of the fused generation (even): fewer, longer tasks publish less."""
+    if c < 2 or c % 2:
+      raise ValueError("children per task must be even and >= 2")
+    self.children_per_task = int(c)
+    self.tasks_host = fused_task_list(self.ns, self.P, self.elite, self.children_per_task)
+    self.tasks = _cu(self.tasks_host.view(np.int32))
 
   # ---- state ------------------------------------------------------------------------------------
   def init_random(self) -> None:
@@ -174,9 +188,28 @@ class BatchedMemeticGA:
     _native.batch_call("tspmi_batch_two_opt_packed_mixed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.xy,
                        self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
 
+  def _fused(self, a, b, d: BatchDraws, g: int, best, block: int, grid: int) -> None:
+    _native.batch_call("tspmi_batch_fused_generation", self.desc, self.B, self.tasks, int(self.tasks_host.size),
+                       self.counters, self.children_per_task, self.max_n, self.P, self.pop[a], self.keys[a], self.off,
+                       self.order, self.pop[b], self.keys[b], self.D, self.xy, self.table, d.tour_cand[g],
+                       self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g], self.rate, self.elite,
+                       self.kc, self.ls_iters, COORD_MIN_N, block, grid, self.stats, best, max(1, self.elite))
+
+  def verify(self) -> None:
+    """Raise if a dependency wait of a fused generation ran out (its results are then invalid)."""
+    if int(self.counters[2].item()) != 0:
+      raise RuntimeError("tspmi_batch_fused_generation: a dependency wait timed out; the batch state is invalid")
+
   def step(self, d: BatchDraws, g: int, best=None) -> None:
     a, b = self.cur, 1 - self.cur
     s = self.shapes
+    if s.get("fused"):
+      self._fused(a, b, d, g, best, s["fused_block"], s["fused_grid"])
+    else:
+      self._staged(a, b, d, g, best, s)
+    self.cur = b
+
+  def _staged(self, a, b, d: BatchDraws, g: int, best, s: dict) -> None:
     self._generation(a, d, g, s["gen_block"], s["gen_grid"])
     _native.batch_call("tspmi_batch_survival", self.desc, self.B, self.keys[a], self.order)
     _native.batch_call("tspmi_batch_gather", self.desc, *self.w_row, self.pop[a], self.keys[a], self.off, self.order,
@@ -186,7 +219,6 @@ class BatchedMemeticGA:
       _native.batch_call("tspmi_batch_cost", self.desc, *self.w_elite, self.pop[b], self.D, self.xy, self.keys[b])
     if best is not None:
       _native.batch_call("tspmi_batch_best", self.desc, self.B, self.keys[b], max(1, self.elite), best)
-    self.cur = b
 
   def run(self, d: BatchDraws, generations: int, best=None, start: int = 0) -> None:
     for t in range(generations):
@@ -216,8 +248,9 @@ class BatchedMemeticGA:
     if cache_path and os.path.exists(cache_path):
       with open(cache_path, encoding="utf-8") as fh:
         cached = json.load(fh).get(sig)
-      if cached:
+      if cached and "children_per_task" in cached:
         self.shapes = dict(cached)
+        self.set_children_per_task(self.shapes["children_per_task"])
         return self.shapes
     cus = torch.cuda.get_device_properties(0).multi_processor_count
 
@@ -235,6 +268,16 @@ class BatchedMemeticGA:
       return best
 
     a = self.cur
+    b = 1 - a
+    # Whole generations write only the other population buffer, the offspring buffers and the
+    # offspring half of the current keys, none of which is state, so both forms can be timed on
+    # the live batch without advancing it.
+    fused_times = {}
+    for cpt in (4, 8):
+      self.set_children_per_task(cpt)
+      for blk in (256, 512):
+        fused_times[(blk, cpt)] = timed(lambda blk=blk: self._fused(a, b, d, g, None, blk, 0))
+    self.verify()
     gen_opts = [(blk, grd) for blk in (128, 256, 512) for grd in (0, 2 * cus, 4 * cus)]
     gen_times = {o: timed(lambda o=o: self._generation(a, d, g, o[0], o[1])) for o in gen_opts}
     scratch = self.pop[a].clone()
@@ -247,9 +290,17 @@ class BatchedMemeticGA:
       ls_times[o] = timed(run_ls)
     gb = min(gen_times, key=gen_times.get)
     lb = min(ls_times, key=ls_times.get)
-    self.shapes = {"gen_block": gb[0], "gen_grid": gb[1], "ls_block": lb[0], "ls_grid": lb[1]}
+    fb = min(fused_times, key=fused_times.get)
+    self.set_children_per_task(fb[1])
+    shapes = {"fused": False, "fused_block": fb[0], "fused_grid": 0, "children_per_task": fb[1], "gen_block": gb[0],
+              "gen_grid": gb[1], "ls_block": lb[0], "ls_grid": lb[1]}
+    staged_ms = timed(lambda: self._staged(a, b, d, g, None, shapes))
+    shapes["fused"] = fused_times[fb] < staged_ms
+    self.shapes = shapes
     self.tuning = {"generation_ms": {f"{k[0]}x{k[1]}": v for k, v in gen_times.items()},
-                   "elite_2opt_ms": {f"{k[0]}x{k[1]}": v for k, v in ls_times.items()}}
+                   "elite_2opt_ms": {f"{k[0]}x{k[1]}": v for k, v in ls_times.items()},
+                   "fused_generation_ms": {f"{k[0]}x{k[1]}": v for k, v in fused_times.items()},
+                   "staged_generation_ms": staged_ms}
     if cache_path:
       data = {}
       if os.path.exists(cache_path):
@@ -259,6 +310,27 @@ class BatchedMemeticGA:
       with open(cache_path, "w", encoding="utf-8") as fh:
         json.dump(data, fh, indent=1)
     return self.shapes
+
+
+def fused_task_list(ns, P: int, elite: int, children_per_task: int) -> np.ndarray:
+  """Task words (kind << 30 | instance << 12 | index) of one fused generation, in dependency order:
+  every child task, then every survive task, then the elite rows; instances longest first within
+  each kind (tspmi_batch_fused_generation)."""
+  ns = np.asarray(ns)
+  if ns.size >= (1 << 18) or P > 4095:
+    raise ValueError("the fused generation supports fewer than 2^18 instances and P < 4096")
+  order = np.argsort(-ns, kind="stable").astype(np.uint32)
+  e = min(int(elite), P)
+
+  def kind(k, per):
+    inst = np.repeat(order, per)
+    idx = np.tile(np.arange(per, dtype=np.uint32), order.size)
+    return (np.uint32(k) << np.uint32(30)) | (inst << np.uint32(12)) | idx
+
+  parts = [kind(TASK_CHILD, -(-P // children_per_task)), kind(TASK_SURVIVE, 1)]
+  if e:
+    parts.append(kind(TASK_ELITE, e))
+  return np.concatenate(parts).astype(np.uint32)
 
 
 def c4_instances(count: int = 256, seed: int = 7, lo: int = 50, hi: int = 2000):

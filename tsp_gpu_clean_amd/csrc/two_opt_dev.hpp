@@ -95,6 +95,8 @@ inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ?
 // (delta, i*k + r) key restores the reference tie-break across lanes.
 //
 // `cost` (nullable) receives the closed cost of the final tour, the sum of its edge costs.
+// `src` (nullable) is where the start tour is read from; the refined tour is always written to
+// `tour` (so a survivor can be copied and refined in one step).
 // `pass_log` (nullable) receives {tour cost, wall clock} int64 pairs: entry 0 at the start, entry
 // m after the m-th move (m <= log_moves; later moves are not logged), the cost kept exactly as the
 // start cost plus the applied deltas and the clock read with wall_clock64() (the device's
@@ -103,7 +105,8 @@ template <int BLOCK, int KP, int UNROLL, int KL = KP, class Dist>
 __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                   int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
                                   int64_t* red64, double* __restrict__ cost = nullptr,
-                                  int64_t* __restrict__ pass_log = nullptr, int log_moves = 0) {
+                                  int64_t* __restrict__ pass_log = nullptr, int log_moves = 0,
+                                  const int32_t* __restrict__ src = nullptr) {
   static_assert(KL <= KP && KL <= kWave, "lanes per row exceed the table row");
   constexpr int NW = BLOCK / kWave;
   constexpr int RPW = kWave / KL;  // rows per wave step
@@ -115,7 +118,7 @@ __device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist 
   const int wid = wave_id();
   const int sub = lane / KL;       // row offset inside the wave step (RPW for a spare lane)
   const int r = lane - sub * KL;   // candidate rank handled by this lane
-  load_tour_lds<BLOCK>(tour, n, s_tour);
+  load_tour_lds<BLOCK>(src != nullptr ? src : tour, n, s_tour);
   __syncthreads();
   {
     // four positions per thread per round, their distance gathers issued together
