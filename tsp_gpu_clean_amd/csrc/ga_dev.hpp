@@ -67,7 +67,7 @@ __device__ __forceinline__ int wave_tournament(const int32_t* __restrict__ cand_
 // owner rewrites it is either the old or the new value, both points of the same path no further
 // than its exit, so the race never changes a result.  The placement -- the exit position of each
 // walk -- is the reference's.
-__device__ void pmx_child(const int32_t* p1, const int32_t* p2, const int32_t* inv1, const int32_t* inv2, int n,
+__device__ __forceinline__ void pmx_child(const int32_t* p1, const int32_t* p2, const int32_t* inv1, const int32_t* inv2, int n,
                           int c1, int c2, int32_t* child, int32_t* jump) {
   for (int x = threadIdx.x; x < n; x += blockDim.x) child[x] = (x >= c1 && x <= c2) ? p1[x] : -1;
   const int seg_len = c2 - c1 + 1;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void finish_child(int r, int32_t* ch, int n, const Di
 // the serial chain of a pair's workgroup (two PMX builds, costs and stores).
 // Child costs come from `dist` (a MatrixDist or, for EUC_2D instances, a CoordDist: same values).
 template <class Dist>
-__device__ void generation_child(int r, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P,
+__device__ __forceinline__ void generation_child(int r, const int32_t* __restrict__ pop, const double* __restrict__ fitness, int P,
                                  int n, const Dist dist, const int32_t* __restrict__ tour_cand,
                                  int k, const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
                                  const double* __restrict__ probs, const int32_t* __restrict__ pos1,
@@ -272,7 +272,7 @@ __device__ void generation_child(int r, const int32_t* __restrict__ pop, const d
 
 // Bitonic sort of (orderable bits, index) for M <= M2 <= kSortMax keys by one workgroup of kSortBlock
 // threads; order[r] = index of rank r for r < keep.
-__device__ void sort_select_body(const double* __restrict__ keys, int M, int M2, int keep, int32_t* __restrict__ order,
+__device__ __forceinline__ void sort_select_body(const double* __restrict__ keys, int M, int M2, int keep, int32_t* __restrict__ order,
                                  uint64_t* s_key, int32_t* s_idx) {
   for (int x = threadIdx.x; x < M2; x += blockDim.x) {
     s_key[x] = x < M ? orderable_bits(keys[x]) : ~0ull;

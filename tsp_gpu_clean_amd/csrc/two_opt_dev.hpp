@@ -1,5 +1,10 @@
 // Device building blocks of the 2-opt kernels, shared by the single-instance kernels (two_opt.hip)
 // and the multi-instance batch kernels (batch.hip).
+//
+// The per-tour bodies are always inlined: once a body has several callers in one source file the
+// compiler may emit it as an out-of-line call, and the call ABI then costs the kernel its
+// registers and occupancy (the batch elite kernel went from 87 VGPRs and 5 waves per SIMD to 152
+// and 3 that way).
 #pragma once
 
 #include "common.hpp"
@@ -102,7 +107,7 @@ inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ?
 // start cost plus the applied deltas and the clock read with wall_clock64() (the device's
 // constant-rate counter, the same on every CU).
 template <int BLOCK, int KP, int UNROLL, int KL = KP, class Dist>
-__device__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+__device__ __forceinline__ void cand_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                   int k, int max_iter, int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
                                   int64_t* red64, double* __restrict__ cost = nullptr,
                                   int64_t* __restrict__ pass_log = nullptr, int log_moves = 0,
@@ -303,7 +308,7 @@ __device__ __forceinline__ void refresh_edge_costs(int4* s_nd, const int32_t* s_
 constexpr int kRegJBits = 13;
 
 template <int BLOCK, int PP, class Dist>
-__device__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+__device__ __forceinline__ void reg_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                  int kp, int k, int max_iter, int64_t* __restrict__ stats, double* __restrict__ cost,
                                  int32_t* lds, uint64_t* red, int64_t* red64) {
   int4* s_nd = reinterpret_cast<int4*>(lds);  // n
@@ -453,7 +458,7 @@ __device__ __forceinline__ void store_relaxed(uint64_t* p, uint64_t v) {
 // written with agent-scope stores): the last tour to finish, counted in word 2 of tour 0's
 // control line (ws0), takes the minimum and clears the count.
 template <int BLOCK, int PP, class Dist>
-__device__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
+__device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                      int kp, int k, int max_iter, int G, int g, uint64_t* __restrict__ ws,
                                      int64_t* __restrict__ stats, double* __restrict__ cost, int32_t* lds,
                                      uint64_t* red, int64_t* red64, uint64_t* s_best, int P, uint64_t* ws0,
