@@ -39,6 +39,8 @@ PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 # (TA_TA_BUSY / TA_FLAT_READ_WAVEFRONTS of the scan, and the tspmi_probe_gather micro-benchmark).
 GATHER_LANES_PER_CLK_CU = 2.0
 GATHER_PEAK = GATHER_LANES_PER_CLK_CU * CUS * PEAK_CLOCK_HZ  # lanes/s
+# VALU issue of the headline launch by its counters (SQ_INSTS_VALU x 4 cycles over 1024 SIMDs).
+VALU_BUSY_BY_COUNTERS = 0.94
 N_CITIES = 1000
 POP = 1024
 
@@ -56,7 +58,7 @@ def c2_inputs(torch, native):
   return xy, D, tours
 
 
-MIX_PERIOD, MIX_ROWS = 8, 0b00100101  # rows 0, 2, 5 of each 8 from the coordinates
+MIX_PERIOD, MIX_ROWS = 16, 0b0101101010110101  # 9 rows of each 16 from the coordinates
 
 
 def diag_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4, mix: bool = False) -> tuple[int, int]:
@@ -64,8 +66,8 @@ def diag_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4, mix: bool = False
   over one tour (256 threads): the carry set-up (two gathers per slot lane), row 0 over every slot,
   then per wave the slot phases -- slot q of wave w runs while i < (n-2) - off(w, q), with
   off(w, q) = 64 (4q + w) on even q and 64 (4q + 3 - w) on odd q.  With `mix`, each phase takes its
-  rows in periods of 8 whose rows 0, 2 and 5 compute the distances from LDS coordinates, and the
-  phase's last (< 8) rows gather."""
+  rows in periods of 16 whose rows 0, 2, 4, 5, 7, 9, 11, 12 and 14 compute the distances from LDS
+  coordinates, and the phase's last (< 16) rows gather."""
   rows = n - 2
   gather_rows = coord_rows = 0
   for w in range(waves):
@@ -587,15 +589,19 @@ def main() -> int:
       "bytes_per_unit": BYTES_PER_DELTA, "units_per_launch": deltas,
     },
     "roofline_onchip": {
-      "bound": "texture-address gather issue (TA) on the gathered rows", "achieved": lanes / (kms * 1e-3),
+      "bound": "VALU issue and texture-address gather issue (TA) together: coordinate rows load the VALU, "
+               "gathered rows the TA, and the row split balances the two",
+      "valu_busy_by_counters": VALU_BUSY_BY_COUNTERS,
+      "valu_busy_source": "stored: rocprofv3 SQ_INSTS_VALU of this launch, profiles/two_opt_c2_counters.md",
+      "achieved": lanes / (kms * 1e-3),
       "peak": GATHER_PEAK, "unit": "gather-lanes/s", "frac": lanes / (kms * 1e-3) / GATHER_PEAK,
       "peak_basis": f"{GATHER_LANES_PER_CLK_CU:g} random 4-byte lanes per clock per CU (a wave64 gather at random "
                     f"offsets of one 4 KB row occupies the TA 32 cycles) x {CUS} CUs x {PEAK_CLOCK_HZ / 1e9:g} GHz",
       "lanes_per_launch": lanes, "lanes_per_pass_per_tour": glanes,
       "coordinate_lanes_per_launch": coord_lanes, "coordinate_lanes_per_pass_per_tour": clanes,
       "passes_per_launch": passes, "probe": probe,
-      "note": "3 rows in 8 take their distances from LDS coordinates (FP64 VALU) and issue no gather; the TA "
-              "fraction counts only the gathered rows",
+      "note": "9 rows in 16 take their distances from LDS coordinates (integer form for integer coordinates, "
+              "VALU) and issue no gather; the TA fraction counts only the gathered rows",
       "counters": "profiles/two_opt_c2_counters.md (TA busy fraction, L1 hit rate, VALU and LDS issue)",
     },
     "matrix_only": {

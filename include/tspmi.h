@@ -58,6 +58,16 @@ int tspmi_probe_gather(const int32_t* rows, int nrows, int blocks, int iters, in
 int tspmi_dist_euc2d(const double* xy, int n, int32_t* D, int ld, int32_t* max_out, void* stream);
 
 /*
+ * EUC_2D matrix from int32 coordinates (n, 2) by the integer form of the same distance that the
+ * full-neighbourhood scan uses for integer instances: equal to tspmi_dist_euc2d on the same
+ * (integer-valued) coordinates whenever every pair differs by at most 2^21 per axis.  A pair
+ * beyond that gets -1 and is counted in *bad_out (device int32, may be NULL).  xy 8-byte aligned.
+ * No reference counterpart: it exposes the integer form so that it can be checked entry by entry
+ * against the FP64 one (src/distances/matrix.py:84-86).
+ */
+int tspmi_dist_euc2d_i32(const int32_t* xy, int n, int32_t* D, int ld, int32_t* bad_out, void* stream);
+
+/*
  * Distance matrix of another TSPLIB 2D metric: metric 0 EUC_2D, 1 CEIL_2D, 2 MAN_2D, 3 MAX_2D,
  * 4 ATT, 5 GEO (xy as DDD.MM latitude/longitude), each with the operations of the NumPy
  * expression it replaces; diagonal 0; max_out as for tspmi_dist_euc2d.  n <= 65535.
@@ -114,9 +124,11 @@ int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const 
 
 /*
  * tspmi_two_opt(mode = BEST, no candidates) for EUC_2D instances with n <= 2050: D must be the
- * matrix tspmi_dist_euc2d built from xy (float64 (n, 2)).  Three rows in eight of the scan compute
- * their distances from the coordinates (euc2d_nint, the matrix's own function), the others gather
- * them from D; the values are the same, so are the tours, moves and counters.
+ * matrix tspmi_dist_euc2d built from xy (float64 (n, 2), 16-byte aligned).  Nine rows in sixteen
+ * of the scan compute their distances from the coordinates, the others gather them from D: by
+ * the integer form of the distance when every coordinate is an integer and each axis spans at
+ * most 2^20 (the kernel checks), else by the matrix's own FP64 function.  The values are the
+ * same, so are the tours, moves and counters.
  * Replaces: two_opt.py:831-899 (full neighbourhood, best) on EUC_2D problems.
  */
 int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy, int max_iter,

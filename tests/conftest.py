@@ -68,3 +68,37 @@ def nearest(dist: np.ndarray, k: int) -> np.ndarray:
   n = dist.shape[0]
   k_eff = min(k, n - 1)
   return np.asarray(np.argsort(dist, axis=1, kind="stable")[:, 1 : k_eff + 1], dtype=np.int32)
+
+
+INT_SPAN = 1 << 20  # per-axis span up to which the integer EUC_2D form is exact
+
+
+def nint_boundary_offsets() -> np.ndarray:
+  """Integer offsets (dx, dy) whose squared length s sits at an nint boundary or next to it:
+  (t^2 - c, t) gives s = m^2 + m + c with m = t^2 - c, the last s whose nint is m (c = 0), the
+  first whose nint is m + 1 (c = 1) and their neighbours (c = -1, 2); t up to the 2^20 span, in
+  all four sign quadrants and with the axes swapped."""
+  ts = sorted({int(v) for v in np.geomspace(1, 1024, 40)} | {1, 2, 3, 1000, 1023, 1024})
+  offs = []
+  for t in ts:
+    for c in (-1, 0, 1, 2):
+      dx = t * t - c
+      if dx < 0 or dx > INT_SPAN:
+        continue
+      for sx in (1, -1):
+        for sy in (1, -1):
+          offs += [(sx * dx, sy * t), (sy * t, sx * dx)]
+  return np.array(offs, dtype=np.int64)
+
+
+def span_limit_int_coords(n: int, seed: int) -> np.ndarray:
+  """n integer points whose per-axis span is exactly 2^20: the corner (-2^19, -2^19), points at
+  the nint-boundary offsets from it (folded into the box), the opposite corner, and uniform fill."""
+  half = INT_SPAN // 2
+  base = np.array([-half, -half], dtype=np.int64)
+  offs = np.abs(nint_boundary_offsets())
+  pts = [base, base + INT_SPAN] + [base + o for o in offs]
+  rng = np.random.default_rng(seed)
+  if len(pts) < n:
+    pts += list(rng.integers(-half, half + 1, (n - len(pts), 2)))
+  return np.array(pts[:n], dtype=np.float64)

@@ -13,7 +13,7 @@ from oracle import init as oinit
 from oracle import operators as ops
 from oracle import two_opt as o2
 from oracle.distances import candidate_boundary_ties, compute_candidate_list, compute_distance_matrix
-from tests.conftest import lattice_matrix, nearest, rounded_matrix
+from tests.conftest import INT_SPAN, lattice_matrix, nearest, nint_boundary_offsets, rounded_matrix, span_limit_int_coords
 
 pytestmark = pytest.mark.gpu
 
@@ -102,6 +102,40 @@ def test_dist_euc2d_matches_oracle(torch_mod, nat, berlin52_coords, kind):
   want = compute_distance_matrix(xy)
   np.testing.assert_array_equal(D.cpu().numpy(), want)
   assert mx == int(want.max())
+
+
+@pytest.mark.parametrize("kind", ["uniform1000", "span_limit", "offset", "boundaries"])
+def test_dist_euc2d_i32_matches_oracle(torch_mod, nat, kind):
+  """The integer EUC_2D form equals the oracle's FP64 nint entry for entry wherever every pair is
+  within the 2^21 span: uniform integers, a box of exactly that span with the nint-boundary pairs,
+  large negative coordinates, and the boundary offsets in all quadrants around one centre."""
+  say(f"dist_euc2d_i32 {kind}")
+  near = nint_boundary_offsets()
+  near = near[np.abs(near).max(axis=1) <= INT_SPAN // 2]  # every pair of them within the span
+  xy = {
+    "uniform1000": lambda: uniform_coords(1000),
+    "span_limit": lambda: span_limit_int_coords(1500, seed=1),
+    "offset": lambda: uniform_coords(800, seed=4) - float(2**30),
+    "boundaries": lambda: np.concatenate([[[0, 0]], near]).astype(np.float64),
+  }[kind]()
+  n = xy.shape[0]
+  D = torch_mod.empty((n, n), dtype=torch_mod.int32, device="cuda")
+  bad = torch_mod.zeros(1, dtype=torch_mod.int32, device="cuda")
+  nat.dist_euc2d_i32(cu(xy.astype(np.int32)), D, bad)
+  np.testing.assert_array_equal(D.cpu().numpy(), compute_distance_matrix(xy))
+  assert int(bad.item()) == 0
+
+
+def test_dist_euc2d_i32_flags_pairs_beyond_the_span(torch_mod, nat):
+  say("dist_euc2d_i32 beyond the span")
+  xy = np.array([[0, 0], [INT_SPAN, 3], [INT_SPAN + 1, 0], [5, -INT_SPAN - 2]], dtype=np.int32)
+  D = torch_mod.empty((4, 4), dtype=torch_mod.int32, device="cuda")
+  bad = torch_mod.zeros(1, dtype=torch_mod.int32, device="cuda")
+  nat.dist_euc2d_i32(cu(xy), D, bad)
+  got, want = D.cpu().numpy(), compute_distance_matrix(xy.astype(np.float64))
+  far = np.abs(xy[:, None, :].astype(np.int64) - xy[None, :, :]).max(axis=2) > INT_SPAN
+  assert (got[far] == -1).all() and np.array_equal(got[~far], want[~far])
+  assert int(bad.item()) == int(far.sum())
 
 
 @pytest.mark.parametrize("n,k", [(1000, 20), (1000, 32), (52, 5), (7, 20), (2, 3)])
@@ -774,15 +808,25 @@ def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iter
 @pytest.mark.parametrize("n,P,iters", [(4, 3, -1), (5, 6, -1), (10, 6, -1), (63, 4, -1), (66, 4, -1), (257, 3, -1),
                                        (258, 3, 60), (700, 3, 40), (1025, 3, 30), (1026, 3, 30), (1283, 2, 20),
                                        (2050, 2, 12)])
-@pytest.mark.parametrize("kind", ["uniform", "float", "lattice"])
+@pytest.mark.parametrize("kind", ["uniform", "float", "lattice", "span_limit", "beyond_span"])
 def test_two_opt_full_euc2d_matches_c_oracle(torch_mod, nat, n, P, iters, kind):
   """Full-neighbourhood steepest scan with rows from the coordinates, around the slot-phase and
-  8-row mix-period boundaries, on integer, fractional and tie-heavy lattice instances: tours and
-  counters equal the C oracle's on the matrix of the same coordinates."""
+  8-row mix-period boundaries, on integer (the integer distance form), fractional (the FP64
+  form) and tie-heavy lattice instances, on integers spanning exactly 2^21 with nint-boundary
+  pairs, and on integers one past that span (FP64 form again): tours and counters equal the C
+  oracle's on the matrix of the same coordinates."""
   say(f"two_opt_full_euc2d n={n} P={P} iters={iters} {kind}")
+
+  def beyond():
+    xy = uniform_coords(n, seed=n + 13, hi=INT_SPAN + 1)
+    xy[0], xy[-1] = (0.0, 0.0), (float(INT_SPAN + 1), float(INT_SPAN + 1))
+    return xy
+
   xy = {"uniform": lambda: uniform_coords(n, seed=n + 11),
         "float": lambda: np.random.default_rng(n).random((n, 2)) * 1000.0,
-        "lattice": lambda: lattice_coords(n)}[kind]()
+        "lattice": lambda: lattice_coords(n),
+        "span_limit": lambda: span_limit_int_coords(n, seed=n),
+        "beyond_span": beyond}[kind]()
   d = compute_distance_matrix(xy)
   D, _ = device_dist(nat, torch_mod, xy)
   np.testing.assert_array_equal(D.cpu().numpy(), d)

@@ -105,3 +105,40 @@ def test_burma14_geo_distances():
   d = compute_distance_matrix(prob.coordinates, "GEO")
   assert d[0, :3].tolist() == [0, 153, 510] and d[1, 2] == 422
   assert np.array_equal(d, d.T) and (np.diag(d) == 0).all()
+
+
+def test_integer_nint_form_equals_fp64_nint():
+  """The integer form of the EUC_2D distance used by the full-neighbourhood scan for integer
+  instances (m = trunc of an FP32 sqrt(dx^2 + dy^2), then m + [t < 0] with t = m^2 + m - s
+  formed modulo 2^32 from the 24-bit products) returns the oracle's FP64 nint for both estimates
+  it can meet, nint - 1 and nint, and moves the wrong one (nint + 1 would stay: the estimate
+  must never reach it); an FP32 estimate is within those two, with room to spare (the sqrt is
+  never further than 0.3 from exact): on the nint-boundary offsets (both sides of each
+  boundary) and random offsets up to the 2^20 span."""
+  from tests.conftest import INT_SPAN, nint_boundary_offsets
+
+  rng = np.random.default_rng(5)
+  offs = np.concatenate([nint_boundary_offsets(), rng.integers(-INT_SPAN, INT_SPAN + 1, (4000, 2)),
+                         [[INT_SPAN, INT_SPAN], [INT_SPAN, 0], [0, 0], [1, 0], [1, 1]]])
+  pts = np.concatenate([[[0, 0]], offs]).astype(np.float64)
+  want = compute_distance_matrix(pts)[0, 1:].astype(np.int64)
+  dx, dy = offs[:, 0], offs[:, 1]
+  s32 = (dx * dx + dy * dy) & 0xFFFFFFFF
+
+  def correct(m, s=s32):
+    t = ((m * m + m - s) & 0xFFFFFFFF).astype(np.int64)
+    t = np.where(t >= 1 << 31, t - (1 << 32), t)
+    return m + (t < 0)
+
+  np.testing.assert_array_equal(correct(want), want)
+  low = want > 0
+  np.testing.assert_array_equal(correct(want[low] - 1, s32[low]), want[low])
+  assert (correct(want + 1) == want + 1).all()  # an overshoot is not repaired
+  fx, fy = dx.astype(np.float32), dy.astype(np.float32)
+  r = np.sqrt(fx * fx + fy * fy).astype(np.float64)
+  exact = np.sqrt((dx * dx + dy * dy).astype(np.float64))
+  assert np.abs(r - exact).max() < 0.3
+  est = r.astype(np.int64)
+  assert ((est == want) | (est == want - 1)).all()
+  np.testing.assert_array_equal(correct(est), want)
+  assert (est != want).any()  # the correction does work on these pairs

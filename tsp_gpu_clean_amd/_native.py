@@ -33,6 +33,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_device_count": [_P],
   "tspmi_probe_gather": [_P, _I, _I, _I, _P, _P],
   "tspmi_dist_euc2d": [_P, _I, _P, _I, _P, _P],
+  "tspmi_dist_euc2d_i32": [_P, _I, _P, _I, _P, _P],
   "tspmi_knn": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_knn_flagged": [_P, _I, _I, _I, _P, _P, _P],
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
@@ -181,6 +182,21 @@ def dist_euc2d(xy, D, max_out=None) -> None:
   if max_out is not None:
     _need(max_out, torch.int32, "max_out")
   _check(_lib.tspmi_dist_euc2d(_ptr(xy), n, _ptr(D), D.shape[1], _ptr(max_out), _stream(torch)), "tspmi_dist_euc2d")
+
+
+def dist_euc2d_i32(xy, D, bad_out=None) -> None:
+  """EUC_2D matrix from int32 coordinates by the integer distance form; pairs more than 2^21 apart
+  on an axis get -1 and are counted in ``bad_out`` (int32 (1,))."""
+  torch = require_device()
+  _need(xy, torch.int32, "xy", 2)
+  _need(D, torch.int32, "D", 2)
+  n = xy.shape[0]
+  if xy.shape[1] != 2 or D.shape[0] < n or D.shape[1] < n:
+    raise ValueError(f"dist_euc2d_i32: xy {tuple(xy.shape)} vs D {tuple(D.shape)}")
+  if bad_out is not None:
+    _need(bad_out, torch.int32, "bad_out")
+  _check(_lib.tspmi_dist_euc2d_i32(_ptr(xy), n, _ptr(D), D.shape[1], _ptr(bad_out), _stream(torch)),
+         "tspmi_dist_euc2d_i32")
 
 
 def probe_gather(rows, blocks: int, iters: int, out) -> None:

@@ -32,6 +32,21 @@ __global__ __launch_bounds__(kDistTile) void dist_euc2d_kernel(const double* __r
   }
 }
 
+// EUC_2D matrix from int32 coordinates by the integer distance form (euc2d_nint_i32); a pair
+// whose per-axis difference exceeds kIntCoordSpan gets -1 and is counted in *bad_out.
+__global__ __launch_bounds__(kDistTile) void dist_euc2d_i32_kernel(const int2* __restrict__ xy, int n,
+                                                                   int32_t* __restrict__ D, int ld,
+                                                                   int32_t* __restrict__ bad_out) {
+  const int u = blockIdx.y;
+  const int v = blockIdx.x * kDistTile + threadIdx.x;
+  if (v >= n) return;
+  const int2 a = xy[u], b = xy[v];
+  const long dx = (long)a.x - b.x, dy = (long)a.y - b.y;
+  const bool fits = dx >= -kIntCoordSpan && dx <= kIntCoordSpan && dy >= -kIntCoordSpan && dy <= kIntCoordSpan;
+  D[(size_t)u * ld + v] = (u == v) ? 0 : (fits ? euc2d_nint_i32(a.x, a.y, b.x, b.y) : -1);
+  if (!fits && bad_out != nullptr) atomicAdd(bad_out, 1);
+}
+
 // Other TSPLIB 2D metrics (src/distances/matrix.py:77-131, pairwise.py:62-102), each evaluated
 // with the operations and order of the NumPy expression (no FMA contraction).
 enum Metric : int { kEuc2d = 0, kCeil2d = 1, kMan2d = 2, kMax2d = 3, kAtt = 4, kGeo = 5 };
@@ -163,6 +178,19 @@ extern "C" int tspmi_dist_euc2d(const double* xy, int n, int32_t* D, int ld, int
   dim3 grid((n + kDistTile - 1) / kDistTile, n);
   hipLaunchKernelGGL(dist_euc2d_kernel, grid, dim3(kDistTile), 0, (hipStream_t)stream, xy, n, D, ld, max_out);
   TSPMI_CHECK_LAUNCH("tspmi_dist_euc2d launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_dist_euc2d_i32(const int32_t* xy, int n, int32_t* D, int ld, int32_t* bad_out, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(n >= 0 && ld >= n && n <= 65535, "tspmi_dist_euc2d_i32: bad shape n=%d ld=%d (n <= 65535)", n, ld);
+  if (n == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(xy != nullptr && D != nullptr, "tspmi_dist_euc2d_i32: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy & 7) == 0, "tspmi_dist_euc2d_i32: xy must be 8-byte aligned");
+  dim3 grid((n + kDistTile - 1) / kDistTile, n);
+  hipLaunchKernelGGL(dist_euc2d_i32_kernel, grid, dim3(kDistTile), 0, (hipStream_t)stream,
+                     reinterpret_cast<const int2*>(xy), n, D, ld, bad_out);
+  TSPMI_CHECK_LAUNCH("tspmi_dist_euc2d_i32 launch");
   return TSPMI_OK;
 }
 

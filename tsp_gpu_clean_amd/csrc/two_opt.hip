@@ -46,25 +46,50 @@ namespace tspmi {
 // Each lane visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per
 // lane and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
 //
-// With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), rows 0, 2 and 5 of every 8
-// compute their new distances from the coordinates with euc2d_nint, the function that built the
-// matrix, instead of gathering them: the carried values are the same either way, and those rows
-// put no work on the texture addresser, whose 64-lane random gathers (32 cycles each) bind the
-// matrix-only scan, while the FP64 arithmetic runs on otherwise idle VALU cycles.  3 rows in 8
-// was the best mix measured at n = 1000 (430 ms per C2 launch, against 574 ms without it and
-// 440 / 472 / 500 ms for 1 row in 3 / 4 / 2).
-constexpr int kMixPeriod = 8;
-constexpr int kMixRows = 0b00100101;
+// With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), rows 0, 2, 4, 5, 7, 9, 11, 12
+// and 14 of every 16 compute their new distances from the coordinates, with the function that
+// built the matrix or its integer form (same values), instead of gathering them: the carried
+// values are the same either way, and those rows put no work on the texture addresser, whose
+// 64-lane random gathers (32 cycles each) bind the matrix-only scan, while the distance
+// arithmetic runs on otherwise idle VALU cycles.  Measured per n = 1000, P = 1024 launch to the
+// local optimum (585 ms with no coordinate rows): FP64 form, 3 rows in 8: 430-440 ms; integer
+// form (14 instructions), k rows in 8 for k = 3..6: 391 / 320 / 313 / 342 ms, 7 and 9 rows in 16:
+// 342 / 308 ms; with a 19-instruction integer form, every other row with a period of 2 / 4 / 8 /
+// 16 rows: 378 / 371 / 365 / 363 ms (the longer unrolled period hides more latency).
+constexpr int kMixPeriod = 16;
+constexpr int kMixRows = 0b0101101010110101;
+
+// Coordinate forms of the MIX rows: FP64 coordinates indexed by node with the FP64 distance, or,
+// when every coordinate is an integer and each axis spans at most kIntCoordSpan, int32 coordinates
+// kept in tour order (entry x holds the coordinates of the node at position x, so a lane's read is
+// contiguous with its neighbours' and needs no lookup through the tour) with the integer form of
+// the same distance (equal values, fewer VALU cycles, a third of the LDS traffic).
+struct XyF64 {
+  using T = double2;
+  static constexpr bool kTourOrder = false;
+  static __device__ __forceinline__ int dist(T a, T b) { return euc2d_nint(a.x, a.y, b.x, b.y); }
+};
+struct XyI32 {
+  using T = int2;
+  static constexpr bool kTourOrder = true;
+  static __device__ __forceinline__ int dist(T a, T b) { return euc2d_nint_i32(a.x, a.y, b.x, b.y); }
+};
+
+// int32 words of LDS before the tour array in two_opt_mix_kernel: n node-indexed double2, or
+// n + 1 + pad tour-ordered int2 (the tour's own length with its mirror entry and padding).
+__host__ __device__ constexpr int mix_coord_words(int n, int pad) {
+  return ((4 * n > 2 * (n + 1 + pad) ? 4 * n : 2 * (n + 1 + pad)) + 3) & ~3;
+}
 
 __device__ __forceinline__ int diag_slot_offset(int w, int q) {
   return kWave * (kTwoOptWaves * q + ((q & 1) ? (kTwoOptWaves - 1 - w) : w));
 }
 
-template <int QMAX, bool MIX = false>
+template <int QMAX, bool MIX = false, class XY = XyF64>
 struct DiagScan {
   const int32_t* s_tour;
   const int32_t* s_edge;
-  const double2* s_xy;  // MIX only
+  const typename XY::T* s_xy;  // MIX only
   const int32_t* __restrict__ D;
   int ld, n, wid;
   int jo_even, jo_odd;  // j - i of the lane in even / odd slots, without the 256 q term
@@ -76,16 +101,17 @@ struct DiagScan {
   // takes the row's new distances from the coordinates.
   template <int QL, bool FIRST, bool COORD = false>
   __device__ __forceinline__ void row(int i) {
-    const int a = __builtin_amdgcn_readfirstlane(s_tour[i + 2]);
     int gnext[QL];
-    if constexpr (COORD) {
-      const double2 pa = s_xy[a];
+    if constexpr (COORD && XY::kTourOrder) {
+      const typename XY::T pa = s_xy[i + 2];
 #pragma unroll
-      for (int q = 0; q < QL; ++q) {
-        const double2 pb = s_xy[s_tour[i + jo(q) + 2]];
-        gnext[q] = euc2d_nint(pa.x, pa.y, pb.x, pb.y);
-      }
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[i + jo(q) + 2]);
+    } else if constexpr (COORD) {
+      const typename XY::T pa = s_xy[__builtin_amdgcn_readfirstlane(s_tour[i + 2])];
+#pragma unroll
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[s_tour[i + jo(q) + 2]]);
     } else {
+      const int a = __builtin_amdgcn_readfirstlane(s_tour[i + 2]);
       const int32_t* __restrict__ rn = D + (size_t)a * ld;
 #pragma unroll
       for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, s_tour[i + jo(q) + 2]);
@@ -130,11 +156,11 @@ struct DiagScan {
   }
 };
 
-template <int QMAX, bool MIX = false>
+template <int QMAX, bool MIX = false, class XY = XyF64>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane,
-                                                        const double2* s_xy = nullptr) {
-  DiagScan<QMAX, MIX> sc;
+                                                        const typename XY::T* s_xy = nullptr) {
+  DiagScan<QMAX, MIX, XY> sc;
   sc.s_tour = s_tour;
   sc.s_edge = s_edge;
   sc.s_xy = s_xy;
@@ -563,9 +589,95 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
   }
 }
 
+// Whether every coordinate is an integer (|v| <= 2^30) and each axis spans at most kIntCoordSpan,
+// so that the MIX rows may use the integer distance form; the same answer in every thread.
+__device__ __forceinline__ bool int_coords_fit(const double2* __restrict__ xy, int n, int* s_box) {
+  const int tid = threadIdx.x;
+  if (tid < 4) s_box[tid] = 0x7fffffff;
+  __syncthreads();
+  int ok = 1, lo_x = 0x7fffffff, lo_y = 0x7fffffff, hi_x = 0x7fffffff, hi_y = 0x7fffffff;  // hi as -max
+  for (int x = tid; x < n; x += blockDim.x) {
+    const double2 p = xy[x];
+    const bool fits = p.x == __builtin_rint(p.x) && p.y == __builtin_rint(p.y) && __builtin_fabs(p.x) <= 1073741824.0 &&
+                      __builtin_fabs(p.y) <= 1073741824.0;
+    ok &= fits ? 1 : 0;
+    if (fits) {
+      const int ix = (int)p.x, iy = (int)p.y;
+      lo_x = min(lo_x, ix);
+      lo_y = min(lo_y, iy);
+      hi_x = min(hi_x, -ix);
+      hi_y = min(hi_y, -iy);
+    }
+  }
+  atomicMin(&s_box[0], lo_x);
+  atomicMin(&s_box[1], hi_x);
+  atomicMin(&s_box[2], lo_y);
+  atomicMin(&s_box[3], hi_y);
+  ok = __syncthreads_and(ok);
+  // -s_box[1] - s_box[0] is the x span (both bounds exist once n >= 1)
+  return ok && (long)-s_box[1] - s_box[0] <= kIntCoordSpan && (long)-s_box[3] - s_box[2] <= kIntCoordSpan;
+}
+
+// Reverse positions i+1 .. j of the tour and of its tour-ordered coordinates, then refresh the
+// touched edge costs from the coordinates (s_xy[n] mirrors s_xy[0], so no wrap test).
+template <class XY>
+__device__ __forceinline__ void apply_move_tour_xy(int32_t* s_tour, typename XY::T* s_xy, int32_t* s_edge, int i,
+                                                   int j) {
+  const int half = (j - i) >> 1;
+  for (int t = threadIdx.x; t < half; t += blockDim.x) {
+    const int lo = i + 1 + t, hi = j - t;
+    const int32_t tmp = s_tour[lo];
+    s_tour[lo] = s_tour[hi];
+    s_tour[hi] = tmp;
+    const typename XY::T c = s_xy[lo];
+    s_xy[lo] = s_xy[hi];
+    s_xy[hi] = c;
+  }
+  __syncthreads();
+  for (int x = i + threadIdx.x; x <= j; x += blockDim.x) s_edge[x] = XY::dist(s_xy[x], s_xy[x + 1]);
+  __syncthreads();
+}
+
+// The pass loop of two_opt_mix_kernel with the coordinates staged in LDS in the form XY (s_tour
+// and its padding already in place).
+template <int QMAX, class XY>
+__device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typename XY::T* s_xy, int32_t* s_tour,
+                                           int32_t* s_edge, int n, int pad, const int32_t* __restrict__ D, int ld,
+                                           int max_iter, uint64_t* red, int& moves, int& passes) {
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = wave_id();
+  if constexpr (XY::kTourOrder) {
+    for (int x = tid; x < n + 1 + pad; x += kTwoOptBlock) {
+      const double2 p = xy[s_tour[x]];  // the mirror entry and the padding (node 0) included
+      s_xy[x] = make_int2((int)p.x, (int)p.y);
+    }
+  } else {
+    for (int x = tid; x < n; x += kTwoOptBlock) s_xy[x] = xy[x];
+  }
+  __syncthreads();
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, s_xy);
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)n);
+    const int j = (int)(idx - (uint32_t)i * (uint32_t)n);
+    __syncthreads();
+    if constexpr (XY::kTourOrder)
+      apply_move_tour_xy<XY>(s_tour, s_xy, s_edge, i, j);
+    else
+      apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
+    ++moves;
+  }
+}
+
 // Full neighbourhood, best improvement, for EUC_2D instances: the diagonal scan with its MIX rows
-// from the coordinates, staged in LDS next to the tour (16 n bytes first, then the tour and edge
+// from the coordinates, staged in LDS before the tour (mix_coord_words, then the tour and edge
 // arrays with their padding).  Same pairs, deltas, tie-break, moves and counters as two_opt_kernel.
+// Integer coordinates within kIntCoordSpan per axis are staged as int32 pairs in tour order and
+// take the integer distance form (the same values); others stay FP64, indexed by node.
 template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __restrict__ tours, int n,
                                                                    const int32_t* __restrict__ D, int ld,
@@ -573,15 +685,13 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
                                                                    int pad, int64_t* __restrict__ stats) {
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[kTwoOptWaves];
-  double2* s_xy = reinterpret_cast<double2*>(lds);  // n
-  int32_t* s_tour = lds + 4 * n;                    // n + 1 + pad
+  __shared__ int s_box[4];
+  int32_t* s_tour = lds + mix_coord_words(n, pad);  // n + 1 + pad, after the coordinates
   int32_t* s_edge = s_tour + n + 1 + pad;           // n + pad
   const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int wid = wave_id();
   int32_t* tour = tours + (size_t)blockIdx.x * n;
+  const bool int_xy = int_coords_fit(xy, n, s_box);
   for (int x = tid; x < n; x += kTwoOptBlock) {
-    s_xy[x] = xy[x];
     const int v = clamp_id(tour[x], n);
     s_tour[x] = v;
     if (x == 0) s_tour[n] = v;
@@ -592,20 +702,13 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
   }
   __syncthreads();
   for (int x = tid; x < n; x += kTwoOptBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
-  __syncthreads();
   int moves = 0, passes = 0;
-  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
-    ++passes;
-    uint64_t best = scan_full_best_diag<QMAX, true>(s_tour, s_edge, n, D, ld, wid, lane, s_xy);
-    best = block_min_u64(best, red);
-    if (best == kNoMove) break;
-    const uint32_t idx = key_index(best);
-    const int i = (int)(idx / (uint32_t)n);
-    const int j = (int)(idx - (uint32_t)i * (uint32_t)n);
-    __syncthreads();
-    apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
-    ++moves;
-  }
+  if (int_xy)
+    mix_passes<QMAX, XyI32>(xy, reinterpret_cast<int2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
+                            passes);
+  else
+    mix_passes<QMAX, XyF64>(xy, reinterpret_cast<double2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
+                            passes);
   for (int x = tid; x < n; x += kTwoOptBlock) tour[x] = s_tour[x];
   if (stats != nullptr && tid == 0) {
     atomicAdd((unsigned long long*)&stats[0], (unsigned long long)((int64_t)passes * n * (n - 3) / 2));
@@ -937,7 +1040,7 @@ extern "C" int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int3
   TSPMI_CHECK_ARG(n - 2 <= kTwoOptWaves * 8 * kWave, "tspmi_two_opt_full_euc2d: n=%d above 2050", n);
   const int q = n - 2 <= kTwoOptWaves * 4 * kWave ? 4 : 8;
   const int pad = q * kTwoOptBlock + kWave;
-  const size_t lds = ((size_t)4 * n + 2 * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
+  const size_t lds = ((size_t)mix_coord_words(n, pad) + 2 * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_full_euc2d: n=%d needs %zu B of LDS", n, lds);
   hipStream_t s = (hipStream_t)stream;
   const double2* p = reinterpret_cast<const double2*>(xy);
