@@ -613,15 +613,21 @@ def main() -> int:
   }
   if not args.no_extras:
     print("bench: candidate_k=20", file=sys.stderr, flush=True)
-    from tsp_gpu_clean_amd.context import coordinate_candidates_pay
+    from tsp_gpu_clean_amd.context import coordinate_candidates_pay, int_coordinates
 
     cand = torch.empty((N_CITIES, 20), dtype=torch.int32, device="cuda")
     _native.knn(D, 20, cand)
-    # the distance source the local-search strategy picks for this shape (coordinates here)
-    cxy = xy_dev if coordinate_candidates_pay(N_CITIES, POP) else None
+    # the distance source the local-search strategy picks for this shape: coordinates, as int32 for
+    # the integer distance form (integer coordinates within the 2^20 span)
+    cxy = None
+    if coordinate_candidates_pay(N_CITIES, POP):
+      ixy = int_coordinates(xy)
+      cxy = torch.from_numpy(ixy).cuda() if ixy is not None else xy_dev
     s20, d20, _, k20ms, _, _ = time_two_opt(torch, _native, D, start, cand, max(1, args.steps), 1, cxy)
     line["candidate_k20"] = {"deltas_per_s": d20 / s20, "ms_per_step": s20 * 1e3, "kernel_ms": k20ms,
-                             "deltas_per_step": d20, "distances": "coordinates" if cxy is not None else "matrix"}
+                             "deltas_per_step": d20,
+                             "distances": "matrix" if cxy is None else
+                                          ("coordinates, integer form" if cxy.dtype == torch.int32 else "coordinates")}
     print(f"bench: GA generations ({args.generations})", file=sys.stderr, flush=True)
     gps, gwall, gbest, gdel = time_generations(torch, _native, args.generations)
     line["gens_per_s"] = gps

@@ -229,6 +229,19 @@ int tspmi_two_opt_packed_euc2d_log(int32_t* tours, int P, int n, const double* x
                                    void* stream);
 
 /*
+ * tspmi_two_opt_packed_euc2d(_cost, _log) from int32 coordinates ixy (n, 2), 8-byte aligned, by the
+ * integer form of the EUC_2D distance: the caller guarantees integer-valued coordinates whose
+ * per-axis span is at most 2^20, where the integer form equals the FP64 one (else the distances
+ * are wrong).  cost and pass_log may be NULL (log_moves is then ignored).  Same tours, moves,
+ * counters, costs and log costs as the float64 entry points on the same coordinates; about half the
+ * VALU work and half the coordinate bytes per distance.
+ * Replaces: two_opt.py:831-899 in candidate mode on EUC_2D problems with integer coordinates.
+ */
+int tspmi_two_opt_packed_euc2d_i32(int32_t* tours, int P, int n, const int32_t* ixy, const int32_t* table, int k,
+                                   int max_iter, int64_t* stats, double* cost, int64_t* pass_log, int log_moves,
+                                   void* stream);
+
+/*
  * Best-improvement 2-opt for instances too large for the LDS-resident kernels (candidate scan
  * above n ~ 13,500, full neighbourhood above n = 16,386): each tour's positions, edge costs and
  * (candidate mode) inverse positions live in the caller's workspace ws, device int32[P * (3n + 1)],
@@ -400,19 +413,23 @@ int tspmi_batch_knn(const int64_t* desc, const int32_t* item_inst, const int32_t
 int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
                             const uint64_t* seeds, int32_t* pop, void* stream);
 
-/* Tour costs of listed (instance, row) items into key slot 2 * row offset + row; with xy_all
- * (may be NULL) the EUC_2D distances are computed from the coordinates instead of read from D_all
- * (same values).  Replaces tour_cost.py:17-65 per instance. */
+/* Tour costs of listed (instance, row) items into key slot 2 * row offset + row; with xy_all or
+ * ixy_all (either may be NULL) the EUC_2D distances are computed from the coordinates instead of
+ * read from D_all (same values).  ixy_all holds the same coordinates as int32 at the same offsets
+ * (descriptor field [2]); when given, it is used, by the integer distance form, and every
+ * instance's coordinates must be integers spanning at most 2^20 per axis.
+ * Replaces tour_cost.py:17-65 per instance. */
 int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
-                     const int32_t* pop, const int32_t* D_all, const double* xy_all, double* keys, void* stream);
+                     const int32_t* pop, const int32_t* D_all, const double* xy_all, const int32_t* ixy_all,
+                     double* keys, void* stream);
 
 /* Fused generation of every instance: items are (instance, child row r); block in
  * {128, 256, 512, 1024}; grid <= 0 means one workgroup per item, else a persistent grid; child
- * costs from xy_all when given (as tspmi_batch_cost).
+ * costs from xy_all or ixy_all when given (as tspmi_batch_cost).
  * Replaces genetic_algorithm.py:120-213 per instance. */
 int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
                            int max_n, const int32_t* popA, double* keysA, const int32_t* D_all, const double* xy_all,
-                           const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
+                           const int32_t* ixy_all, const int32_t* tour_cand, int k, const int32_t* cut_lo, const int32_t* cut_hi,
                            const double* probs, const int32_t* pos1, const int32_t* pos2, double rate, int32_t* off,
                            int block, int grid, void* stream);
 
@@ -433,11 +450,12 @@ int tspmi_batch_two_opt_packed(const int64_t* desc, const int32_t* item_inst, co
                                int max_iter, int block, int grid, int64_t* stats, void* stream);
 
 /* As tspmi_batch_two_opt_packed; instances with n >= coord_min_n compute EUC_2D distances from
- * xy_all (descriptor field [2]) instead of reading their matrix (same tours). */
+ * xy_all (descriptor field [2]) instead of reading their matrix (same tours); from ixy_all, by the
+ * integer form, when it is given (as tspmi_batch_cost). */
 int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
                                      int nitems, int max_n, int32_t* pop, const int32_t* D_all, const double* xy_all,
-                                     const int32_t* table_all, int k, int max_iter, int block, int grid,
-                                     int coord_min_n, int64_t* stats, void* stream);
+                                     const int32_t* ixy_all, const int32_t* table_all, int k, int max_iter, int block,
+                                     int grid, int coord_min_n, int64_t* stats, void* stream);
 
 /* First minimum among the first `count` keys of every instance.  Replaces genetic_algorithm.py:708-739. */
 int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, int32_t* best, void* stream);
@@ -456,14 +474,15 @@ int tspmi_batch_best(const int64_t* desc, int B, const double* keys, int count, 
  * results are then invalid), word 3 counts calls, the others are back at zero after every call,
  * so consecutive calls on one stream and graph replays need no clearing; calls that share
  * `counters` must not overlap.  block in {256, 512, 1024}; grid <= 0 launches as many workgroups
- * as fit on the device at once.
+ * as fit on the device at once.  Distances from ixy_all / xy_all as in tspmi_batch_two_opt_packed_mixed.
  * Replaces genetic_algorithm.py:120-296 (selection, crossover, mutation, evaluation, survival
  * steps) and pipeline_steps.py:153-195 (EliteOnlyLocalSearchStep) per instance; the reference
  * has no multi-instance batch. */
 int tspmi_batch_fused_generation(const int64_t* desc, int B, const uint32_t* tasks, int ntasks, int32_t* counters,
                                  int children_per_task, int max_n, int max_p, const int32_t* popA, double* keysA,
                                  int32_t* off, int32_t* order, int32_t* popB, double* keysB, const int32_t* D_all,
-                                 const double* xy_all, const int32_t* table_all, const int32_t* tour_cand,
+                                 const double* xy_all, const int32_t* ixy_all, const int32_t* table_all,
+                                 const int32_t* tour_cand,
                                  int k_tour, const int32_t* cut_lo, const int32_t* cut_hi, const double* probs,
                                  const int32_t* pos1, const int32_t* pos2, double rate, int elite, int k,
                                  int max_iter, int coord_min_n, int block, int grid, int64_t* stats, int32_t* best,

@@ -184,6 +184,43 @@ def test_fused_generation_matches_staged(P, E, kc, iters, ns):
   _same_state(staged, fused, best_s, best_f)
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_batch_integer_distances_match_fp64(fused):
+  """The batch on integer instances with every distance from int32 coordinates (integer form)
+  evolves exactly as with the FP64 form: populations, keys, best rows and counters, staged and
+  fused."""
+  import torch
+
+  from tsp_gpu_clean_amd.batch import BatchedMemeticGA
+
+  ns = [4200, 300, 57, 6, 1500, 90]
+  say(f"batch integer vs FP64 distances fused={fused} ns={ns}")
+  coords = _coords(ns, seed=13)
+  G = 5
+  runs = []
+  for integer in (False, True):
+    ga = BatchedMemeticGA(coords, population_size=16, mutation_rate=0.3, tournament_k=5, elite_count=5,
+                          ls_max_iterations=10, ls_candidate_k=20, seeds=[70 + i for i in range(len(ns))],
+                          integer_distances=integer)
+    assert (ga.ixy is not None) == integer
+    ga.shapes["fused"] = fused
+    ga.shapes["fused_block"] = 512
+    ga.init_random()
+    d = ga.draw(G)
+    best = torch.empty((G, len(ns)), dtype=torch.int32, device="cuda")
+    ga.run(d, G, best)
+    if fused:
+      ga.verify()
+    torch.cuda.synchronize()
+    runs.append((ga.pop[ga.cur].cpu().numpy(), [ga.fitness(i).cpu().numpy() for i in range(len(ns))],
+                 best.cpu().numpy(), ga.stats.cpu().numpy()))
+  np.testing.assert_array_equal(runs[0][0], runs[1][0])
+  for a, b in zip(runs[0][1], runs[1][1]):
+    np.testing.assert_array_equal(a, b)
+  np.testing.assert_array_equal(runs[0][2], runs[1][2])
+  np.testing.assert_array_equal(runs[0][3], runs[1][3])
+
+
 def test_fused_generation_config4_full_batch_matches_staged():
   import torch
 

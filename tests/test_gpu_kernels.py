@@ -787,10 +787,17 @@ def test_two_opt_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, iters, kind):
 
 @pytest.mark.parametrize("n,P,k,iters", [(40, 6, 8, -1), (1000, 24, 20, -1), (1000, 300, 20, -1), (1000, 600, 32, -1),
                                          (10000, 4, 8, 40)])
-def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iters):
-  say(f"two_opt_packed_euc2d n={n} P={P} k={k}")
-  xy = uniform_coords(n, seed=3 * n)
+@pytest.mark.parametrize("form", ["f64", "i32", "i32_span_limit", "i32_lattice"])
+def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iters, form):
+  """Candidate scan with every distance from coordinates, float64 (FP64 form) or int32 (integer
+  form: uniform integers, a box of exactly the 2^20 span with nint-boundary pairs, a tie-heavy
+  lattice): tours and counters equal the matrix kernel's."""
+  say(f"two_opt_packed_euc2d n={n} P={P} k={k} {form}")
+  xy = {"f64": lambda: uniform_coords(n, seed=3 * n), "i32": lambda: uniform_coords(n, seed=3 * n),
+        "i32_span_limit": lambda: span_limit_int_coords(n, seed=n),
+        "i32_lattice": lambda: lattice_coords(n)}[form]()
   D, _ = device_dist(nat, torch_mod, xy)
+  X = cu(xy.astype(np.int32)) if form.startswith("i32") else cu(xy)
   cand = torch_mod.empty((n, k), dtype=torch_mod.int32, device="cuda")
   nat.knn(D, k, cand)
   table = torch_mod.empty((n, nat.candidate_lanes(k), 2), dtype=torch_mod.int32, device="cuda")
@@ -800,7 +807,7 @@ def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iter
   sa = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
   sb = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
   nat.two_opt_packed(a, D, table, k, iters, sa)
-  nat.two_opt_packed_euc2d(b, cu(xy), table, k, iters, sb)
+  nat.two_opt_packed_euc2d(b, X, table, k, iters, sb)
   np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy())
   assert sb.cpu().tolist() == sa.cpu().tolist()
 

@@ -265,6 +265,34 @@ def test_local_search_strategy_with_learned_candidates(env):
   assert not np.array_equal(np.sort(cand, axis=1), np.sort(knn, axis=1))  # the lists really differ
 
 
+def test_pass_log_integer_form_equals_fp64_form(env):
+  """The logged candidate scan from int32 coordinates (integer distance form) leaves the same
+  tours, counters and final costs and logs the same cost after every move as from float64."""
+  torch, nat = env
+  n, P, k = 3000, 24, 8
+  xy_h = np.random.default_rng(6).integers(0, 1_000_001, (n, 2)).astype(np.float64)
+  xy = torch.from_numpy(xy_h).cuda()
+  ixy = torch.from_numpy(xy_h.astype(np.int32)).cuda()
+  D = torch.empty((n, n), dtype=torch.int32, device="cuda")
+  nat.dist_euc2d(xy, D)
+  cand = torch.empty((n, k), dtype=torch.int32, device="cuda")
+  nat.knn(D, k, cand)
+  tab = torch.empty((n, nat.candidate_lanes(k), 2), dtype=torch.int32, device="cuda")
+  nat.candidate_table(cand, D, tab)
+  start = torch.from_numpy(np.stack([np.random.default_rng(50 + i).permutation(n) for i in range(P)])
+                           .astype(np.int32)).cuda()
+  out = {}
+  for name, X in (("f64", xy), ("i32", ixy)):
+    log = torch.full((P, 6001, 2), -1, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+    cost = torch.empty(P, dtype=torch.float64, device="cuda")
+    t = start.clone()
+    nat.two_opt_packed_euc2d_log(t, X, tab, k, -1, log, stats, cost)
+    out[name] = (t.cpu().numpy(), stats.cpu().tolist(), cost.cpu().numpy(), log.cpu().numpy()[:, :, 0])
+  for a, b in zip(out["f64"], out["i32"]):
+    np.testing.assert_array_equal(a, b)
+
+
 def test_pass_log_records_exact_costs_after_each_move(env):
   """The logged coordinate scan: same tours as the plain launch; entry m of a tour's log is the
   cost of the tour the plain kernel leaves after m passes (one move per pass until the optimum);

@@ -254,3 +254,21 @@ def test_bench_diag_lane_counts_match_brute_force():
     assert mg + mc == g
   g, c = bench.diag_lanes_per_pass(1000, mix=True)
   assert 0.52 < c / (g + c) < 0.5625
+
+
+def test_int_coordinates_only_for_integer_instances_within_the_span():
+  """The integer distance form is offered only where it equals the FP64 one: integer coordinates
+  (|v| <= 2^30) with each axis spanning at most 2^20; anything else keeps the FP64 form."""
+  from tsp_gpu_clean_amd.context import INT_COORD_SPAN, int_coordinates
+
+  ok = np.array([[0.0, 5.0], [INT_COORD_SPAN, 7.0], [3.0, -INT_COORD_SPAN + 7.0]])
+  got = int_coordinates(ok)
+  assert got.dtype == np.int32 and np.array_equal(got, ok.astype(np.int32))
+  far = -(2.0**30)
+  assert int_coordinates(np.array([[far, 1.0], [far + 5.0, 2.0]])) is not None
+  assert int_coordinates(np.array([[0.0, 0.0], [INT_COORD_SPAN + 1.0, 0.0]])) is None  # span
+  assert int_coordinates(np.array([[0.0, 0.0], [0.0, INT_COORD_SPAN + 1.0]])) is None
+  assert int_coordinates(np.array([[0.5, 0.0], [1.0, 2.0]])) is None  # fractional
+  assert int_coordinates(np.array([[0.0, np.nan], [1.0, 2.0]])) is None
+  assert int_coordinates(np.array([[2.0**31, 0.0], [2.0**31 + 1, 0.0]])) is None  # beyond int32
+  assert int_coordinates(np.zeros((0, 2))) is None

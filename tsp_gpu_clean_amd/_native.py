@@ -57,15 +57,15 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_batch_dist_euc2d": [_P, _P, _P, _I, _P, _P, _P],
   "tspmi_batch_knn": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_batch_random_init": [_P, _P, _P, _I, _P, _P, _P],
-  "tspmi_batch_cost": [_P, _P, _P, _I, _P, _P, _P, _P, _P],
-  "tspmi_batch_generation": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _I, _P],
+  "tspmi_batch_cost": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+  "tspmi_batch_generation": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _I, _P],
   "tspmi_batch_survival": [_P, _I, _P, _P, _P],
   "tspmi_batch_gather": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_batch_two_opt_packed": [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P],
-  "tspmi_batch_two_opt_packed_mixed": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
+  "tspmi_batch_two_opt_packed_mixed": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
   "tspmi_batch_best": [_P, _I, _P, _I, _P, _P],
-  "tspmi_batch_fused_generation": [_P, _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P,
-                                   _P, _P, _D, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P],
+  "tspmi_batch_fused_generation": [_P, _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P,
+                                   _P, _P, _P, _P, _D, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P],
   "tspmi_nn_greedy_init": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_ox": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_dist_metric": [_P, _I, _I, _P, _I, _P, _P],
@@ -74,6 +74,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_packed_euc2d": [_P, _I, _I, _P, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_euc2d_cost": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_two_opt_packed_euc2d_log": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
+  "tspmi_two_opt_packed_euc2d_i32": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
   "tspmi_wall_clock_khz": [_P],
   "tspmi_two_opt_large": [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_scorer_param_count": [],
@@ -585,11 +586,14 @@ def two_opt_euc2d(tours, xy, max_iter: int = 10, stats=None) -> None:
 
 
 def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None, cost=None) -> None:
-  """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates;
-  ``cost`` as for :func:`two_opt_packed` (needs n >= 4 and max_iter != 0)."""
+  """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates:
+  float64 ``xy``, or int32 ``xy`` (integer coordinates spanning at most 2^20 per axis, as
+  ``ProblemContext.get_gpu_int_coordinates`` gives them) for the integer distance form; same
+  results.  ``cost`` as for :func:`two_opt_packed` (needs n >= 4 and max_iter != 0)."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
-  _need(xy, torch.float64, "xy", 2)
+  int_xy = isinstance(xy, torch.Tensor) and xy.dtype == torch.int32
+  _need(xy, torch.int32 if int_xy else torch.float64, "xy", 2)
   _need(table, torch.int32, "table", 3)
   if stats is not None:
     _need(stats, torch.int64, "stats")
@@ -600,8 +604,14 @@ def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=Non
     _need(cost, torch.float64, "cost", 1)
     if cost.shape[0] < P:
       raise ValueError("two_opt_packed_euc2d: cost shorter than the number of tours")
-  _check(_lib.tspmi_two_opt_packed_euc2d_cost(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
-                                              _ptr(stats), _ptr(cost), _stream(torch)), "tspmi_two_opt_packed_euc2d_cost")
+  if int_xy:
+    _check(_lib.tspmi_two_opt_packed_euc2d_i32(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
+                                               _ptr(stats), _ptr(cost), None, 0, _stream(torch)),
+           "tspmi_two_opt_packed_euc2d_i32")
+  else:
+    _check(_lib.tspmi_two_opt_packed_euc2d_cost(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
+                                                _ptr(stats), _ptr(cost), _stream(torch)),
+           "tspmi_two_opt_packed_euc2d_cost")
 
 
 def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, stats=None, cost=None) -> None:
@@ -613,7 +623,8 @@ def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, 
   if pass_log.shape[0] != P or pass_log.shape[2] != 2 or pass_log.shape[1] < 1:
     raise ValueError(f"two_opt_packed_euc2d_log: pass_log {tuple(pass_log.shape)} is not (P, M + 1, 2)")
   _need(tours, torch.int32, "tours", 2)
-  _need(xy, torch.float64, "xy", 2)
+  int_xy = isinstance(xy, torch.Tensor) and xy.dtype == torch.int32
+  _need(xy, torch.int32 if int_xy else torch.float64, "xy", 2)
   _need(table, torch.int32, "table", 3)
   if stats is not None:
     _need(stats, torch.int64, "stats")
@@ -621,9 +632,10 @@ def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, 
     raise ValueError("two_opt_packed_euc2d_log: xy / table do not match n / k")
   if cost is not None:
     _need(cost, torch.float64, "cost", 1)
-  _check(_lib.tspmi_two_opt_packed_euc2d_log(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
-                                             _ptr(stats), _ptr(cost), _ptr(pass_log), pass_log.shape[1] - 1,
-                                             _stream(torch)), "tspmi_two_opt_packed_euc2d_log")
+  fn = _lib.tspmi_two_opt_packed_euc2d_i32 if int_xy else _lib.tspmi_two_opt_packed_euc2d_log
+  _check(fn(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter), _ptr(stats), _ptr(cost), _ptr(pass_log),
+            pass_log.shape[1] - 1, _stream(torch)),
+         "tspmi_two_opt_packed_euc2d_i32" if int_xy else "tspmi_two_opt_packed_euc2d_log")
 
 
 #: Largest n of the LDS-resident kernels: candidate scans keep (3n + 1) int32 in LDS next to 1 KiB

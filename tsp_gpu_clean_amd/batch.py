@@ -21,6 +21,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
+from .context import int_coordinates
 from .engine import capture_graph, draw_generations
 from .rng import draw_row_seeds, make_rng
 
@@ -53,7 +54,8 @@ def _cu(a):
 
 class BatchedMemeticGA:
   def __init__(self, coords, population_size: int = 64, mutation_rate: float = 0.05, tournament_k: int = 5,
-               elite_count: int = 16, ls_max_iterations: int = 10, ls_candidate_k: int = 20, seeds=None) -> None:
+               elite_count: int = 16, ls_max_iterations: int = 10, ls_candidate_k: int = 20, seeds=None,
+               integer_distances: bool = True) -> None:
     torch = _native.require_device()
     self.B = len(coords)
     self.ns = np.array([int(np.asarray(c).shape[0]) for c in coords], dtype=np.int64)
@@ -108,6 +110,12 @@ class BatchedMemeticGA:
     self.counters = torch.zeros(4 + 3 * B, dtype=torch.int32, device="cuda")
     xy = np.concatenate([np.ascontiguousarray(c, dtype=np.float64).reshape(-1) for c in coords])
     self.xy = _cu(xy)
+    # Integer instances (every one within the 2^20 span) take the integer distance form: the same
+    # distances from int32 coordinates at the same offsets, about half the work per distance.
+    # ``integer_distances=False`` keeps the FP64 form (same results) for comparison.
+    ixy = [int_coordinates(np.asarray(c, dtype=np.float64).reshape(-1, 2)) for c in coords]
+    use_int = integer_distances and all(a is not None for a in ixy)
+    self.ixy = _cu(np.concatenate([a.reshape(-1) for a in ixy])) if use_int else None
     total_pop = int((P * ns).sum())
     dev = "cuda"
     self.D = torch.empty(int((ns * ns).sum()), dtype=torch.int32, device=dev)
@@ -137,7 +145,7 @@ class BatchedMemeticGA:
     """SplitMix64 random permutations from each instance's generator (random_init.py:30-55)."""
     seeds = np.concatenate([draw_row_seeds(r, self.P) for r in self.rngs])
     _native.batch_call("tspmi_batch_random_init", self.desc, *self.w_row, _cu(seeds.view(np.int64)), self.pop[self.cur])
-    _native.batch_call("tspmi_batch_cost", self.desc, *self.w_row, self.pop[self.cur], self.D, self.xy,
+    _native.batch_call("tspmi_batch_cost", self.desc, *self.w_row, self.pop[self.cur], self.D, self.xy, self.ixy,
                        self.keys[self.cur])
 
   def population(self, i: int):
@@ -181,17 +189,17 @@ class BatchedMemeticGA:
   # ---- one generation ---------------------------------------------------------------------------
   def _generation(self, a, d: BatchDraws, g: int, block: int, grid: int) -> None:
     _native.batch_call("tspmi_batch_generation", self.desc, *self.w_row, self.max_n, self.pop[a], self.keys[a], self.D,
-                       self.xy, d.tour_cand[g], self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g],
+                       self.xy, self.ixy, d.tour_cand[g], self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g],
                        self.rate, self.off, block, grid)
 
   def _elite_ls(self, pop, block: int, grid: int) -> None:
     _native.batch_call("tspmi_batch_two_opt_packed_mixed", self.desc, *self.w_elite, self.max_n, pop, self.D, self.xy,
-                       self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
+                       self.ixy, self.table, self.kc, self.ls_iters, block, grid, COORD_MIN_N, self.stats)
 
   def _fused(self, a, b, d: BatchDraws, g: int, best, block: int, grid: int) -> None:
     _native.batch_call("tspmi_batch_fused_generation", self.desc, self.B, self.tasks, int(self.tasks_host.size),
                        self.counters, self.children_per_task, self.max_n, self.P, self.pop[a], self.keys[a], self.off,
-                       self.order, self.pop[b], self.keys[b], self.D, self.xy, self.table, d.tour_cand[g],
+                       self.order, self.pop[b], self.keys[b], self.D, self.xy, self.ixy, self.table, d.tour_cand[g],
                        self.k_tour, d.cut_lo[g], d.cut_hi[g], d.probs[g], d.pos1[g], d.pos2[g], self.rate, self.elite,
                        self.kc, self.ls_iters, COORD_MIN_N, block, grid, self.stats, best, max(1, self.elite))
 
@@ -216,7 +224,8 @@ class BatchedMemeticGA:
                        self.pop[b], self.keys[b])
     if self.elite > 0:
       self._elite_ls(self.pop[b], s["ls_block"], s["ls_grid"])
-      _native.batch_call("tspmi_batch_cost", self.desc, *self.w_elite, self.pop[b], self.D, self.xy, self.keys[b])
+      _native.batch_call("tspmi_batch_cost", self.desc, *self.w_elite, self.pop[b], self.D, self.xy, self.ixy,
+                         self.keys[b])
     if best is not None:
       _native.batch_call("tspmi_batch_best", self.desc, self.B, self.keys[b], max(1, self.elite), best)
 

@@ -37,6 +37,24 @@ def coordinate_candidates_pay(n: int, tours: int | None = None) -> bool:
   return n > 4096
 
 
+#: Largest per-axis coordinate span for which the integer EUC_2D distance form equals the FP64 one.
+INT_COORD_SPAN = 1 << 20
+
+
+def int_coordinates(xy) -> np.ndarray | None:
+  """(n, 2) int32 copy of EUC_2D coordinates for the kernels' integer distance form, or None when
+  it does not apply: every coordinate must be an integer (|v| <= 2^30) and each axis must span at
+  most 2^20, where that form gives the FP64 nint exactly."""
+  a = np.asarray(xy, dtype=np.float64)
+  if a.ndim != 2 or a.shape[1] != 2 or a.shape[0] == 0 or not np.isfinite(a).all():
+    return None
+  if not (a == np.rint(a)).all() or np.abs(a).max() > 2.0**30:
+    return None
+  if (a.max(axis=0) - a.min(axis=0)).max() > INT_COORD_SPAN:
+    return None
+  return np.ascontiguousarray(a.astype(np.int32))
+
+
 def _cu_count() -> int:
   try:
     import torch
@@ -104,6 +122,7 @@ class ProblemContext:
     self._gpu_distances = None
     self._cpu_distances: np.ndarray | None = None
     self._gpu_coordinates = None
+    self._gpu_int_coordinates = None  # False once known not to apply
     self._gpu_candidates: dict[int, Any] = {}
     self._cpu_candidates: dict[int, np.ndarray] = candidate_cache if candidate_cache is not None else {}
     self._candidate_ties: dict[int, int] = {}
@@ -138,6 +157,20 @@ class ProblemContext:
     if self.problem.edge_type != "EUC_2D" or self.problem.coordinates is None or self.problem.distances is not None:
       return None
     return self.get_gpu_coordinates()
+
+  def get_gpu_int_coordinates(self):
+    """Device (n, 2) int32 coordinates for the integer form of the EUC_2D distance (same values,
+    about half the work per distance), or None unless the distances are EUC_2D of integer
+    coordinates spanning at most 2^20 per axis (``int_coordinates``)."""
+    torch = _native.require_device()
+    if self._gpu_int_coordinates is None:
+      ixy = None
+      if self.get_gpu_euc2d_coordinates() is not None:
+        ixy = int_coordinates(self.problem.coordinates)
+      self._gpu_int_coordinates = False if ixy is None else torch.from_numpy(ixy).to("cuda")
+      if ixy is not None:
+        self.h2d_bytes += int(ixy.nbytes)
+    return self._gpu_int_coordinates if self._gpu_int_coordinates is not False else None
 
   def get_gpu_distances(self):
     """Dense (n, n) int32 matrix on the device, built once (problem_context.py:337-397)."""

@@ -38,6 +38,13 @@ __device__ __forceinline__ Inst load_inst(const int64_t* __restrict__ desc, int 
   return Inst{(int)r[kBN], (int)r[kBP], r[kBXY], r[kBD], r[kBCT], r[kBCand], r[kBPop], r[kBRow], r[kBPair]};
 }
 
+// The instance's coordinates in the form given: int32 pairs (the integer distance form) when
+// ixy_all is set, else float64.  Both arrays hold two entries per city at the instance's offset.
+__device__ __forceinline__ AnyCoordDist batch_coords(const double* xy_all, const int32_t* ixy_all, const Inst& I) {
+  return AnyCoordDist{xy_all != nullptr ? reinterpret_cast<const double2*>(xy_all + I.xy) : nullptr,
+                      ixy_all != nullptr ? reinterpret_cast<const int2*>(ixy_all + I.xy) : nullptr};
+}
+
 // D rows of every instance: one workgroup per (instance, row u).
 __global__ __launch_bounds__(256) void batch_dist_kernel(const int64_t* __restrict__ desc,
                                                          const int32_t* __restrict__ item_inst,
@@ -126,7 +133,8 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restri
                                                          const int32_t* __restrict__ item_inst,
                                                          const int32_t* __restrict__ item_idx, int nitems,
                                                          const int32_t* __restrict__ pop, const int32_t* __restrict__ D_all,
-                                                         const double* __restrict__ xy_all, double* __restrict__ keys) {
+                                                         const double* __restrict__ xy_all,
+                                                         const int32_t* __restrict__ ixy_all, double* __restrict__ keys) {
   const int it = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (it >= nitems) return;
@@ -134,12 +142,13 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const int64_t* __restri
   const int r = item_idx[it];
   const int32_t* t = pop + I.pop + (size_t)r * I.n;
   const MatrixDist md{D_all + I.d, I.n};
-  const CoordDist cd{reinterpret_cast<const double2*>(xy_all + I.xy)};
+  const AnyCoordDist cd = batch_coords(xy_all, ixy_all, I);
+  const bool coords = xy_all != nullptr || ixy_all != nullptr;
   int64_t s = 0;
   for (int x = lane; x < I.n; x += kWave) {
     const int a = clamp_id(t[x], I.n);
     const int b = clamp_id(t[x + 1 == I.n ? 0 : x + 1], I.n);
-    if (I.n > 1) s += xy_all != nullptr ? cd(a, b) : md(a, b);
+    if (I.n > 1) s += coords ? cd(a, b) : md(a, b);
   }
   s = wave_sum_i64(s);
   if (lane == 0) keys[2 * I.row + r] = (double)s;
@@ -150,7 +159,7 @@ __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const 
                                         const int32_t* __restrict__ item_idx, int nitems,
                                         const int32_t* __restrict__ popA, double* __restrict__ keysA,
                                         const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
-                                        const int32_t* __restrict__ tour_cand, int k,
+                                        const int32_t* __restrict__ ixy_all, const int32_t* __restrict__ tour_cand, int k,
                                         const int32_t* __restrict__ cut_lo, const int32_t* __restrict__ cut_hi,
                                         const double* __restrict__ probs, const int32_t* __restrict__ pos1,
                                         const int32_t* __restrict__ pos2, double rate, int32_t* __restrict__ off) {
@@ -160,9 +169,9 @@ __global__ void batch_generation_kernel(const int64_t* __restrict__ desc, const 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Inst I = load_inst(desc, item_inst[it]);
     double* keys = keysA + 2 * I.row;
-    if (xy_all != nullptr)
-      generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n,
-                       CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)}, tour_cand + (size_t)I.row * k, k,
+    if (xy_all != nullptr || ixy_all != nullptr)
+      generation_child(item_idx[it], popA + I.pop, keys, I.P, I.n, batch_coords(xy_all, ixy_all, I),
+                       tour_cand + (size_t)I.row * k, k,
                        cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate, off + I.pop,
                        keys + I.P, lds, red, s_w);
     else
@@ -237,7 +246,8 @@ template <int BLOCK, int KL>
 __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst, const int32_t* __restrict__ item_idx,
     int nitems, int32_t* __restrict__ pop, const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
-    const int2* __restrict__ ct_all, int k, int max_iter, int coord_min_n, int64_t* __restrict__ stats) {
+    const int32_t* __restrict__ ixy_all, const int2* __restrict__ ct_all, int k, int max_iter, int coord_min_n,
+    int64_t* __restrict__ stats) {
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
@@ -246,7 +256,10 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int ke = k < I.n - 1 ? k : I.n - 1;
     int32_t* tour = pop + I.pop + (size_t)item_idx[it] * I.n;
     if (I.n >= 4 && ke >= 1) {
-      if (I.n >= coord_min_n)
+      if (I.n >= coord_min_n && ixy_all != nullptr)
+        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, IntCoordDist{reinterpret_cast<const int2*>(ixy_all + I.xy)},
+                                            ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+      else if (I.n >= coord_min_n)
         cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
                                             ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
       else
@@ -401,8 +414,8 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
     const int64_t* __restrict__ desc, int B, const uint32_t* __restrict__ tasks, int ntasks, int* __restrict__ ctr,
     int cpt, const int32_t* __restrict__ popA, double* __restrict__ keysA, int32_t* __restrict__ off,
     int32_t* __restrict__ order, int32_t* __restrict__ popB, double* __restrict__ keysB,
-    const int32_t* __restrict__ D_all, const double* __restrict__ xy_all, const int2* __restrict__ ct_all,
-    const int32_t* __restrict__ tour_cand, int k_tour, const int32_t* __restrict__ cut_lo,
+    const int32_t* __restrict__ D_all, const double* __restrict__ xy_all, const int32_t* __restrict__ ixy_all,
+    const int2* __restrict__ ct_all, const int32_t* __restrict__ tour_cand, int k_tour, const int32_t* __restrict__ cut_lo,
     const int32_t* __restrict__ cut_hi, const double* __restrict__ probs, const int32_t* __restrict__ pos1,
     const int32_t* __restrict__ pos2, double rate, int elite, int kc, int max_iter, int coord_min_n,
     int64_t* __restrict__ stats, int32_t* __restrict__ best, int best_count) {
@@ -434,14 +447,15 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
     const Inst I = load_inst(desc, inst);
     const int n = I.n;
     const int e = elite < I.P ? elite : I.P;
-    const CoordDist cd{reinterpret_cast<const double2*>(xy_all + I.xy)};
+    const AnyCoordDist cd = batch_coords(xy_all, ixy_all, I);
+    const bool coords = xy_all != nullptr || ixy_all != nullptr;
     const MatrixDist md{D_all + I.d, n};
     const uint32_t kind = task_kind(task);
     if (kind == kTaskChild) {
       double* keys = keysA + 2 * I.row;
       const int r1 = (idx + 1) * cpt < I.P ? (idx + 1) * cpt : I.P;
       for (int r = idx * cpt; r < r1; ++r) {
-        if (xy_all != nullptr)
+        if (coords)
           generation_child(r, popA + I.pop, keys, I.P, n, cd, tour_cand + (size_t)I.row * k_tour, k_tour,
                            cut_lo + I.pair, cut_hi + I.pair, probs + I.row, pos1 + I.row, pos2 + I.row, rate,
                            off + I.pop, keys + I.P, lds, red64, s_w);
@@ -497,9 +511,14 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
       double* kout = keysB + 2 * I.row + idx;
       const int ke = kc < n - 1 ? kc : n - 1;
       if (n >= 4 && ke >= 1 && max_iter != 0) {
-        if (n >= coord_min_n)
-          cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, cd, ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout,
-                                              nullptr, 0, src);
+        if (n >= coord_min_n && ixy_all != nullptr)
+          cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, IntCoordDist{reinterpret_cast<const int2*>(ixy_all + I.xy)},
+                                              ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout, nullptr, 0,
+                                              src);
+        else if (n >= coord_min_n)
+          cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
+                                              ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout, nullptr, 0,
+                                              src);
         else
           cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, md, ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout,
                                               nullptr, 0, src);
@@ -509,7 +528,7 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
           dst[x] = src[x];
           const int a = clamp_id(src[x], n);
           const int b = clamp_id(src[x + 1 == n ? 0 : x + 1], n);
-          if (n > 1) part += xy_all != nullptr ? cd(a, b) : md(a, b);
+          if (n > 1) part += coords ? cd(a, b) : md(a, b);
         }
         const int64_t total = block_sum_i64(part, red64);
         if (tid == 0) *kout = (double)total;
@@ -583,21 +602,23 @@ extern "C" int tspmi_batch_random_init(const int64_t* desc, const int32_t* item_
 }
 
 extern "C" int tspmi_batch_cost(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx, int nitems,
-                                const int32_t* pop, const int32_t* D_all, const double* xy_all, double* keys,
-                                void* stream) {
+                                const int32_t* pop, const int32_t* D_all, const double* xy_all, const int32_t* ixy_all,
+                                double* keys, void* stream) {
   TSPMI_BATCH_ARGS("tspmi_batch_cost");
   if (nitems == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(pop && (D_all || xy_all) && keys, "tspmi_batch_cost: null pointer");
-  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_cost: xy must be 16-byte aligned");
+  TSPMI_CHECK_ARG(pop && (D_all || xy_all || ixy_all) && keys, "tspmi_batch_cost: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0 && ((uintptr_t)ixy_all & 7) == 0,
+                  "tspmi_batch_cost: xy must be 16-byte and ixy 8-byte aligned");
   hipLaunchKernelGGL(batch_cost_kernel, dim3((nitems + 3) / 4), dim3(256), 0, (hipStream_t)stream, desc, item_inst,
-                     item_idx, nitems, pop, D_all, xy_all, keys);
+                     item_idx, nitems, pop, D_all, xy_all, ixy_all, keys);
   TSPMI_CHECK_LAUNCH("tspmi_batch_cost launch");
   return TSPMI_OK;
 }
 
 extern "C" int tspmi_batch_generation(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
                                       int nitems, int max_n, const int32_t* popA, double* keysA, const int32_t* D_all,
-                                      const double* xy_all, const int32_t* tour_cand, int k, const int32_t* cut_lo,
+                                      const double* xy_all, const int32_t* ixy_all, const int32_t* tour_cand, int k,
+                                      const int32_t* cut_lo,
                                       const int32_t* cut_hi,
                                       const double* probs, const int32_t* pos1, const int32_t* pos2, double rate,
                                       int32_t* off, int block, int grid, void* stream) {
@@ -605,15 +626,17 @@ extern "C" int tspmi_batch_generation(const int64_t* desc, const int32_t* item_i
   TSPMI_CHECK_ARG(block == 128 || block == 256 || block == 512 || block == 1024,
                   "tspmi_batch_generation: block %d not in {128, 256, 512, 1024}", block);
   if (nitems == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_generation: xy must be 16-byte aligned");
-  TSPMI_CHECK_ARG(popA && keysA && (D_all || xy_all) && tour_cand && cut_lo && cut_hi && probs && pos1 && pos2 && off &&
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0 && ((uintptr_t)ixy_all & 7) == 0,
+                  "tspmi_batch_generation: xy must be 16-byte and ixy 8-byte aligned");
+  TSPMI_CHECK_ARG(popA && keysA && (D_all || xy_all || ixy_all) && tour_cand && cut_lo && cut_hi && probs && pos1 && pos2 && off &&
                       k >= 1,
                   "tspmi_batch_generation: null pointer");
   const size_t lds = (size_t)6 * max_n * sizeof(int32_t);
   TSPMI_CHECK_ARG(max_n >= 2 && lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_generation: max_n=%d exceeds LDS", max_n);
   const int g = grid > 0 && grid < nitems ? grid : nitems;
   hipLaunchKernelGGL(batch_generation_kernel, dim3(g), dim3(block), lds, (hipStream_t)stream, desc, item_inst, item_idx,
-                     nitems, popA, keysA, D_all, xy_all, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate, off);
+                     nitems, popA, keysA, D_all, xy_all, ixy_all, tour_cand, k, cut_lo, cut_hi, probs, pos1, pos2, rate,
+                     off);
   TSPMI_CHECK_LAUNCH("tspmi_batch_generation launch");
   return TSPMI_OK;
 }
@@ -680,13 +703,15 @@ extern "C" int tspmi_batch_best(const int64_t* desc, int B, const double* keys, 
 
 extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32_t* item_inst, const int32_t* item_idx,
                                                 int nitems, int max_n, int32_t* pop, const int32_t* D_all,
-                                                const double* xy_all, const int32_t* table_all, int k, int max_iter,
-                                                int block, int grid, int coord_min_n, int64_t* stats, void* stream) {
+                                                const double* xy_all, const int32_t* ixy_all, const int32_t* table_all,
+                                                int k, int max_iter, int block, int grid, int coord_min_n,
+                                                int64_t* stats, void* stream) {
   TSPMI_CHECK_ARG(desc && nitems >= 0 && max_n >= 0 && k >= 1 && k <= 32, "tspmi_batch_two_opt_packed_mixed: bad args");
   if (nitems == 0 || max_iter == 0) return TSPMI_OK;
-  TSPMI_CHECK_ARG(item_inst && item_idx && pop && D_all && xy_all && table_all,
+  TSPMI_CHECK_ARG(item_inst && item_idx && pop && D_all && (xy_all || ixy_all) && table_all,
                   "tspmi_batch_two_opt_packed_mixed: null pointer");
-  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_two_opt_packed_mixed: xy must be 16-byte aligned");
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0 && ((uintptr_t)ixy_all & 7) == 0,
+                  "tspmi_batch_two_opt_packed_mixed: xy must be 16-byte and ixy 8-byte aligned");
   const size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_two_opt_packed_mixed: n=%d exceeds the LDS limit",
                   max_n);
@@ -695,7 +720,7 @@ extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32
   const int2* ct = reinterpret_cast<const int2*>(table_all);
 #define TSPMI_BATCH_MIXED(BL, KPV, KLV)                                                                        \
   hipLaunchKernelGGL((batch_two_opt_mixed_kernel<BL, KLV>), dim3(g), dim3(BL), lds, s, desc, item_inst, item_idx,   \
-                     nitems, pop, D_all, xy_all, ct, k, max_iter, coord_min_n, stats)
+                     nitems, pop, D_all, xy_all, ixy_all, ct, k, max_iter, coord_min_n, stats)
   if (block == 256) {
     TSPMI_CAND_BY_K(TSPMI_BATCH_MIXED, 256, k)
   } else if (block == 512) {
@@ -713,7 +738,8 @@ namespace {
 template <int BL, int KLV>
 int fused_launch(int grid, size_t lds, hipStream_t s, const int64_t* desc, int B, const uint32_t* tasks, int ntasks,
                  int32_t* counters, int cpt, const int32_t* popA, double* keysA, int32_t* off, int32_t* order, int32_t* popB,
-                 double* keysB, const int32_t* D_all, const double* xy_all, const int2* ct, const int32_t* tour_cand,
+                 double* keysB, const int32_t* D_all, const double* xy_all, const int32_t* ixy_all, const int2* ct,
+                 const int32_t* tour_cand,
                  int k_tour, const int32_t* cut_lo, const int32_t* cut_hi, const double* probs, const int32_t* pos1,
                  const int32_t* pos2, double rate, int elite, int k, int max_iter, int coord_min_n, int64_t* stats,
                  int32_t* best, int best_count) {
@@ -737,7 +763,8 @@ int fused_launch(int grid, size_t lds, hipStream_t s, const int64_t* desc, int B
     grid = it->second;
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), lds, s, desc, B, tasks, ntasks, counters, cpt, popA, keysA, off, order,
-                     popB, keysB, D_all, xy_all, ct, tour_cand, k_tour, cut_lo, cut_hi, probs, pos1, pos2, rate, elite,
+                     popB, keysB, D_all, xy_all, ixy_all, ct, tour_cand, k_tour, cut_lo, cut_hi, probs, pos1, pos2, rate,
+                     elite,
                      k, max_iter, coord_min_n, stats, best, best_count);
   TSPMI_CHECK_LAUNCH("tspmi_batch_fused_generation launch");
   return TSPMI_OK;
@@ -748,7 +775,8 @@ int fused_launch(int grid, size_t lds, hipStream_t s, const int64_t* desc, int B
 extern "C" int tspmi_batch_fused_generation(
     const int64_t* desc, int B, const uint32_t* tasks, int ntasks, int32_t* counters, int children_per_task,
     int max_n, int max_p, const int32_t* popA, double* keysA, int32_t* off, int32_t* order, int32_t* popB, double* keysB,
-    const int32_t* D_all, const double* xy_all, const int32_t* table_all, const int32_t* tour_cand, int k_tour,
+    const int32_t* D_all, const double* xy_all, const int32_t* ixy_all, const int32_t* table_all,
+    const int32_t* tour_cand, int k_tour,
     const int32_t* cut_lo, const int32_t* cut_hi, const double* probs, const int32_t* pos1, const int32_t* pos2,
     double rate, int elite, int k, int max_iter, int coord_min_n, int block, int grid, int64_t* stats, int32_t* best,
     int best_count, void* stream) {
@@ -761,11 +789,13 @@ extern "C" int tspmi_batch_fused_generation(
                       elite >= 0 && best_count >= 1,
                   "tspmi_batch_fused_generation: B=%d P=%d k_tour=%d k=%d elite=%d outside the supported range", B,
                   max_p, k_tour, k, elite);
-  TSPMI_CHECK_ARG(popA && keysA && off && order && popB && keysB && (D_all || xy_all) && table_all && tour_cand &&
+  TSPMI_CHECK_ARG(popA && keysA && off && order && popB && keysB && (D_all || xy_all || ixy_all) && table_all &&
+                      tour_cand &&
                       cut_lo && cut_hi && probs && pos1 && pos2 && stats,
                   "tspmi_batch_fused_generation: null pointer");
-  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0, "tspmi_batch_fused_generation: xy must be 16-byte aligned");
-  TSPMI_CHECK_ARG(xy_all != nullptr || coord_min_n > max_n,
+  TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0 && ((uintptr_t)ixy_all & 7) == 0,
+                  "tspmi_batch_fused_generation: xy must be 16-byte and ixy 8-byte aligned");
+  TSPMI_CHECK_ARG(xy_all != nullptr || ixy_all != nullptr || coord_min_n > max_n,
                   "tspmi_batch_fused_generation: coordinates needed for n >= coord_min_n");
   TSPMI_CHECK_ARG(D_all != nullptr || coord_min_n <= 4, "tspmi_batch_fused_generation: a matrix is needed below "
                   "coord_min_n");
@@ -782,7 +812,8 @@ extern "C" int tspmi_batch_fused_generation(
   int rc = TSPMI_OK;
 #define TSPMI_BATCH_FUSED(BL, KPV, KLV)                                                                          \
   rc = fused_launch<BL, KLV>(grid, lds, s, desc, B, tasks, ntasks, counters, children_per_task, popA, keysA, off,    \
-                             order, popB, keysB, D_all, xy_all, ct, tour_cand, k_tour, cut_lo, cut_hi, probs, pos1,    \
+                             order, popB, keysB, D_all, xy_all, ixy_all, ct, tour_cand, k_tour, cut_lo, cut_hi, probs,   \
+                             pos1,                                                                                     \
                              pos2, rate, elite, k, max_iter, coord_min_n, stats, best, best_count)
   if (block == 256) {
     TSPMI_CAND_BY_K(TSPMI_BATCH_FUSED, 256, k)

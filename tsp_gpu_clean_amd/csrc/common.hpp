@@ -191,5 +191,26 @@ struct CoordDist {
   }
 };
 
+// The same distances from int32 coordinates by the integer form (every per-axis difference at
+// most kIntCoordSpan, which the caller guarantees): half the bytes per coordinate read and about
+// half the VALU cycles per distance.
+struct IntCoordDist {
+  const int2* __restrict__ xy;
+  __device__ __forceinline__ int operator()(int u, int v) const {
+    const int2 a = xy[u], b = xy[v];
+    return euc2d_nint_i32(a.x, a.y, b.x, b.y);
+  }
+};
+
+// Either coordinate form, chosen per launch (ixy given: the integer form); for the kernels'
+// cold paths, where one uniform branch per distance costs nothing that matters.
+struct AnyCoordDist {
+  const double2* __restrict__ xy;
+  const int2* __restrict__ ixy;
+  __device__ __forceinline__ int operator()(int u, int v) const {
+    return ixy != nullptr ? IntCoordDist{ixy}(u, v) : CoordDist{xy}(u, v);
+  }
+};
+
 
 }  // namespace tspmi

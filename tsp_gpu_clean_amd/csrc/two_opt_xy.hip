@@ -114,9 +114,9 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
   }
 }
 
-template <int BLOCK, int KP, int UNROLL, int KL>
-__global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n,
-                                                                const double2* __restrict__ xy,
+// XYD: CoordDist (float64 coordinates) or IntCoordDist (int32 coordinates, the integer form).
+template <int BLOCK, int KP, int UNROLL, int KL, class XYD>
+__global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n, XYD dist,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
                                                                 int64_t* __restrict__ stats,
                                                                 double* __restrict__ cost,
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restr
   extern __shared__ int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, CoordDist{xy}, ct, k, max_iter, stats,
+  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, dist, ct, k, max_iter, stats,
                                            lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr,
                                            pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1)
                                                                : nullptr,
@@ -164,8 +164,9 @@ extern "C" int tspmi_two_opt_euc2d(int32_t* tours, int P, int n, const double* x
 
 namespace {
 
-int packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* table, int k, int max_iter,
-                 int64_t* stats, double* cost, int64_t* pass_log, int log_moves, void* stream) {
+// `xy` is float64 (n, 2) (16-byte aligned), or, with int_xy, int32 (n, 2) (8-byte aligned).
+int packed_euc2d(int32_t* tours, int P, int n, const void* xy, bool int_xy, const int32_t* table, int k,
+                 int max_iter, int64_t* stats, double* cost, int64_t* pass_log, int log_moves, void* stream) {
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_packed_euc2d: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_packed_euc2d: k=%d outside [1, 32]", k);
   if (P == 0) return TSPMI_OK;
@@ -173,21 +174,26 @@ int packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* 
                   "tspmi_two_opt_packed_euc2d: costs need n >= 4 and max_iter != 0");
   if (n < 4 || max_iter == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(tours && xy && table, "tspmi_two_opt_packed_euc2d: null pointer");
-  TSPMI_CHECK_ARG(((uintptr_t)xy & 15) == 0 && ((uintptr_t)table & 7) == 0,
-                  "tspmi_two_opt_packed_euc2d: xy must be 16-byte and table 8-byte aligned");
+  TSPMI_CHECK_ARG(((uintptr_t)xy & (int_xy ? 7 : 15)) == 0 && ((uintptr_t)table & 7) == 0,
+                  "tspmi_two_opt_packed_euc2d: xy must be %d-byte and table 8-byte aligned", int_xy ? 8 : 16);
   TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed_euc2d: (n-2)*k too large");
   const size_t lds = (size_t)(3 * n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_packed_euc2d: n=%d exceeds the LDS limit", n);
   hipStream_t s = (hipStream_t)stream;
-  const double2* p = reinterpret_cast<const double2*>(xy);
   const int2* ct = reinterpret_cast<const int2*>(table);
   // One tour per CU: 16 waves each; up to four per CU: 8 waves (measured at P = 1024, n = 1000 and
   // 2000, k = 20: 0.90 / 1.73 ms per 20-pass launch at 512 threads vs 0.97 / 1.93 at 256); more: 4.
   const int cus = device_cu_count();
   const int block = P <= cus ? 1024 : (P <= 4 * cus ? 512 : 256);
 #define TSPMI_CXY(BL, KPV, KLV)                                                                                      \
-  hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV>), dim3(P), dim3(BL), lds, s, tours, n, p, ct, k, max_iter, \
-                     stats, cost, pass_log, log_moves)
+  if (int_xy)                                                                                                        \
+    hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV, IntCoordDist>), dim3(P), dim3(BL), lds, s, tours, n, \
+                       IntCoordDist{reinterpret_cast<const int2*>(xy)}, ct, k, max_iter, stats, cost, pass_log,      \
+                       log_moves);                                                                                   \
+  else                                                                                                               \
+    hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV, CoordDist>), dim3(P), dim3(BL), lds, s, tours, n,    \
+                       CoordDist{reinterpret_cast<const double2*>(xy)}, ct, k, max_iter, stats, cost, pass_log,      \
+                       log_moves)
   if (block == 1024) {
     TSPMI_CAND_BY_K(TSPMI_CXY, 1024, k)
   } else if (block == 512) {
@@ -204,14 +210,22 @@ int packed_euc2d(int32_t* tours, int P, int n, const double* xy, const int32_t* 
 
 extern "C" int tspmi_two_opt_packed_euc2d_cost(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
                                                int k, int max_iter, int64_t* stats, double* cost, void* stream) {
-  return packed_euc2d(tours, P, n, xy, table, k, max_iter, stats, cost, nullptr, 0, stream);
+  return packed_euc2d(tours, P, n, xy, false, table, k, max_iter, stats, cost, nullptr, 0, stream);
 }
 
 extern "C" int tspmi_two_opt_packed_euc2d_log(int32_t* tours, int P, int n, const double* xy, const int32_t* table,
                                               int k, int max_iter, int64_t* stats, double* cost, int64_t* pass_log,
                                               int log_moves, void* stream) {
   TSPMI_CHECK_ARG(pass_log != nullptr && log_moves >= 0, "tspmi_two_opt_packed_euc2d_log: bad pass log");
-  return packed_euc2d(tours, P, n, xy, table, k, max_iter, stats, cost, pass_log, log_moves, stream);
+  return packed_euc2d(tours, P, n, xy, false, table, k, max_iter, stats, cost, pass_log, log_moves, stream);
+}
+
+extern "C" int tspmi_two_opt_packed_euc2d_i32(int32_t* tours, int P, int n, const int32_t* ixy, const int32_t* table,
+                                              int k, int max_iter, int64_t* stats, double* cost, int64_t* pass_log,
+                                              int log_moves, void* stream) {
+  TSPMI_CHECK_ARG(pass_log == nullptr || log_moves >= 0, "tspmi_two_opt_packed_euc2d_i32: bad pass log");
+  return packed_euc2d(tours, P, n, ixy, true, table, k, max_iter, stats, cost, pass_log, pass_log ? log_moves : 0,
+                      stream);
 }
 
 extern "C" int tspmi_wall_clock_khz(int* khz) {

@@ -147,6 +147,8 @@ class DeviceMemeticLoop:
                   if self.cand is not None and 1 <= self.cand_k <= 32 else None)
     # Above a few thousand cities the candidate scan computes distances from the coordinates.
     self.xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(self.n) else None
+    if self.xy is not None and context.get_gpu_int_coordinates() is not None:
+      self.xy = context.get_gpu_int_coordinates()  # integer instances: the integer distance form
     P, n = self.P, self.n
     dev = "cuda"
     self.pop = [torch.empty((P, n), dtype=torch.int32, device=dev) for _ in range(2)]

@@ -364,15 +364,21 @@ def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, 
 
   Every candidate scheme runs the same kernel: the coordinate form of the candidate scan when
   ``coords`` (default: when the matrix outgrows the caches, n > 4096), else the matrix form."""
-  from .context import coordinate_candidates_pay
+  from .context import coordinate_candidates_pay, int_coordinates
   torch = _native.require_device()
-  xy = torch.from_numpy(uniform_coords(n, seed)).cuda()
+  xy_host = uniform_coords(n, seed)
+  xy = torch.from_numpy(xy_host).cuda()
+  ixy = int_coordinates(xy_host)
+  # the candidate scans' coordinates: int32 for the integer distance form when the instance allows
+  scan_xy = torch.from_numpy(ixy).cuda() if ixy is not None else xy
   D = torch.empty((n, n), dtype=torch.int32, device="cuda")
   _native.dist_euc2d(xy, D)
   start = greedy_start(D, P, seed + 1)
   tables = candidate_tables(D, xy, scorer, k_learned)
   coords = coordinate_candidates_pay(n) if coords is None else bool(coords)
-  res = {"n": n, "P": P, "gap": gap, "candidate_kernel": "coordinates" if coords else "matrix", "methods": {}}
+  res = {"n": n, "P": P, "gap": gap, "methods": {},
+         "candidate_kernel": ("coordinates, " + ("integer form" if ixy is not None else "FP64 form")) if coords
+                             else "matrix"}
   tick = None
   if progress:
     tick = lambda p, m: progress("full-chunk", {"seconds": round(p[0], 2), "best": p[1], "moves": m})  # noqa: E731
@@ -387,7 +393,7 @@ def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, 
   for name, (tab, k, prep) in tables.items():
     if coords:
       # one launch to the local optimum, the population's best known after every move
-      log, tours, wall, hz = ls_logged(start, xy, tab, k)
+      log, tours, wall, hz = ls_logged(start, scan_xy, tab, k)
       cost = torch.empty(P, dtype=torch.float64, device="cuda")
       _native.tour_cost(tours, D, cost)
       final = cost.cpu().numpy()
