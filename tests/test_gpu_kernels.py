@@ -422,7 +422,7 @@ def test_rank_select_matches_stable_argsort(torch_mod, nat, M, keep):
   np.testing.assert_array_equal(order.cpu().numpy(), np.argsort(keys, kind="stable")[:keep])
 
 
-@pytest.mark.parametrize("n", [1, 21, 1000, 1003])
+@pytest.mark.parametrize("n", [1, 21, 1000, 1003, 4100])
 def test_gather_rows_widths(torch_mod, nat, n):
   """Rows of every width, 16-byte copies (n % 4 == 0) and element copies alike."""
   say(f"gather_rows n={n}")

@@ -99,21 +99,27 @@ __global__ __launch_bounds__(kRowBlock) void generation_kernel(
 // Stable ascending order of M float64 keys: order[rank(i)] = i for rank(i) < keep, with
 // rank(i) = #{j : (b_j, j) < (b_i, i)} over the orderable key bits b (the total order the bitonic
 // network of the batch survival kernel sorts by, so both select the same rows).  A workgroup owns
-// 64 keys, one per lane; the keys pass through LDS in tiles of kCountTile (already as orderable
-// bits), its 16 waves split each tile, every lane of a wave reading the same key j (a broadcast,
-// no bank conflict), and the per-wave counts are added at the end.  A 2048-key selection is one
-// short pass instead of 66 sorting-network stages.
+// KEYS keys; lane L of a wave holds key L mod KEYS and counts over its own share of the j range
+// (the 64 / KEYS lane groups of a wave and the 16 waves split it), so a small selection still
+// spreads over many CUs: 2048 keys are 256 workgroups of 8 keys, 16 comparisons per lane, where
+// 32 workgroups of 64 keys (128 comparisons per lane, a VALU-bound 8.6 us) used 32 CUs.  The keys
+// pass through LDS in tiles of kCountTile (already as orderable bits); the lanes of a group read
+// the same key j (a broadcast), and the counts are added per key at the end.
 constexpr int kCountBlock = 1024;
 constexpr int kCountWaves = kCountBlock / kWave;
 constexpr int kCountTile = 4096;
 
+template <int KEYS>
 __global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* __restrict__ keys, int M, int keep,
                                                                  int32_t* __restrict__ order) {
+  static_assert(kWave % KEYS == 0, "keys per workgroup must divide the wave");
+  constexpr int GROUPS = kCountWaves * (kWave / KEYS);  // j-range shares per tile
   __shared__ uint64_t tile[kCountTile];
   __shared__ int part[kCountWaves][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
-  const int i = blockIdx.x * kWave + lane;
+  const int g = wid * (kWave / KEYS) + lane / KEYS;  // this lane's share
+  const int i = blockIdx.x * KEYS + (lane & (KEYS - 1));
   const uint64_t bi = i < M ? orderable_bits(keys[i]) : 0ull;
   int cnt = 0;
   for (int base = 0; base < M; base += kCountTile) {
@@ -121,8 +127,8 @@ __global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* _
     __syncthreads();
     for (int x = threadIdx.x; x < len; x += kCountBlock) tile[x] = orderable_bits(keys[base + x]);
     __syncthreads();
-    const int chunk = (len + kCountWaves - 1) / kCountWaves;
-    const int j0 = wid * chunk;
+    const int chunk = (len + GROUPS - 1) / GROUPS;
+    const int j0 = g * chunk;
     const int j1 = (j0 + chunk) < len ? (j0 + chunk) : len;
     const int ib = i - base;  // j < i  <=>  x < ib
 #pragma unroll 4
@@ -131,9 +137,11 @@ __global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* _
       cnt += (bj < bi) | ((bj == bi) & (x < ib));
     }
   }
+#pragma unroll
+  for (int m = KEYS; m < kWave; m <<= 1) cnt += __shfl_xor(cnt, m, kWave);  // the wave's groups, per key
   part[wid][lane] = cnt;
   __syncthreads();
-  if (wid == 0 && i < M) {
+  if (wid == 0 && lane < KEYS && i < M) {
     int rank = 0;
 #pragma unroll
     for (int w = 0; w < kCountWaves; ++w) rank += part[w][lane];
@@ -155,9 +163,21 @@ __global__ __launch_bounds__(kRowBlock) void gather_rows_kernel(const int32_t* _
   const int s = src[d];
   const int32_t* row = s < P0 ? A + (size_t)s * n : B + (size_t)(s - P0) * n;
   if (vec) {
+    // four 16-byte loads per lane in flight before any store: one memory round trip per 4 KB row
     const int4* r4 = reinterpret_cast<const int4*>(row);
     int4* o4 = reinterpret_cast<int4*>(out + (size_t)d * n);
-    for (int x = lane; x < (n >> 2); x += kWave) o4[x] = r4[x];
+    const int n4 = n >> 2;
+    for (int x0 = lane; x0 < n4; x0 += 4 * kWave) {
+      int4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {  // unconditional (clamped) loads, so all four are issued together
+        const int x = x0 + u * kWave;
+        v[u] = r4[x < n4 ? x : n4 - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (x0 + u * kWave < n4) o4[x0 + u * kWave] = v[u];
+    }
   } else {
     for (int x = lane; x < n; x += kWave) out[(size_t)d * n + x] = row[x];
   }
@@ -312,8 +332,14 @@ extern "C" int tspmi_rank_select(const double* keys, int M, int keep, int32_t* o
   if (M == 0 || keep == 0) return TSPMI_OK;
   TSPMI_CHECK_ARG(keys && order, "tspmi_rank_select: null pointer");
   TSPMI_CHECK_ARG(M <= (1 << 24), "tspmi_rank_select: M=%d above 2^24", M);
-  hipLaunchKernelGGL(rank_count_kernel, dim3((M + kWave - 1) / kWave), dim3(kCountBlock), 0, (hipStream_t)stream, keys,
-                     M, keep, order);
+  // small selections: few keys per workgroup, so that the counting spreads over the CUs; large
+  // ones: 64 keys per workgroup, so that each workgroup's pass over all M keys serves more of them
+  if (M <= 8192)
+    hipLaunchKernelGGL(rank_count_kernel<8>, dim3((M + 7) / 8), dim3(kCountBlock), 0, (hipStream_t)stream, keys, M, keep,
+                       order);
+  else
+    hipLaunchKernelGGL(rank_count_kernel<kWave>, dim3((M + kWave - 1) / kWave), dim3(kCountBlock), 0,
+                       (hipStream_t)stream, keys, M, keep, order);
   TSPMI_CHECK_LAUNCH("tspmi_rank_select launch");
   return TSPMI_OK;
 }
