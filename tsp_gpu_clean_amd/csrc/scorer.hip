@@ -54,17 +54,23 @@ __device__ __forceinline__ bf16x8 pack_slots(const float* v) {
                               pack_bf16x2(v[6], v[7])));
 }
 
+// Row stride of W2 and W2^T in LDS, in bf16: 8 past the row, so that the 32 rows a half-wave reads
+// for one MFMA operand start 16 bytes apart in the banks instead of all in one (at a stride of
+// 128 elements every lane of the half-wave hit the same bank: 90 % of the forward kernel's LDS
+// cycles were bank conflicts).
+constexpr int kW2Ld = kH + 8;
+
 // Weights of the forward pass in LDS (bf16 matrices, fp32 biases).
 struct alignas(16) FwdLds {
   uint16_t w1[kH * kF];
-  uint16_t w2[kH * kH];
+  uint16_t w2[kH * kW2Ld];
   float b1[kH], b2[kH], w3[kH];
   float b3;
 };
 
 // Forward weights plus W2 transposed for the backward pass.
 struct alignas(16) ScorerLds : FwdLds {
-  uint16_t w2t[kH * kH];
+  uint16_t w2t[kH * kW2Ld];
 };
 
 // The weights into LDS with 16-byte loads, every load of a thread issued before it stores any:
@@ -93,11 +99,18 @@ __device__ void load_fwd(FwdLds& L, const uint16_t* __restrict__ wbf, uint16_t* 
   }
   if (t == 0) b3 = bf2f(wbf[kOffB3]);
   reinterpret_cast<uint4*>(L.w1)[t] = r1;
+  // 16 uint4 per 128-element row; row r starts at r * kW2Ld elements (16-byte aligned)
 #pragma unroll
-  for (int e = 0; e < E2; ++e) reinterpret_cast<uint4*>(L.w2)[t + e * kMlpBlock] = r2[e];
+  for (int e = 0; e < E2; ++e) {
+    const int v = t + e * kMlpBlock;
+    reinterpret_cast<uint4*>(L.w2 + (v >> 4) * kW2Ld)[v & 15] = r2[e];
+  }
   if (w2t != nullptr) {
 #pragma unroll
-    for (int e = 0; e < E2; ++e) reinterpret_cast<uint4*>(w2t)[t + e * kMlpBlock] = rt[e];
+    for (int e = 0; e < E2; ++e) {
+      const int v = t + e * kMlpBlock;
+      reinterpret_cast<uint4*>(w2t + (v >> 4) * kW2Ld)[v & 15] = rt[e];
+    }
   }
   if (t < kH) {
     L.b1[t] = b1;
@@ -160,7 +173,7 @@ __device__ __forceinline__ void forward32(const FwdLds& L, const uint16_t* __res
     for (int T = 0; T < 4; ++T)
 #pragma unroll
       for (int c = 0; c < 2; ++c)
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2, kH, 32 * To + s, 32 * T + 16 * c + 4 * h), bop[T][c],
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2, kW2Ld, 32 * To + s, 32 * T + 16 * c + 4 * h), bop[T][c],
                                                       acc, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -201,7 +214,7 @@ __device__ __forceinline__ float forward32_z(const FwdLds& L, const uint16_t* __
     for (int T = 0; T < 4; ++T)
 #pragma unroll
       for (int c = 0; c < 2; ++c)
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2, kH, 32 * To + s, 32 * T + 16 * c + 4 * h), bop[T][c],
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2, kW2Ld, 32 * To + s, 32 * T + 16 * c + 4 * h), bop[T][c],
                                                       acc, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -288,7 +301,7 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_backward_kernel(const uint16_t*
         for (int To = 0; To < 4; ++To)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2t, kH, 32 * Ti + s, 32 * To + 16 * c + 4 * h),
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_perm(L.w2t, kW2Ld, 32 * Ti + s, 32 * To + 16 * c + 4 * h),
                                                           bop[To][c], acc, 0, 0, 0);
         if (valid) {
 #pragma unroll
@@ -341,18 +354,35 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __
                                                               const uint16_t* __restrict__ a1T,
                                                               const uint16_t* __restrict__ d2T,
                                                               const uint16_t* __restrict__ d1T, float* __restrict__ part) {
+  // the chunk's feature rows, read by every unit's dW1 thread below: staged once (8 KB) instead of
+  // 128 loads of each row from the cache
+  __shared__ uint4 sx[kGradChunk * kF / 8];
   const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
   const int s = lane & 31, h = lane >> 5;
   const int lo = blockIdx.x * kGradChunk;
   const int hi = min(lo + kGradChunk, S);
   float* out = part + (size_t)blockIdx.x * kOffW3;  // dW1, db1, dW2, db2 (W3/b3 come from pass 1)
+  for (int v = threadIdx.x; v < (hi - lo) * (kF / 8); v += kMlpBlock)
+    sx[v] = reinterpret_cast<const uint4*>(X + (size_t)lo * kF)[v];
   // dW2 row-tile To = wid, column tiles Ti = 0..3
   f32x16 acc[4] = {};
-  for (int k0 = lo; k0 < hi; k0 += 16) {
-    // operands for samples k0 + 8h .. +7 (zero past the end)
+  const int o = 32 * wid + s;
+  // whole 16-sample steps without bounds tests, four unrolled so their loads overlap
+  const int full = lo + ((hi - lo) & ~15);
+#pragma unroll 4
+  for (int k0 = lo; k0 < full; k0 += 16) {
+    const int kk = k0 + 8 * h;
+    const uint4 av = *reinterpret_cast<const uint4*>(d2T + (size_t)o * S + kk);
+    uint4 bv[4];
+#pragma unroll
+    for (int Ti = 0; Ti < 4; ++Ti) bv[Ti] = *reinterpret_cast<const uint4*>(a1T + (size_t)(32 * Ti + s) * S + kk);
+#pragma unroll
+    for (int Ti = 0; Ti < 4; ++Ti)
+      acc[Ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(av), as_bf16x8(bv[Ti]), acc[Ti], 0, 0, 0);
+  }
+  for (int k0 = full; k0 < hi; k0 += 16) {  // the last partial step (8 samples), zero past the end
     const int kk = k0 + 8 * h;
     uint4 av = make_uint4(0, 0, 0, 0);
-    const int o = 32 * wid + s;
     if (kk + 8 <= hi) av = *reinterpret_cast<const uint4*>(d2T + (size_t)o * S + kk);
     else {
       uint16_t t[8];
@@ -384,6 +414,8 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __
   const bool odd = (threadIdx.x & 1) != 0;
   float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float gb = 0.0f;  // db1 (even threads) or db2 (odd threads)
+  __syncthreads();  // sx staged
+#pragma unroll 4
   for (int k = lo; k < hi; k += 8) {
     const uint4 dv = *reinterpret_cast<const uint4*>(d1T + (size_t)u * S + k);
     const uint4 ev = *reinterpret_cast<const uint4*>((odd ? d2T : d1T) + (size_t)u * S + k);
@@ -393,7 +425,7 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __
     for (int e = 0; e < 8; ++e) {
       const float d1 = bf2f((uint16_t)(ds[e >> 1] >> (16 * (e & 1))));
       gb += bf2f((uint16_t)(es[e >> 1] >> (16 * (e & 1))));
-      const uint4 xv = *reinterpret_cast<const uint4*>(X + (size_t)(k + e) * kF + f0);
+      const uint4 xv = sx[(k - lo + e) * (kF / 8) + (f0 >> 3)];
       const uint32_t xs[4] = {xv.x, xv.y, xv.z, xv.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -408,26 +440,41 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_wgrad_kernel(const uint16_t* __
 }
 
 // grad = sum of the per-block partials, returning the loss in grad[kScorerParams].  A workgroup
-// covers 64 parameters with 4 lanes each; lane group y sums partial sets y, y+4, y+8, ... and the
-// four group sums are added in a fixed order, so the result does not depend on scheduling.
+// covers 64 parameters with 16 lanes each; lane group y sums partial sets y, y+16, y+32, ... (in
+// that order, eight loads in flight at a time) and the sixteen group sums are added in a fixed
+// order, so the result does not depend on scheduling.
 constexpr int kSumParams = 64;
+constexpr int kSumGroups = 16;
 
-__global__ __launch_bounds__(256) void mlp_grad_sum_kernel(const float* __restrict__ wpart, int nw,
-                                                           const float* __restrict__ p1part, int n1,
-                                                           float* __restrict__ grad) {
-  __shared__ float acc[4][kSumParams];
+__global__ __launch_bounds__(kSumParams * kSumGroups) void mlp_grad_sum_kernel(const float* __restrict__ wpart, int nw,
+                                                                            const float* __restrict__ p1part, int n1,
+                                                                            float* __restrict__ grad) {
+  __shared__ float acc[kSumGroups][kSumParams];
   const int tx = threadIdx.x % kSumParams, ty = threadIdx.x / kSumParams;
   const int x = blockIdx.x * kSumParams + tx;
   float v = 0.0f;
   if (x < kOffW3) {
-    for (int b = ty; b < nw; b += 4) v += wpart[(size_t)b * kOffW3 + x];
+    int b = ty;
+    for (; b + 7 * kSumGroups < nw; b += 8 * kSumGroups) {
+      float t[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) t[q] = wpart[(size_t)(b + kSumGroups * q) * kOffW3 + x];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v += t[q];
+    }
+    for (; b < nw; b += kSumGroups) v += wpart[(size_t)b * kOffW3 + x];
   } else if (x < kScorerParams + 1) {
     const int t = x - kOffW3;  // w3 [0, 128), b3 128, loss 129
-    for (int b = ty; b < n1; b += 4) v += p1part[(size_t)b * (kH + 2) + t];
+    for (int b = ty; b < n1; b += kSumGroups) v += p1part[(size_t)b * (kH + 2) + t];
   }
   acc[ty][tx] = v;
   __syncthreads();
-  if (ty == 0 && x < kScorerParams + 1) grad[x] = ((acc[0][tx] + acc[1][tx]) + acc[2][tx]) + acc[3][tx];
+  if (ty == 0 && x < kScorerParams + 1) {
+    float r = acc[0][tx];
+#pragma unroll
+    for (int y = 1; y < kSumGroups; ++y) r += acc[y][tx];
+    grad[x] = r;
+  }
 }
 
 // Adam on the fp32 master weights, then refresh the bf16 copy (padding features stay zero).
@@ -591,7 +638,8 @@ extern "C" int tspmi_mlp_grad(const uint16_t* X, const float* y, int S, const ui
   hipLaunchKernelGGL(mlp_backward_kernel, dim3(n1), dim3(kMlpBlock), 0, s, X, y, S, wbf, w2t, 1.0f / (float)S, a1T, d2T,
                      d1T, p1);
   hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(nw), dim3(kMlpBlock), 0, s, X, S, a1T, d2T, d1T, pw);
-  hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3((kScorerParams + 1 + kSumParams - 1) / kSumParams), dim3(256), 0, s, pw,
+  hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3((kScorerParams + 1 + kSumParams - 1) / kSumParams),
+                     dim3(kSumParams * kSumGroups), 0, s, pw,
                      nw, p1, n1, grad);
   TSPMI_CHECK_LAUNCH("tspmi_mlp_grad launch");
   return TSPMI_OK;
