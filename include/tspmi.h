@@ -350,6 +350,18 @@ int tspmi_gather_rows(const int32_t* A, const double* fitA, int P0, const int32_
                       int n, const int32_t* src, int rows, int32_t* out, double* out_fit, void* stream);
 
 /*
+ * (mu + lambda) survival in one launch: tspmi_rank_select over keys (M float64: the fitness of
+ * A's P0 rows, then of B's M - P0 rows), then the survivors' rows in stable rank order,
+ * out[r] = row order[r] of A (order[r] < P0) or of B, with out_fit[r] = keys[order[r]] and
+ * order[r] itself (out_fit and order may be NULL), for r < keep.  Same result as tspmi_rank_select
+ * followed by tspmi_gather_rows, without the second launch.  out must not overlap A or B.
+ * Replaces: _select_survivors_assessed, genetic_algorithm.py:618-706 (lexsort, then the
+ *           combined_pop[best_indices] gather).
+ */
+int tspmi_survive(const double* keys, int M, int keep, const int32_t* A, int P0, const int32_t* B, int n,
+                  int32_t* out, double* out_fit, int32_t* order, void* stream);
+
+/*
  * Row scatter: dst[idx[r]] = src[r] (and dst_fit[idx[r]] = src_fit[r] when non-NULL).
  * Replaces: the elite write-back of pipeline_steps.py:183-184.
  */

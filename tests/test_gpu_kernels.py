@@ -434,6 +434,29 @@ def test_gather_rows_widths(torch_mod, nat, n):
   np.testing.assert_array_equal(out.cpu().numpy(), np.vstack([A, B])[src])
 
 
+@pytest.mark.parametrize("P0,PB,keep,n", [(1024, 1024, 1024, 1000), (16, 16, 16, 52), (7, 5, 9, 21), (3, 0, 2, 8),
+                                          (5000, 5000, 4000, 12), (128, 130, 100, 1003)])
+def test_survive_equals_rank_select_and_gather(torch_mod, nat, P0, PB, keep, n):
+  """The one-launch survival leaves the rows, keys and source order that the stable rank followed
+  by the row gather leave (numpy's stable argsort of the keys, ties and NaNs included), with
+  16-byte and element copies, and above the small-selection size (M > 8192)."""
+  say(f"survive P0={P0} PB={PB} keep={keep} n={n}")
+  rng = np.random.default_rng(P0 + PB + n)
+  M = P0 + PB
+  keys = rng.integers(0, 60, M).astype(np.float64)
+  keys[rng.random(M) < 0.05] = np.nan
+  A = _population(n, P0, 5)
+  B = _population(n, max(PB, 1), 6)[:PB]
+  out = torch_mod.full((keep, n), -1, dtype=torch_mod.int32, device="cuda")
+  out_fit = torch_mod.empty(keep, dtype=torch_mod.float64, device="cuda")
+  order = torch_mod.empty(keep, dtype=torch_mod.int32, device="cuda")
+  nat.survive(cu(keys), keep, cu(A), cu(B) if PB else None, out, out_fit, order)
+  want = np.argsort(keys, kind="stable")[:keep]
+  np.testing.assert_array_equal(order.cpu().numpy(), want)
+  np.testing.assert_array_equal(out.cpu().numpy(), np.vstack([A, B])[want] if PB else A[want])
+  np.testing.assert_array_equal(out_fit.cpu().numpy(), keys[want])
+
+
 def test_gather_and_scatter_rows(torch_mod, nat):
   say("gather/scatter rows")
   A = _population(20, 6, 1)

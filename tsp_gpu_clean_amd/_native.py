@@ -50,6 +50,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_generation": [_P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P],
   "tspmi_draw_generations": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
   "tspmi_rank_select": [_P, _I, _I, _P, _P],
+  "tspmi_survive": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P],
   "tspmi_gather_rows": [_P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P],
   "tspmi_scatter_rows": [_P, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_random_init": [_P, _I, _P, _I, _P, _P],
@@ -508,6 +509,32 @@ def rank_select(keys, keep: int, order) -> None:
   if order.shape[0] < keep:
     raise ValueError("rank_select: order shorter than keep")
   _check(_lib.tspmi_rank_select(_ptr(keys), keys.shape[0], int(keep), _ptr(order), _stream(torch)), "tspmi_rank_select")
+
+
+def survive(keys, keep: int, A, B, out, out_fit=None, order=None) -> None:
+  """(mu + lambda) survival in one launch: ``out`` (keep, n) receives the rows of ``A`` then ``B``
+  (``keys`` holds their fitness in that order) in stable rank order, ``out_fit`` their keys and
+  ``order`` their source indices; as :func:`rank_select` followed by :func:`gather_rows`."""
+  torch = require_device()
+  _need(keys, torch.float64, "keys", 1)
+  _need(A, torch.int32, "A", 2)
+  _need(out, torch.int32, "out", 2)
+  n = A.shape[1]
+  M, P0 = keys.shape[0], A.shape[0]
+  if B is not None:
+    _need(B, torch.int32, "B", 2)
+    if B.shape[1] != n or P0 + B.shape[0] < M:
+      raise ValueError("survive: B does not match A / keys")
+  elif P0 < M:
+    raise ValueError("survive: keys beyond A's rows need B")
+  if out.shape[0] < keep or out.shape[1] != n:
+    raise ValueError("survive: out is not (keep, n)")
+  if out_fit is not None:
+    _need(out_fit, torch.float64, "out_fit", 1)
+  if order is not None:
+    _need(order, torch.int32, "order", 1)
+  _check(_lib.tspmi_survive(_ptr(keys), M, int(keep), _ptr(A), P0, _ptr(B), n, _ptr(out), _ptr(out_fit), _ptr(order),
+                            _stream(torch)), "tspmi_survive")
 
 
 def gather_rows(A, fitA, B, fitB, src, out, out_fit=None) -> None:

@@ -109,13 +109,13 @@ constexpr int kCountBlock = 1024;
 constexpr int kCountWaves = kCountBlock / kWave;
 constexpr int kCountTile = 4096;
 
+// Stable rank of this workgroup's KEYS keys (blockIdx.x * KEYS + lane for lanes < KEYS of wave 0,
+// which get the result; other lanes get partial counts).  Every thread must call it.
 template <int KEYS>
-__global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* __restrict__ keys, int M, int keep,
-                                                                 int32_t* __restrict__ order) {
+__device__ __forceinline__ int count_rank(const double* __restrict__ keys, int M, uint64_t* tile,
+                                          int (*part)[kWave]) {
   static_assert(kWave % KEYS == 0, "keys per workgroup must divide the wave");
   constexpr int GROUPS = kCountWaves * (kWave / KEYS);  // j-range shares per tile
-  __shared__ uint64_t tile[kCountTile];
-  __shared__ int part[kCountWaves][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const int g = wid * (kWave / KEYS) + lane / KEYS;  // this lane's share
@@ -141,11 +141,72 @@ __global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* _
   for (int m = KEYS; m < kWave; m <<= 1) cnt += __shfl_xor(cnt, m, kWave);  // the wave's groups, per key
   part[wid][lane] = cnt;
   __syncthreads();
-  if (wid == 0 && lane < KEYS && i < M) {
-    int rank = 0;
+  int rank = 0;
 #pragma unroll
-    for (int w = 0; w < kCountWaves; ++w) rank += part[w][lane];
-    if (rank < keep) order[rank] = i;
+  for (int w = 0; w < kCountWaves; ++w) rank += part[w][lane & (KEYS - 1)];
+  return rank;
+}
+
+template <int KEYS>
+__global__ __launch_bounds__(kCountBlock) void rank_count_kernel(const double* __restrict__ keys, int M, int keep,
+                                                                 int32_t* __restrict__ order) {
+  __shared__ uint64_t tile[kCountTile];
+  __shared__ int part[kCountWaves][kWave];
+  const int rank = count_rank<KEYS>(keys, M, tile, part);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = blockIdx.x * KEYS + lane;
+  if (threadIdx.x < KEYS && i < M && rank < keep) order[rank] = i;
+}
+
+// (mu + lambda) survival in one launch: the stable rank of every key (as rank_count_kernel), then
+// each workgroup copies its own surviving rows into place -- out[rank(i)] = row i of A (i < P0)
+// or of B (i >= P0), out_fit[rank(i)] = keys[i], order[rank(i)] = i -- with kCountWaves / KEYS
+// waves per row (16-byte copies when `vec`), so the survivors move while the count's workgroups
+// are resident instead of in a second launch.
+template <int KEYS>
+__global__ __launch_bounds__(kCountBlock) void survive_kernel(const double* __restrict__ keys, int M, int keep,
+                                                              const int32_t* __restrict__ A, int P0,
+                                                              const int32_t* __restrict__ B, int n,
+                                                              int32_t* __restrict__ out, double* __restrict__ out_fit,
+                                                              int32_t* __restrict__ order, int vec) {
+  constexpr int WPR = kCountWaves / KEYS > 0 ? kCountWaves / KEYS : 1;  // waves per row
+  __shared__ uint64_t tile[kCountTile];
+  __shared__ int part[kCountWaves][kWave];
+  __shared__ int s_rank[KEYS];
+  const int rank = count_rank<KEYS>(keys, M, tile, part);
+  if (threadIdx.x < KEYS) s_rank[threadIdx.x] = blockIdx.x * KEYS + (int)threadIdx.x < M ? rank : keep;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int sub = wid % WPR;
+  for (int k = wid / WPR; k < KEYS; k += kCountWaves / WPR) {
+    const int r = s_rank[k];
+    if (r >= keep) continue;  // uniform per wave
+    const int i = blockIdx.x * KEYS + k;
+    const int32_t* row = i < P0 ? A + (size_t)i * n : B + (size_t)(i - P0) * n;
+    int32_t* dst = out + (size_t)r * n;
+    if (vec) {
+      const int4* r4 = reinterpret_cast<const int4*>(row);
+      int4* o4 = reinterpret_cast<int4*>(dst);
+      const int n4 = n >> 2;
+      for (int x0 = sub * kWave + lane; x0 < n4; x0 += 4 * WPR * kWave) {
+        int4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {  // unconditional (clamped) loads, issued together
+          const int x = x0 + u * WPR * kWave;
+          v[u] = r4[x < n4 ? x : n4 - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (x0 + u * WPR * kWave < n4) o4[x0 + u * WPR * kWave] = v[u];
+      }
+    } else {
+      for (int x = sub * kWave + lane; x < n; x += WPR * kWave) dst[x] = row[x];
+    }
+    if (sub == 0 && lane == 0) {
+      if (out_fit != nullptr) out_fit[r] = keys[i];
+      if (order != nullptr) order[r] = i;
+    }
   }
 }
 
@@ -341,6 +402,24 @@ extern "C" int tspmi_rank_select(const double* keys, int M, int keep, int32_t* o
     hipLaunchKernelGGL(rank_count_kernel<kWave>, dim3((M + kWave - 1) / kWave), dim3(kCountBlock), 0,
                        (hipStream_t)stream, keys, M, keep, order);
   TSPMI_CHECK_LAUNCH("tspmi_rank_select launch");
+  return TSPMI_OK;
+}
+
+extern "C" int tspmi_survive(const double* keys, int M, int keep, const int32_t* A, int P0, const int32_t* B, int n,
+                             int32_t* out, double* out_fit, int32_t* order, void* stream) {
+  TSPMI_CHECK_ARG(M >= 0 && keep >= 0 && keep <= M && P0 >= 0 && P0 <= M && n >= 0,
+                  "tspmi_survive: bad sizes M=%d keep=%d P0=%d n=%d", M, keep, P0, n);
+  if (M == 0 || keep == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(keys && out && (A || P0 == 0) && (B || P0 == M), "tspmi_survive: null pointer");
+  TSPMI_CHECK_ARG(M <= (1 << 24), "tspmi_survive: M=%d above 2^24", M);
+  const int vec = (n % 4 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  if (M <= 8192)
+    hipLaunchKernelGGL(survive_kernel<8>, dim3((M + 7) / 8), dim3(kCountBlock), 0, (hipStream_t)stream, keys, M, keep, A,
+                       P0, B, n, out, out_fit, order, vec);
+  else
+    hipLaunchKernelGGL(survive_kernel<kWave>, dim3((M + kWave - 1) / kWave), dim3(kCountBlock), 0, (hipStream_t)stream,
+                       keys, M, keep, A, P0, B, n, out, out_fit, order, vec);
+  TSPMI_CHECK_LAUNCH("tspmi_survive launch");
   return TSPMI_OK;
 }
 

@@ -194,8 +194,7 @@ class DeviceMemeticLoop:
     ka, kb = self.keys[a], self.keys[b]
     _native.generation(self.pop[a], ka[:P], self.D, draws.tour_cand[g], draws.cut_lo[g], draws.cut_hi[g],
                        draws.probs[g], draws.pos1[g], draws.pos2[g], self.rate, self.off, ka[P:])
-    _native.rank_select(ka, P, self.order)
-    _native.gather_rows(self.pop[a], ka[:P], self.off, ka[P:], self.order, self.pop[b], kb[:P])
+    _native.survive(ka, P, self.pop[a], self.off, self.pop[b], kb[:P], self.order)
     best_done = False
     if self.elite:
       E = self.elite
