@@ -71,6 +71,21 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// The same minimum for a 32-bit signed value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_min_i32(int v) {
+  return min(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xf, false));
+}
+__device__ __forceinline__ int wave_min_i32(int v) {
+  v = dpp_min_i32<0xb1, 0xf>(v);
+  v = dpp_min_i32<0x4e, 0xf>(v);
+  v = dpp_min_i32<0x141, 0xf>(v);
+  v = dpp_min_i32<0x140, 0xf>(v);
+  v = dpp_min_i32<0x142, 0xa>(v);
+  v = dpp_min_i32<0x143, 0xc>(v);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
@@ -127,6 +142,14 @@ __device__ __forceinline__ uint64_t orderable_bits(double v) {
 // Key of np.argmin over doubles: a NaN propagates, so the first NaN is the minimum; other values
 // order as orderable_bits (which never returns 0 for a number: -inf maps to 2^52 - 1).
 __device__ __forceinline__ uint64_t argmin_bits(double v) { return v != v ? 0ull : orderable_bits(v); }
+
+// v unchanged, in a VGPR, through an empty asm statement: the optimiser cannot see through it, so
+// it neither reassociates the arithmetic that produced v with what consumes it nor relates two
+// values laundered separately.  Emits no instruction.
+__device__ __forceinline__ int opaque_i32(int v) {
+  asm("" : "+v"(v));
+  return v;
+}
 
 // Node ids outside [0, limit) are an invalid input; clamping keeps every access in bounds.
 __device__ __forceinline__ int clamp_id(int v, int limit) { return (unsigned)v < (unsigned)limit ? v : 0; }

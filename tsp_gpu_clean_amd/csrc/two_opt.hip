@@ -31,8 +31,8 @@ namespace tspmi {
 // and every delta costs one distance gather instead of two.  The gathers of row i+1 are issued
 // before row i is evaluated (one-row software pipeline).  The loop body is branch-free so every
 // load is unconditional and the pipeline really overlaps: s_tour and s_edge carry `pad` zero
-// entries past their ends (node 0 is a valid gather target), lanes past the row end compute a
-// harmless delta that the select discards.  The distance-row base is wave-uniform and the gather
+// entries past their ends (node 0 is a valid gather target; the edge padding is -2^30), so lanes
+// past the row end compute a large positive delta that never wins.  The distance-row base is wave-uniform and the gather
 // uses a 32-bit unsigned byte offset.  s_tour[n] mirrors s_tour[0], so the successor of position
 // n-1 needs no wrap test.
 //
@@ -43,8 +43,13 @@ namespace tspmi {
 // pass for 498,500 pairs, against 923,648 when every slot runs to the wave's last row).  The
 // snake order of off() over the waves gives every wave the same number of live slot rows.
 //
-// Each lane visits its pairs in increasing scan order, so a strict `<` keeps the first minimum per
-// lane and the packed-key block minimum then restores the global lowest-(i, j) tie-break.
+// The scan keeps only each lane's smallest delta per slot; the row of the workgroup's smallest
+// delta is recovered afterwards by walking the diagonals that reached it (scan_full_best_diag),
+// which restores the lowest-(i, j) tie-break of the reference's scan order.  Per slot and row the
+// bookkeeping is then one three-operand add (delta), one subtraction (the next row's carry, with
+// its edge cost folded in) and half a three-operand minimum, and the LDS reads of a block of rows
+// share one address per slot (DiagScan::Lanes): 740 VALU instructions per 16-row period of four
+// slots against 889 when the row of each slot minimum was tracked and every address recomputed.
 //
 // With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), rows 0, 2, 4, 5, 7, 9, 11, 12
 // and 14 of every 16 compute their new distances from the coordinates, with the function that
@@ -55,7 +60,10 @@ namespace tspmi {
 // local optimum (585 ms with no coordinate rows): FP64 form, 3 rows in 8: 430-440 ms; integer
 // form (14 instructions), k rows in 8 for k = 3..6: 391 / 320 / 313 / 342 ms, 7 and 9 rows in 16:
 // 342 / 308 ms; with a 19-instruction integer form, every other row with a period of 2 / 4 / 8 /
-// 16 rows: 378 / 371 / 365 / 363 ms (the longer unrolled period hides more latency).
+// 16 rows: 378 / 371 / 365 / 363 ms (the longer unrolled period hides more latency).  With the
+// lighter bookkeeping (minimum only, shared addresses, rows fenced for the scheduler): 8 / 9 / 10
+// / 11 / 12 rows in 16: 304 / 277 / 276 / 286 / 295 ms, now with the texture addresser 92 % busy
+// and VALU 78 %.
 constexpr int kMixPeriod = 16;
 constexpr int kMixRows = 0b0101101010110101;
 
@@ -87,54 +95,87 @@ __device__ __forceinline__ int diag_slot_offset(int w, int q) {
 
 template <int QMAX, bool MIX = false, class XY = XyF64>
 struct DiagScan {
+  using T = typename XY::T;
   const int32_t* s_tour;
   const int32_t* s_edge;
-  const typename XY::T* s_xy;  // MIX only
+  const T* s_xy;  // MIX only
   const int32_t* __restrict__ D;
   int ld, n, wid;
-  int jo_even, jo_odd;  // j - i of the lane in even / odd slots, without the 256 q term
-  int carry[QMAX], gcur[QMAX], bd[QMAX], bi[QMAX];
+  int jo_even, jo_odd;  // j - i of the lane in slots 0 and 1; slot q adds 512 * (q >> 1)
+  // carry: D[t_i, t_j] - e_j of the row about to be evaluated; gcur: D[t_{i+1}, t_{j+1}];
+  // bd: the slot's smallest delta (its row is found after the scan).
+  int carry[QMAX], gcur[QMAX], bd[QMAX];
 
-  __device__ __forceinline__ int jo(int q) const { return ((q & 1) ? jo_odd : jo_even) + q * kTwoOptBlock; }
+  static constexpr int kSlotStride = 2 * kTwoOptBlock;
+  __device__ __forceinline__ int jo(int q) const { return ((q & 1) ? jo_odd : jo_even) + (q >> 1) * kSlotStride; }
 
-  // Row i over the QL lowest slots; FIRST marks row 0, whose pair (0, n-1) is excluded; COORD
-  // takes the row's new distances from the coordinates.
-  template <int QL, bool FIRST, bool COORD = false>
-  __device__ __forceinline__ void row(int i) {
-    int gnext[QL];
-    if constexpr (COORD && XY::kTourOrder) {
-      const typename XY::T pa = s_xy[i + 2];
-#pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[i + jo(q) + 2]);
-    } else if constexpr (COORD) {
-      const typename XY::T pa = s_xy[__builtin_amdgcn_readfirstlane(s_tour[i + 2])];
-#pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[s_tour[i + jo(q) + 2]]);
-    } else {
-      const int a = __builtin_amdgcn_readfirstlane(s_tour[i + 2]);
-      const int32_t* __restrict__ rn = D + (size_t)a * ld;
-#pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, s_tour[i + jo(q) + 2]);
-    }
-    const int neg_eab = -s_edge[i];
+  // The lane's LDS positions at row i, per slot: row i + R reads entry [R] from them, a constant
+  // that the LDS instruction takes as its immediate offset, so a block of rows shares one address
+  // computation per array and slot.  The slot's index is made opaque first: otherwise the
+  // compiler pairs the reads of slots q and q + 2 (512 entries apart) into one two-address LDS
+  // instruction, whose offsets are multiples of 64 entries, and then spends a VALU add per pair
+  // and row on the address.
+  struct Lanes {
+    const int32_t* tour[QMAX];  // &s_tour[i + jo + 2]
+    const int32_t* edge[QMAX];  // &s_edge[i + jo + 1]
+    const T* xy[QMAX];          // &s_xy[i + jo + 2] (tour-ordered coordinates)
+  };
+  template <int QL>
+  __device__ __forceinline__ Lanes lanes(int i) const {
+    Lanes b;
 #pragma unroll
     for (int q = 0; q < QL; ++q) {
-      int delta = carry[q] + gcur[q] + neg_eab - s_edge[i + jo(q)];
+      const int x = opaque_i32(i + jo(q));
+      b.tour[q] = s_tour + x + 2;
+      b.edge[q] = s_edge + x + 1;
+      if constexpr (MIX && XY::kTourOrder) b.xy[q] = s_xy + x + 2;
+    }
+    return b;
+  }
+
+  // Row i + R over the QL lowest slots; FIRST marks row 0, whose pair (0, n-1) is excluded; COORD
+  // takes the row's new distances from the coordinates.  delta = carry + gcur - e_i, one
+  // three-operand add, and the next row's carry folds in its e_j with one subtraction.
+  template <int QL, bool FIRST, bool COORD = false>
+  __device__ __forceinline__ void row(const Lanes& b, int i, int R) {
+    // MIX: no instruction crosses a row boundary.  Left free, the scheduler hoists the LDS reads
+    // and distance arithmetic of later rows of the 16-row period and needs 157 VGPRs (3 waves
+    // per SIMD instead of the 4 that four resident workgroups per CU need); with the rows fenced
+    // it needs about 90, and the launch runs 281 ms instead of 310 at n = 1000, P = 1024.
+    if constexpr (MIX) __builtin_amdgcn_sched_barrier(0);
+    const int r = i + R;
+    int gnext[QL];
+    if constexpr (COORD && XY::kTourOrder) {
+      const T pa = s_xy[r + 2];
+#pragma unroll
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, b.xy[q][R]);
+    } else if constexpr (COORD) {
+      const T pa = s_xy[__builtin_amdgcn_readfirstlane(s_tour[r + 2])];
+#pragma unroll
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[b.tour[q][R]]);
+    } else {
+      const int a = __builtin_amdgcn_readfirstlane(s_tour[r + 2]);
+      const int32_t* __restrict__ rn = D + (size_t)a * ld;
+#pragma unroll
+      for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, b.tour[q][R]);
+    }
+    const int neg_eab = opaque_i32(-s_edge[r]);  // one negation per row, then a three-operand add
+#pragma unroll
+    for (int q = 0; q < QL; ++q) {
+      int delta = carry[q] + gcur[q] + neg_eab;
       if (FIRST) delta = (jo(q) < n - 1) ? delta : 1;
-      const bool take = delta < bd[q];
-      bd[q] = take ? delta : bd[q];
-      bi[q] = take ? i : bi[q];
-      carry[q] = gcur[q];
+      bd[q] = min(bd[q], delta);
+      carry[q] = opaque_i32(gcur[q] - b.edge[q][R]);  // kept: else the sum is reassociated
       gcur[q] = gnext[q];
     }
   }
 
   // Rows i + R .. i + kMixPeriod - 1 of a MIX period.
   template <int QL, int R = 0>
-  __device__ __forceinline__ void mix_rows(int i) {
+  __device__ __forceinline__ void mix_rows(const Lanes& b, int i) {
     if constexpr (R < kMixPeriod) {
-      row<QL, false, ((kMixRows >> R) & 1) != 0>(i + R);
-      mix_rows<QL, R + 1>(i);
+      row<QL, false, ((kMixRows >> R) & 1) != 0>(b, i, R);
+      mix_rows<QL, R + 1>(b, i);
     }
   }
 
@@ -144,22 +185,63 @@ struct DiagScan {
     if constexpr (QL > 0) {
       const int end = (n - 2) - diag_slot_offset(wid, QL - 1);
       if constexpr (MIX) {
-        for (; i + kMixPeriod - 1 < end; i += kMixPeriod) mix_rows<QL>(i);
+        for (; i + kMixPeriod - 1 < end; i += kMixPeriod) mix_rows<QL>(lanes<QL>(i), i);
       }
       for (; i + 1 < end; i += 2) {  // two rows per trip: the carry moves become register renames
-        row<QL, false>(i);
-        row<QL, false>(i + 1);
+        const Lanes b = lanes<QL>(i);
+        row<QL, false>(b, i, 0);
+        row<QL, false>(b, i, 1);
       }
-      if (i < end) row<QL, false>(i++);
+      if (i < end) {
+        row<QL, false>(lanes<QL>(i), i, 0);
+        ++i;
+      }
       phases<QL - 1>(i);
     }
   }
 };
 
+// Scan index i * n + j of the first pair, in lane L's diagonal of slot q of this wave, whose delta
+// is dstar: the wave walks that diagonal 64 rows at a time over exactly the rows the scan gave the
+// slot (row 0, then rows 1 .. (n - 2) - off(w, q) - 1), recomputing each delta from the matrix
+// (the same values the scan used, gathered or computed); ~0u if none (not reached for a lane whose
+// slot minimum is dstar).
+__device__ __forceinline__ uint32_t diag_first_row(const int32_t* s_tour, const int32_t* s_edge, int n,
+                                                const int32_t* __restrict__ D, int ld, int wid, int lane, int q,
+                                                int L, int dstar) {
+  const int off = diag_slot_offset(wid, q);
+  const int jo = 2 + off + L;
+  const int end = (n - 2) - off;
+  const int last = end > 1 ? end : 1;
+  for (int r0 = 0; r0 < last; r0 += kWave) {
+    const int i = r0 + lane;
+    const int j = i + jo;
+    const bool ok = i < last && j <= n - 1 && !(i == 0 && j == n - 1);
+    int delta = 1;
+    if (ok) {
+      const int a = s_tour[i], b = s_tour[i + 1], c = s_tour[j], d = s_tour[j + 1];
+      delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
+    }
+    const uint64_t hit = __ballot(ok && delta == dstar);
+    if (hit != 0) {
+      const int f = r0 + __ffsll((unsigned long long)hit) - 1;
+      return (uint32_t)f * (uint32_t)n + (uint32_t)(f + jo);
+    }
+  }
+  return ~0u;
+}
+
+// The pass's best move as a packed (delta, scan index) key, the same in every thread of the
+// workgroup (kNoMove if no delta is negative).  The scan keeps only each slot's smallest delta
+// (one instruction per pair instead of a compare and two selects that would also track its row);
+// the workgroup's smallest delta dstar is then located: every lane whose slot minimum is dstar has
+// its diagonal walked again by its wave (diag_first_row), normally one lane in the workgroup, and
+// the lowest scan index among them is the reference's first minimum in scan order.  red holds
+// 2 * waves ints.
 template <int QMAX, bool MIX = false, class XY = XyF64>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane,
-                                                        const typename XY::T* s_xy = nullptr) {
+                                                        int* red, const typename XY::T* s_xy = nullptr) {
   DiagScan<QMAX, MIX, XY> sc;
   sc.s_tour = s_tour;
   sc.s_edge = s_edge;
@@ -169,31 +251,50 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
   sc.n = n;
   sc.wid = wid;
   sc.jo_even = 2 + diag_slot_offset(wid, 0) + lane;
-  sc.jo_odd = 2 + diag_slot_offset(wid, 1) - kTwoOptBlock + lane;
+  sc.jo_odd = 2 + diag_slot_offset(wid, 1) + lane;
   {
     const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
     const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int j = sc.jo(q);
-      sc.carry[q] = row_at(r0, s_tour[j]);
+      sc.carry[q] = row_at(r0, s_tour[j]) - s_edge[j];
       sc.gcur[q] = row_at(r1, s_tour[j + 1]);
       sc.bd[q] = 0;
-      sc.bi[q] = 0;
     }
   }
-  sc.template row<QMAX, true>(0);
+  sc.template row<QMAX, true>(sc.template lanes<QMAX>(0), 0, 0);
   sc.template phases<QMAX>(1);
-  uint64_t best = kNoMove;
+  int m = sc.bd[0];
 #pragma unroll
+  for (int q = 1; q < QMAX; ++q) m = min(m, sc.bd[q]);
+  m = wave_min_i32(m);
+  // red[0 .. waves) is last read in the previous pass, before barriers of the move
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  int dstar = red[0];
+  for (int w = 1; w < kTwoOptWaves; ++w) dstar = min(dstar, red[w]);
+  if (dstar >= 0) return kNoMove;  // the same in every thread
+  uint32_t hit_slots = 0;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) hit_slots |= (sc.bd[q] == dstar ? 1u : 0u) << q;
+  uint32_t idx = ~0u;
   for (int q = 0; q < QMAX; ++q) {
-    if (sc.bd[q] < 0) {
-      const uint32_t r = (uint32_t)sc.bi[q];
-      const uint64_t key = pack_key(sc.bd[q], r * (uint32_t)n + r + (uint32_t)sc.jo(q));
-      best = key < best ? key : best;
+    uint64_t mask = __ballot((hit_slots >> q) & 1u);
+    while (mask != 0) {
+      const int L = __ffsll((unsigned long long)mask) - 1;
+      mask &= mask - 1;
+      const uint32_t f = diag_first_row(s_tour, s_edge, n, D, ld, wid, lane, q, L, dstar);
+      idx = f < idx ? f : idx;
     }
   }
-  return best;
+  if (lane == 0) red[kTwoOptWaves + wid] = (int)idx;
+  __syncthreads();
+  uint32_t best = (uint32_t)red[kTwoOptWaves];
+  for (int w = 1; w < kTwoOptWaves; ++w) best = min(best, (uint32_t)red[kTwoOptWaves + w]);
+  // Not found only if the matrix disagrees with the coordinates the scan computed from (an
+  // invalid input): end the search rather than apply a move that is not there.
+  return best == ~0u ? kNoMove : pack_key(dstar, best);
 }
 
 // Large-n variant of the diagonal scan: 16 waves per tour and clamped LDS indices instead of
@@ -466,9 +567,9 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
     ++passes;
     uint64_t best = kNoMove;
     if (cand == nullptr && mode == TSPMI_MODE_BEST && QMAX > 0) {
-      best = scan_full_best_diag<(QMAX > 0 ? QMAX : 1)>(s_tour, s_edge, n, D, ld, wid, lane);
+      best = scan_full_best_diag<(QMAX > 0 ? QMAX : 1)>(s_tour, s_edge, n, D, ld, wid, lane,
+                                                        reinterpret_cast<int*>(sh.red));
       if (tid == 0) my_deltas += (int64_t)n * (n - 3) / 2;
-      best = block_min_u64(best, sh.red);
     } else if (cand == nullptr && mode == TSPMI_MODE_BEST) {
       for (int i = wid; i < rows; i += kTwoOptWaves) {
         const int a = s_tour[i], b = s_tour[i + 1];
@@ -658,8 +759,8 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
   __syncthreads();
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
-    uint64_t best = scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, s_xy);
-    best = block_min_u64(best, red);
+    const uint64_t best =
+        scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, reinterpret_cast<int*>(red), s_xy);
     if (best == kNoMove) break;
     const uint32_t idx = key_index(best);
     const int i = (int)(idx / (uint32_t)n);
