@@ -124,11 +124,13 @@ int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int ld, const 
 
 /*
  * tspmi_two_opt(mode = BEST, no candidates) for EUC_2D instances with n <= 2050: D must be the
- * matrix tspmi_dist_euc2d built from xy (float64 (n, 2), 16-byte aligned).  Nine rows in sixteen
- * of the scan compute their distances from the coordinates, the others gather them from D: by
- * the integer form of the distance when every coordinate is an integer and each axis spans at
- * most 2^20 (the kernel checks), else by the matrix's own FP64 function.  The values are the
- * same, so are the tours, moves and counters.
+ * matrix tspmi_dist_euc2d built from xy (float64 (n, 2), 16-byte aligned).  Thirteen rows in
+ * sixteen of the scan compute their distances from the coordinates, the others gather them from
+ * D.  When every coordinate is an integer and each axis spans at most 2^20 (the kernel checks),
+ * the computed distances are lower bounds at most 1 below the matrix entries and each pass
+ * settles its best move by re-walking, with D, every diagonal whose bound comes within 2 of the
+ * smallest; otherwise they are the matrix's own FP64 values.  Either way the moves, tours and
+ * counters are those of tspmi_two_opt.
  * Replaces: two_opt.py:831-899 (full neighbourhood, best) on EUC_2D problems.
  */
 int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy, int max_iter,

@@ -838,13 +838,14 @@ def test_two_opt_packed_euc2d_equals_matrix_kernel(torch_mod, nat, n, P, k, iter
 @pytest.mark.parametrize("n,P,iters", [(4, 3, -1), (5, 6, -1), (10, 6, -1), (63, 4, -1), (66, 4, -1), (257, 3, -1),
                                        (258, 3, 60), (700, 3, 40), (1025, 3, 30), (1026, 3, 30), (1283, 2, 20),
                                        (2050, 2, 12)])
-@pytest.mark.parametrize("kind", ["uniform", "float", "lattice", "span_limit", "beyond_span"])
+@pytest.mark.parametrize("kind", ["uniform", "float", "lattice", "span_limit", "beyond_span", "stacked"])
 def test_two_opt_full_euc2d_matches_c_oracle(torch_mod, nat, n, P, iters, kind):
   """Full-neighbourhood steepest scan with rows from the coordinates, around the slot-phase and
-  8-row mix-period boundaries, on integer (the integer distance form), fractional (the FP64
-  form) and tie-heavy lattice instances, on integers spanning exactly 2^21 with nint-boundary
-  pairs, and on integers one past that span (FP64 form again): tours and counters equal the C
-  oracle's on the matrix of the same coordinates."""
+  16-row mix-period boundaries, on integer (lower-bound rows, exact re-walk of the candidate
+  diagonals), fractional (the FP64 form) and tie-heavy lattice instances, on integers spanning
+  exactly the integer form's limit with nint-boundary pairs, on integers one past that span (FP64
+  form again), and on points stacked on 12 sites (most deltas tie, so many lanes are candidates
+  every pass): tours and counters equal the C oracle's on the matrix of the same coordinates."""
   say(f"two_opt_full_euc2d n={n} P={P} iters={iters} {kind}")
 
   def beyond():
@@ -856,7 +857,8 @@ def test_two_opt_full_euc2d_matches_c_oracle(torch_mod, nat, n, P, iters, kind):
         "float": lambda: np.random.default_rng(n).random((n, 2)) * 1000.0,
         "lattice": lambda: lattice_coords(n),
         "span_limit": lambda: span_limit_int_coords(n, seed=n),
-        "beyond_span": beyond}[kind]()
+        "beyond_span": beyond,
+        "stacked": lambda: uniform_coords(12, seed=n + 17)[np.random.default_rng(n).integers(0, 12, n)]}[kind]()
   d = compute_distance_matrix(xy)
   D, _ = device_dist(nat, torch_mod, xy)
   np.testing.assert_array_equal(D.cpu().numpy(), d)

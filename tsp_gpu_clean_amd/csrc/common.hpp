@@ -191,6 +191,17 @@ __device__ __forceinline__ int euc2d_nint_i32(int xu, int yu, int xv, int yv) {
   return m - (t >> 31);
 }
 
+// A lower bound of euc2d_nint_i32 that is at most 1 below it, in six VALU instructions, for
+// integer coordinates held as FP32 values (exact: each is an integer of magnitude at most 2^24)
+// whose per-axis differences are at most 2^20: the differences are then exact too, and the
+// truncated FP32 root is the m of euc2d_nint_i32 before its exact correction, M - 1 or M (same
+// argument).  For scans that only need to bracket the best delta and then settle the few pairs
+// near it with exact distances.
+__device__ __forceinline__ int euc2d_floor_f32(float xu, float yu, float xv, float yv) {
+  const float fx = xu - xv, fy = yu - yv;
+  return (int)__builtin_amdgcn_sqrtf(__builtin_fmaf(fx, fx, fy * fy));
+}
+
 // Distance policies: a row-major int32 matrix, or EUC_2D coordinates with the distance computed
 // on the fly by the same FP64 function that builds the matrix (bit-identical values).
 // Every entry point that reads a matrix through MatrixDist requires n * ld < 2^30, so an element's

@@ -51,40 +51,57 @@ namespace tspmi {
 // share one address per slot (DiagScan::Lanes): 740 VALU instructions per 16-row period of four
 // slots against 889 when the row of each slot minimum was tracked and every address recomputed.
 //
-// With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), rows 0, 2, 4, 5, 7, 9, 11, 12
-// and 14 of every 16 compute their new distances from the coordinates, with the function that
-// built the matrix or its integer form (same values), instead of gathering them: the carried
-// values are the same either way, and those rows put no work on the texture addresser, whose
-// 64-lane random gathers (32 cycles each) bind the matrix-only scan, while the distance
-// arithmetic runs on otherwise idle VALU cycles.  Measured per n = 1000, P = 1024 launch to the
-// local optimum (585 ms with no coordinate rows): FP64 form, 3 rows in 8: 430-440 ms; integer
-// form (14 instructions), k rows in 8 for k = 3..6: 391 / 320 / 313 / 342 ms, 7 and 9 rows in 16:
-// 342 / 308 ms; with a 19-instruction integer form, every other row with a period of 2 / 4 / 8 /
-// 16 rows: 378 / 371 / 365 / 363 ms (the longer unrolled period hides more latency).  With the
-// lighter bookkeeping (minimum only, shared addresses, rows fenced for the scheduler): 8 / 9 / 10
-// / 11 / 12 rows in 16: 304 / 277 / 276 / 286 / 295 ms, now with the texture addresser 92 % busy
-// and VALU 78 %.
+// With MIX (EUC_2D instances, coordinates staged in LDS as s_xy), 13 rows of every 16 (all but
+// rows 0, 6 and 11) compute their new distances from the coordinates instead of gathering them:
+// with the FP64 kind by the function that built the matrix (the same values); with the integer
+// kind by euc2d_floor_f32, a lower bound at most 1 below the matrix entry, in six instructions
+// (the scan then settles the pass's best move exactly, scan_full_best_diag).  Those rows put no
+// work on the texture addresser, whose 64-lane random gathers (32 cycles each) bind the
+// matrix-only scan.  Measured per n = 1000, P = 1024 launch to the local optimum (585 ms with no
+// coordinate rows): FP64 form, 3 rows in 8: 430-440 ms; exact integer form (14 instructions), k
+// rows in 8 for k = 3..6: 391 / 320 / 313 / 342 ms, 9 rows in 16: 308 ms; with the bookkeeping
+// cut to a minimum per slot and shared addresses: 8 / 9 / 10 / 11 rows in 16: 304 / 277 / 276 /
+// 286 ms; with lower-bound rows (integer coordinates in LDS): 9 / 10 / 11 / 12 / 13 rows: 265 /
+// 237 / 222 / 224 / 226 ms; with those coordinates held as exact FP32 (no conversions): 10 / 11 /
+// 12 / 13 / 14 rows: 232 / 206 / 198 / 197 / 199 ms.
 constexpr int kMixPeriod = 16;
-constexpr int kMixRows = 0b0101101010110101;
+constexpr int kMixRows = 0b1111101111011110;
 
 // Coordinate forms of the MIX rows: FP64 coordinates indexed by node with the FP64 distance, or,
-// when every coordinate is an integer and each axis spans at most kIntCoordSpan, int32 coordinates
-// kept in tour order (entry x holds the coordinates of the node at position x, so a lane's read is
-// contiguous with its neighbours' and needs no lookup through the tour) with the integer form of
-// the same distance (equal values, fewer VALU cycles, a third of the LDS traffic).
+// when every coordinate is an integer and each axis spans at most kIntCoordSpan, the coordinates
+// minus the box's low corner as exact FP32 pairs kept in tour order (entry x holds the node at
+// position x, so a lane's read is contiguous with its neighbours' and needs no lookup through the
+// tour).  dist is the exact distance (edge costs, the move's new edges; the integer form for the
+// integer kind); scan_dist is what the scan's coordinate rows use, at most kSlack below dist: the
+// integer kind takes the truncated FP32 root, so its deltas are lower bounds within 2 of the exact
+// ones (two distances per delta).
 struct XyF64 {
   using T = double2;
   static constexpr bool kTourOrder = false;
+  static constexpr int kSlack = 0;
   static __device__ __forceinline__ int dist(T a, T b) { return euc2d_nint(a.x, a.y, b.x, b.y); }
+  static __device__ __forceinline__ int scan_dist(T a, T b) { return dist(a, b); }
 };
-struct XyI32 {
-  using T = int2;
+// Integer coordinates minus the bounding box's low corner, as FP32 (exact: 0 .. kIntCoordSpan).
+struct XyInt {
+  using T = float2;
   static constexpr bool kTourOrder = true;
-  static __device__ __forceinline__ int dist(T a, T b) { return euc2d_nint_i32(a.x, a.y, b.x, b.y); }
+  static constexpr int kSlack = 2;
+  static __device__ __forceinline__ T stage(double2 p, int lo_x, int lo_y) {
+    return make_float2((float)(p.x - lo_x), (float)(p.y - lo_y));
+  }
+  static __device__ __forceinline__ int dist(T a, T b) {
+    return euc2d_nint_i32((int)a.x, (int)a.y, (int)b.x, (int)b.y);
+  }
+  static __device__ __forceinline__ int scan_dist(T a, T b) { return euc2d_floor_f32(a.x, a.y, b.x, b.y); }
 };
 
+// Delta of pairs the scan must not pick (row 0's pair (0, n-1)), and the start of each slot
+// minimum: far above every real delta, so such a lane is never a candidate.
+constexpr int kFarDelta = 0x3fffffff;
+
 // int32 words of LDS before the tour array in two_opt_mix_kernel: n node-indexed double2, or
-// n + 1 + pad tour-ordered int2 (the tour's own length with its mirror entry and padding).
+// n + 1 + pad tour-ordered float2 (the tour's own length with its mirror entry and padding).
 __host__ __device__ constexpr int mix_coord_words(int n, int pad) {
   return ((4 * n > 2 * (n + 1 + pad) ? 4 * n : 2 * (n + 1 + pad)) + 3) & ~3;
 }
@@ -148,11 +165,11 @@ struct DiagScan {
     if constexpr (COORD && XY::kTourOrder) {
       const T pa = s_xy[r + 2];
 #pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, b.xy[q][R]);
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::scan_dist(pa, b.xy[q][R]);
     } else if constexpr (COORD) {
       const T pa = s_xy[__builtin_amdgcn_readfirstlane(s_tour[r + 2])];
 #pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = XY::dist(pa, s_xy[b.tour[q][R]]);
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::scan_dist(pa, s_xy[b.tour[q][R]]);
     } else {
       const int a = __builtin_amdgcn_readfirstlane(s_tour[r + 2]);
       const int32_t* __restrict__ rn = D + (size_t)a * ld;
@@ -163,7 +180,7 @@ struct DiagScan {
 #pragma unroll
     for (int q = 0; q < QL; ++q) {
       int delta = carry[q] + gcur[q] + neg_eab;
-      if (FIRST) delta = (jo(q) < n - 1) ? delta : 1;
+      if (FIRST) delta = (jo(q) < n - 1) ? delta : kFarDelta;
       bd[q] = min(bd[q], delta);
       carry[q] = opaque_i32(gcur[q] - b.edge[q][R]);  // kept: else the sum is reassociated
       gcur[q] = gnext[q];
@@ -201,47 +218,45 @@ struct DiagScan {
   }
 };
 
-// Scan index i * n + j of the first pair, in lane L's diagonal of slot q of this wave, whose delta
-// is dstar: the wave walks that diagonal 64 rows at a time over exactly the rows the scan gave the
-// slot (row 0, then rows 1 .. (n - 2) - off(w, q) - 1), recomputing each delta from the matrix
-// (the same values the scan used, gathered or computed); ~0u if none (not reached for a lane whose
-// slot minimum is dstar).
-__device__ __forceinline__ uint32_t diag_first_row(const int32_t* s_tour, const int32_t* s_edge, int n,
-                                                const int32_t* __restrict__ D, int ld, int wid, int lane, int q,
-                                                int L, int dstar) {
+// The smallest exact (delta, scan index) key over lane L's diagonal of slot q of this wave, the
+// same in every lane: the wave walks the diagonal 64 rows at a time over exactly the rows the scan
+// gave the slot (row 0, then rows 1 .. (n - 2) - off(w, q) - 1) and takes every delta from the
+// matrix.
+__device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const int32_t* s_edge, int n,
+                                                  const int32_t* __restrict__ D, int ld, int wid, int lane, int q,
+                                                  int L) {
   const int off = diag_slot_offset(wid, q);
   const int jo = 2 + off + L;
   const int end = (n - 2) - off;
   const int last = end > 1 ? end : 1;
+  uint64_t best = kNoMove;
   for (int r0 = 0; r0 < last; r0 += kWave) {
     const int i = r0 + lane;
     const int j = i + jo;
-    const bool ok = i < last && j <= n - 1 && !(i == 0 && j == n - 1);
-    int delta = 1;
-    if (ok) {
+    if (i < last && j <= n - 1 && !(i == 0 && j == n - 1)) {
       const int a = s_tour[i], b = s_tour[i + 1], c = s_tour[j], d = s_tour[j + 1];
-      delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
-    }
-    const uint64_t hit = __ballot(ok && delta == dstar);
-    if (hit != 0) {
-      const int f = r0 + __ffsll((unsigned long long)hit) - 1;
-      return (uint32_t)f * (uint32_t)n + (uint32_t)(f + jo);
+      const int delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
+      const uint64_t key = pack_key(delta, (uint32_t)i * (uint32_t)n + (uint32_t)j);
+      best = key < best ? key : best;
     }
   }
-  return ~0u;
+  return wave_min_u64(best);
 }
 
 // The pass's best move as a packed (delta, scan index) key, the same in every thread of the
 // workgroup (kNoMove if no delta is negative).  The scan keeps only each slot's smallest delta
-// (one instruction per pair instead of a compare and two selects that would also track its row);
-// the workgroup's smallest delta dstar is then located: every lane whose slot minimum is dstar has
-// its diagonal walked again by its wave (diag_first_row), normally one lane in the workgroup, and
-// the lowest scan index among them is the reference's first minimum in scan order.  red holds
-// 2 * waves ints.
+// (one instruction per pair instead of a compare and two selects that would also track its row),
+// and with the integer coordinate kind its coordinate rows use lower-bound distances (kSlack): a
+// scanned delta d~ satisfies d - kSlack <= d~ <= d.  With m the workgroup's smallest d~: every
+// exact delta is >= m, so m >= 0 means no improving move; otherwise a pair holding the exact
+// minimum d* <= (the exact delta of m's pair) <= m + kSlack has d~ <= d* <= m + kSlack, so every
+// lane whose slot minimum is within m + kSlack has its diagonal walked again with exact deltas by
+// its wave (diag_best_key) -- normally one lane in the workgroup -- and the smallest key among
+// them is the reference's first minimum in scan order.  red holds 2 * waves entries.
 template <int QMAX, bool MIX = false, class XY = XyF64>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane,
-                                                        int* red, const typename XY::T* s_xy = nullptr) {
+                                                        uint64_t* red, const typename XY::T* s_xy = nullptr) {
   DiagScan<QMAX, MIX, XY> sc;
   sc.s_tour = s_tour;
   sc.s_edge = s_edge;
@@ -260,7 +275,7 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
       const int j = sc.jo(q);
       sc.carry[q] = row_at(r0, s_tour[j]) - s_edge[j];
       sc.gcur[q] = row_at(r1, s_tour[j + 1]);
-      sc.bd[q] = 0;
+      sc.bd[q] = kFarDelta;
     }
   }
   sc.template row<QMAX, true>(sc.template lanes<QMAX>(0), 0, 0);
@@ -270,31 +285,31 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
   for (int q = 1; q < QMAX; ++q) m = min(m, sc.bd[q]);
   m = wave_min_i32(m);
   // red[0 .. waves) is last read in the previous pass, before barriers of the move
-  if (lane == 0) red[wid] = m;
+  int* red_m = reinterpret_cast<int*>(red);
+  if (lane == 0) red_m[wid] = m;
   __syncthreads();
-  int dstar = red[0];
-  for (int w = 1; w < kTwoOptWaves; ++w) dstar = min(dstar, red[w]);
-  if (dstar >= 0) return kNoMove;  // the same in every thread
-  uint32_t hit_slots = 0;
+  int dmin = red_m[0];
+  for (int w = 1; w < kTwoOptWaves; ++w) dmin = min(dmin, red_m[w]);
+  if (dmin >= 0) return kNoMove;  // the same in every thread
+  const int limit = dmin + (MIX ? XY::kSlack : 0);
+  uint32_t cand_slots = 0;
 #pragma unroll
-  for (int q = 0; q < QMAX; ++q) hit_slots |= (sc.bd[q] == dstar ? 1u : 0u) << q;
-  uint32_t idx = ~0u;
+  for (int q = 0; q < QMAX; ++q) cand_slots |= (sc.bd[q] <= limit ? 1u : 0u) << q;
+  uint64_t best = kNoMove;
   for (int q = 0; q < QMAX; ++q) {
-    uint64_t mask = __ballot((hit_slots >> q) & 1u);
+    uint64_t mask = __ballot((cand_slots >> q) & 1u);
     while (mask != 0) {
       const int L = __ffsll((unsigned long long)mask) - 1;
       mask &= mask - 1;
-      const uint32_t f = diag_first_row(s_tour, s_edge, n, D, ld, wid, lane, q, L, dstar);
-      idx = f < idx ? f : idx;
+      const uint64_t key = diag_best_key(s_tour, s_edge, n, D, ld, wid, lane, q, L);
+      best = key < best ? key : best;
     }
   }
-  if (lane == 0) red[kTwoOptWaves + wid] = (int)idx;
+  if (lane == 0) red[kTwoOptWaves + wid] = best;
   __syncthreads();
-  uint32_t best = (uint32_t)red[kTwoOptWaves];
-  for (int w = 1; w < kTwoOptWaves; ++w) best = min(best, (uint32_t)red[kTwoOptWaves + w]);
-  // Not found only if the matrix disagrees with the coordinates the scan computed from (an
-  // invalid input): end the search rather than apply a move that is not there.
-  return best == ~0u ? kNoMove : pack_key(dstar, best);
+  for (int w = 0; w < kTwoOptWaves; ++w) best = red[kTwoOptWaves + w] < best ? red[kTwoOptWaves + w] : best;
+  // The exact minimum may be >= 0 although a lower bound was negative: then there is no move.
+  return best < pack_key(0, 0) ? best : kNoMove;
 }
 
 // Large-n variant of the diagonal scan: 16 waves per tour and clamped LDS indices instead of
@@ -567,8 +582,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_kernel(int32_t* __restri
     ++passes;
     uint64_t best = kNoMove;
     if (cand == nullptr && mode == TSPMI_MODE_BEST && QMAX > 0) {
-      best = scan_full_best_diag<(QMAX > 0 ? QMAX : 1)>(s_tour, s_edge, n, D, ld, wid, lane,
-                                                        reinterpret_cast<int*>(sh.red));
+      best = scan_full_best_diag<(QMAX > 0 ? QMAX : 1)>(s_tour, s_edge, n, D, ld, wid, lane, sh.red);
       if (tid == 0) my_deltas += (int64_t)n * (n - 3) / 2;
     } else if (cand == nullptr && mode == TSPMI_MODE_BEST) {
       for (int i = wid; i < rows; i += kTwoOptWaves) {
@@ -744,14 +758,15 @@ __device__ __forceinline__ void apply_move_tour_xy(int32_t* s_tour, typename XY:
 template <int QMAX, class XY>
 __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typename XY::T* s_xy, int32_t* s_tour,
                                            int32_t* s_edge, int n, int pad, const int32_t* __restrict__ D, int ld,
-                                           int max_iter, uint64_t* red, int& moves, int& passes) {
+                                           int max_iter, uint64_t* red, int& moves, int& passes, int lo_x = 0,
+                                           int lo_y = 0) {
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = wave_id();
   if constexpr (XY::kTourOrder) {
     for (int x = tid; x < n + 1 + pad; x += kTwoOptBlock) {
-      const double2 p = xy[s_tour[x]];  // the mirror entry and the padding (node 0) included
-      s_xy[x] = make_int2((int)p.x, (int)p.y);
+      // the mirror entry and the padding (node 0) included
+      s_xy[x] = XY::stage(xy[s_tour[x]], lo_x, lo_y);
     }
   } else {
     for (int x = tid; x < n; x += kTwoOptBlock) s_xy[x] = xy[x];
@@ -760,7 +775,7 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
     const uint64_t best =
-        scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, reinterpret_cast<int*>(red), s_xy);
+        scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, red, s_xy);
     if (best == kNoMove) break;
     const uint32_t idx = key_index(best);
     const int i = (int)(idx / (uint32_t)n);
@@ -777,15 +792,15 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
 // Full neighbourhood, best improvement, for EUC_2D instances: the diagonal scan with its MIX rows
 // from the coordinates, staged in LDS before the tour (mix_coord_words, then the tour and edge
 // arrays with their padding).  Same pairs, deltas, tie-break, moves and counters as two_opt_kernel.
-// Integer coordinates within kIntCoordSpan per axis are staged as int32 pairs in tour order and
-// take the integer distance form (the same values); others stay FP64, indexed by node.
+// Integer coordinates within kIntCoordSpan per axis are staged as exact FP32 offsets from the box
+// corner in tour order and scanned with lower-bound distances; others stay FP64, indexed by node.
 template <int QMAX>
 __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __restrict__ tours, int n,
                                                                    const int32_t* __restrict__ D, int ld,
                                                                    const double2* __restrict__ xy, int max_iter,
                                                                    int pad, int64_t* __restrict__ stats) {
   extern __shared__ __align__(16) int32_t lds[];
-  __shared__ uint64_t red[kTwoOptWaves];
+  __shared__ uint64_t red[2 * kTwoOptWaves];
   __shared__ int s_box[4];
   int32_t* s_tour = lds + mix_coord_words(n, pad);  // n + 1 + pad, after the coordinates
   int32_t* s_edge = s_tour + n + 1 + pad;           // n + pad
@@ -805,8 +820,8 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
   for (int x = tid; x < n; x += kTwoOptBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
   int moves = 0, passes = 0;
   if (int_xy)
-    mix_passes<QMAX, XyI32>(xy, reinterpret_cast<int2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
-                            passes);
+    mix_passes<QMAX, XyInt>(xy, reinterpret_cast<float2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
+                            passes, s_box[0], s_box[2]);
   else
     mix_passes<QMAX, XyF64>(xy, reinterpret_cast<double2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
                             passes);

@@ -15,7 +15,7 @@ constexpr int kTwoOptBlock = 256;
 constexpr int kTwoOptWaves = kTwoOptBlock / kWave;
 
 struct TwoOptShared {
-  uint64_t red[kTwoOptWaves];
+  uint64_t red[2 * kTwoOptWaves];  // the full-neighbourhood scan uses both halves
   int64_t red64[kTwoOptWaves];
 };
 
