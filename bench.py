@@ -58,7 +58,7 @@ def c2_inputs(torch, native):
   return xy, D, tours
 
 
-MIX_PERIOD, MIX_ROWS = 16, 0b0101101010110101  # 9 rows of each 16 from the coordinates
+MIX_PERIOD, MIX_ROWS = 16, 0b1111101111011110  # 13 rows of each 16 from the coordinates
 
 
 def diag_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4, mix: bool = False) -> tuple[int, int]:
@@ -66,8 +66,9 @@ def diag_lanes_per_pass(n: int, qmax: int = 4, waves: int = 4, mix: bool = False
   over one tour (256 threads): the carry set-up (two gathers per slot lane), row 0 over every slot,
   then per wave the slot phases -- slot q of wave w runs while i < (n-2) - off(w, q), with
   off(w, q) = 64 (4q + w) on even q and 64 (4q + 3 - w) on odd q.  With `mix`, each phase takes its
-  rows in periods of 16 whose rows 0, 2, 4, 5, 7, 9, 11, 12 and 14 compute the distances from LDS
-  coordinates, and the phase's last (< 16) rows gather."""
+  rows in periods of 16 of which all but rows 0, 6 and 11 compute the distances from LDS
+  coordinates, and the phase's last (< 16) rows gather.  Not counted: the exact re-walk of the
+  pass's candidate diagonals (normally one diagonal, two gathers per row)."""
   rows = n - 2
   gather_rows = coord_rows = 0
   for w in range(waves):
@@ -600,8 +601,9 @@ def main() -> int:
       "lanes_per_launch": lanes, "lanes_per_pass_per_tour": glanes,
       "coordinate_lanes_per_launch": coord_lanes, "coordinate_lanes_per_pass_per_tour": clanes,
       "passes_per_launch": passes, "probe": probe,
-      "note": "9 rows in 16 take their distances from LDS coordinates (integer form for integer coordinates, "
-              "VALU) and issue no gather; the TA fraction counts only the gathered rows",
+      "note": "13 rows in 16 take their distances from LDS coordinates (VALU: a lower bound from the FP32 root "
+              "for integer coordinates, then an exact re-walk of the pass's candidate diagonals) and issue no "
+              "gather; the TA fraction counts only the gathered rows",
       "counters": "profiles/two_opt_c2_counters.md (TA busy fraction, L1 hit rate, VALU and LDS issue)",
     },
     "matrix_only": {

@@ -221,7 +221,8 @@ struct DiagScan {
 // The smallest exact (delta, scan index) key over lane L's diagonal of slot q of this wave, the
 // same in every lane: the wave walks the diagonal 64 rows at a time over exactly the rows the scan
 // gave the slot (row 0, then rows 1 .. (n - 2) - off(w, q) - 1) and takes every delta from the
-// matrix.
+// matrix.  Four blocks of 64 rows per trip, branch-free (indices clamped into the tour, the key
+// of an inadmissible lane discarded), so the loads of the four are in flight together.
 __device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                   const int32_t* __restrict__ D, int ld, int wid, int lane, int q,
                                                   int L) {
@@ -230,13 +231,16 @@ __device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const i
   const int end = (n - 2) - off;
   const int last = end > 1 ? end : 1;
   uint64_t best = kNoMove;
-  for (int r0 = 0; r0 < last; r0 += kWave) {
-    const int i = r0 + lane;
-    const int j = i + jo;
-    if (i < last && j <= n - 1 && !(i == 0 && j == n - 1)) {
-      const int a = s_tour[i], b = s_tour[i + 1], c = s_tour[j], d = s_tour[j + 1];
-      const int delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[i] - s_edge[j];
-      const uint64_t key = pack_key(delta, (uint32_t)i * (uint32_t)n + (uint32_t)j);
+  for (int r0 = 0; r0 < last; r0 += 4 * kWave) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = r0 + u * kWave + lane;
+      const int j = i + jo;
+      const bool ok = i < last && j <= n - 1 && !(i == 0 && j == n - 1);
+      const int ic = ok ? i : 0, jc = ok ? j : 2;  // (0, 2) is a pair of every tour with n >= 4
+      const int a = s_tour[ic], b = s_tour[ic + 1], c = s_tour[jc], d = s_tour[jc + 1];
+      const int delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[ic] - s_edge[jc];
+      const uint64_t key = ok ? pack_key(delta, (uint32_t)i * (uint32_t)n + (uint32_t)j) : kNoMove;
       best = key < best ? key : best;
     }
   }
