@@ -39,8 +39,10 @@ PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 # (TA_TA_BUSY / TA_FLAT_READ_WAVEFRONTS of the scan, and the tspmi_probe_gather micro-benchmark).
 GATHER_LANES_PER_CLK_CU = 2.0
 GATHER_PEAK = GATHER_LANES_PER_CLK_CU * CUS * PEAK_CLOCK_HZ  # lanes/s
-# VALU issue of the headline launch by its counters (SQ_INSTS_VALU x 4 cycles over 1024 SIMDs).
-VALU_BUSY_BY_COUNTERS = 0.94
+# VALU issue of the headline launch by its counters (SQ_INSTS_VALU x 4 cycles over 1024 SIMDs), and
+# the same with every issue cycle counted (SQ_ACTIVE_INST_VALU: the FP32 square root takes two).
+VALU_BUSY_BY_COUNTERS = 0.82
+VALU_ACTIVE_BY_COUNTERS = 0.90
 N_CITIES = 1000
 POP = 1024
 
@@ -128,7 +130,7 @@ def c3_coords():
 def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
   """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch, refined tours,
   passes per step).  With device coordinates `xy`, the candidate scan computes every distance from
-  them and the full scan 3 rows in 8 (tspmi_two_opt_full_euc2d)."""
+  them and the full scan 13 rows in 16 (tspmi_two_opt_full_euc2d)."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
   if cand is not None:
@@ -590,9 +592,10 @@ def main() -> int:
       "bytes_per_unit": BYTES_PER_DELTA, "units_per_launch": deltas,
     },
     "roofline_onchip": {
-      "bound": "VALU issue and texture-address gather issue (TA) together: coordinate rows load the VALU, "
-               "gathered rows the TA, and the row split balances the two",
+      "bound": "VALU issue (coordinate rows), with texture-address gather issue (TA, gathered rows) below it; "
+               "the row split balances the two",
       "valu_busy_by_counters": VALU_BUSY_BY_COUNTERS,
+      "valu_active_by_counters": VALU_ACTIVE_BY_COUNTERS,
       "valu_busy_source": "stored: rocprofv3 SQ_INSTS_VALU of this launch, profiles/two_opt_c2_counters.md",
       "achieved": lanes / (kms * 1e-3),
       "peak": GATHER_PEAK, "unit": "gather-lanes/s", "frac": lanes / (kms * 1e-3) / GATHER_PEAK,
