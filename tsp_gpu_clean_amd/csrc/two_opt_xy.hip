@@ -114,17 +114,32 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
   }
 }
 
+// int32 words of LDS that cand_two_opt_tour uses (tour, edge costs, positions), rounded up to an
+// even count so that int2 coordinates can follow.
+__host__ __device__ constexpr int cand_lds_words(int n) { return (3 * n + 1 + 1) & ~1; }
+
 // XYD: CoordDist (float64 coordinates) or IntCoordDist (int32 coordinates, the integer form).
-template <int BLOCK, int KP, int UNROLL, int KL, class XYD>
+// XY_LDS (IntCoordDist only): the workgroup first copies the n int32 coordinate pairs into LDS
+// after the scan's arrays and reads them there: the two coordinate reads of every candidate pair
+// then cost no texture-addresser cycles (with them in global memory the addresser is about 90 %
+// busy at P = 1024, n = 1000, k = 20, against 76 % for VALU issue).
+template <int BLOCK, int KP, int UNROLL, int KL, class XYD, bool XY_LDS = false>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n, XYD dist,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
                                                                 int64_t* __restrict__ stats,
                                                                 double* __restrict__ cost,
                                                                 int64_t* __restrict__ pass_log, int log_moves) {
-  extern __shared__ int32_t lds[];
+  extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, dist, ct, k, max_iter, stats,
+  XYD d = dist;
+  if constexpr (XY_LDS) {
+    int2* s_xy = reinterpret_cast<int2*>(lds + cand_lds_words(n));
+    // visible to the scan after the barrier that follows cand_two_opt_tour's tour load
+    for (int x = (int)threadIdx.x; x < n; x += BLOCK) s_xy[x] = dist.xy[x];
+    d = XYD{s_xy};
+  }
+  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, d, ct, k, max_iter, stats,
                                            lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr,
                                            pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1)
                                                                : nullptr,
@@ -179,6 +194,8 @@ int packed_euc2d(int32_t* tours, int P, int n, const void* xy, bool int_xy, cons
   TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed_euc2d: (n-2)*k too large");
   const size_t lds = (size_t)(3 * n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_packed_euc2d: n=%d exceeds the LDS limit", n);
+  const size_t lds_xy = ((size_t)cand_lds_words(n) + 2 * (size_t)n) * sizeof(int32_t);
+  const bool xy_lds = int_xy && lds_xy + 1024 <= (size_t)kMaxLds;
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table);
   // One tour per CU: 16 waves each; up to four per CU: 8 waves (measured at P = 1024, n = 1000 and
@@ -186,7 +203,11 @@ int packed_euc2d(int32_t* tours, int P, int n, const void* xy, bool int_xy, cons
   const int cus = device_cu_count();
   const int block = P <= cus ? 1024 : (P <= 4 * cus ? 512 : 256);
 #define TSPMI_CXY(BL, KPV, KLV)                                                                                      \
-  if (int_xy)                                                                                                        \
+  if (xy_lds)                                                                                                        \
+    hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV, IntCoordDist, true>), dim3(P), dim3(BL), lds_xy, s,  \
+                       tours, n, IntCoordDist{reinterpret_cast<const int2*>(xy)}, ct, k, max_iter, stats, cost,      \
+                       pass_log, log_moves);                                                                         \
+  else if (int_xy)                                                                                                   \
     hipLaunchKernelGGL((two_opt_cand_xy_kernel<BL, KPV, 4, KLV, IntCoordDist>), dim3(P), dim3(BL), lds, s, tours, n, \
                        IntCoordDist{reinterpret_cast<const int2*>(xy)}, ct, k, max_iter, stats, cost, pass_log,      \
                        log_moves);                                                                                   \
