@@ -241,14 +241,14 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __r
 
 // As batch_two_opt_kernel, but instances with n >= coord_min_n compute their distances from the
 // EUC_2D coordinates (same tours) instead of gathering them from a matrix that no longer fits the
-// caches.
+// caches; with xy_lds, int32 coordinates are first copied into LDS (stage_int_coords).
 template <int BLOCK, int KL>
 __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst, const int32_t* __restrict__ item_idx,
     int nitems, int32_t* __restrict__ pop, const int32_t* __restrict__ D_all, const double* __restrict__ xy_all,
     const int32_t* __restrict__ ixy_all, const int2* __restrict__ ct_all, int k, int max_iter, int coord_min_n,
-    int64_t* __restrict__ stats) {
-  extern __shared__ int32_t lds[];
+    int xy_lds, int64_t* __restrict__ stats) {
+  extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -256,9 +256,11 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int ke = k < I.n - 1 ? k : I.n - 1;
     int32_t* tour = pop + I.pop + (size_t)item_idx[it] * I.n;
     if (I.n >= 4 && ke >= 1) {
-      if (I.n >= coord_min_n && ixy_all != nullptr)
-        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, IntCoordDist{reinterpret_cast<const int2*>(ixy_all + I.xy)},
-                                            ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+      if (I.n >= coord_min_n && ixy_all != nullptr) {
+        const int2* ixy = reinterpret_cast<const int2*>(ixy_all + I.xy);
+        const IntCoordDist dist = xy_lds ? stage_int_coords<BLOCK>(ixy, I.n, lds) : IntCoordDist{ixy};
+        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, dist, ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+      }
       else if (I.n >= coord_min_n)
         cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
                                             ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
     const int32_t* __restrict__ cut_hi, const double* __restrict__ probs, const int32_t* __restrict__ pos1,
     const int32_t* __restrict__ pos2, double rate, int elite, int kc, int max_iter, int coord_min_n,
     int64_t* __restrict__ stats, int32_t* __restrict__ best, int best_count) {
-  extern __shared__ int32_t lds[];
+  extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   __shared__ int s_w[2];
@@ -511,10 +513,10 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
       double* kout = keysB + 2 * I.row + idx;
       const int ke = kc < n - 1 ? kc : n - 1;
       if (n >= 4 && ke >= 1 && max_iter != 0) {
-        if (n >= coord_min_n && ixy_all != nullptr)
-          cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, IntCoordDist{reinterpret_cast<const int2*>(ixy_all + I.xy)},
-                                              ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout, nullptr, 0,
-                                              src);
+        if (n >= coord_min_n && ixy_all != nullptr)  // the child task's 6n words leave room for the coordinates
+          cand_two_opt_tour<BLOCK, 32, 4, KL>(
+              dst, n, stage_int_coords<BLOCK>(reinterpret_cast<const int2*>(ixy_all + I.xy), n, lds), ct_all + I.ct,
+              ke, max_iter, stats, lds, red, red64, kout, nullptr, 0, src);
         else if (n >= coord_min_n)
           cand_two_opt_tour<BLOCK, 32, 4, KL>(dst, n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
                                               ct_all + I.ct, ke, max_iter, stats, lds, red, red64, kout, nullptr, 0,
@@ -712,15 +714,18 @@ extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32
                   "tspmi_batch_two_opt_packed_mixed: null pointer");
   TSPMI_CHECK_ARG(((uintptr_t)xy_all & 15) == 0 && ((uintptr_t)ixy_all & 7) == 0,
                   "tspmi_batch_two_opt_packed_mixed: xy must be 16-byte and ixy 8-byte aligned");
-  const size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
+  size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_two_opt_packed_mixed: n=%d exceeds the LDS limit",
                   max_n);
+  const size_t lds_xy = ((size_t)cand_lds_words(max_n) + 2 * (size_t)max_n) * sizeof(int32_t);
+  const int xy_lds = ixy_all != nullptr && lds_xy + 1024 <= (size_t)kMaxLds;
+  if (xy_lds) lds = lds_xy;
   const int g = grid > 0 && grid < nitems ? grid : nitems;
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table_all);
 #define TSPMI_BATCH_MIXED(BL, KPV, KLV)                                                                        \
   hipLaunchKernelGGL((batch_two_opt_mixed_kernel<BL, KLV>), dim3(g), dim3(BL), lds, s, desc, item_inst, item_idx,   \
-                     nitems, pop, D_all, xy_all, ixy_all, ct, k, max_iter, coord_min_n, stats)
+                     nitems, pop, D_all, xy_all, ixy_all, ct, k, max_iter, coord_min_n, xy_lds, stats)
   if (block == 256) {
     TSPMI_CAND_BY_K(TSPMI_BATCH_MIXED, 256, k)
   } else if (block == 512) {

@@ -74,6 +74,22 @@ __device__ __forceinline__ void load_tour_lds(const int32_t* __restrict__ tour, 
   }
 }
 
+// int32 words of LDS that cand_two_opt_tour uses (tour, edge costs, positions), rounded up to an
+// even count so that int2 coordinates can follow.
+__host__ __device__ constexpr int cand_lds_words(int n) { return (3 * n + 2) & ~1; }
+
+// Copies the n int32 coordinate pairs of an instance into LDS after cand_two_opt_tour's arrays and
+// returns the distance policy that reads them there: the scan's two coordinate reads per candidate
+// pair then cost no texture-addresser cycles, which random reads from global memory make the
+// scan's binding unit.  The copies are visible to the scan after the barrier that follows its
+// tour load; the caller must have finished with that LDS (a barrier after the previous scan).
+template <int BLOCK>
+__device__ __forceinline__ IntCoordDist stage_int_coords(const int2* __restrict__ xy, int n, int32_t* lds) {
+  int2* s_xy = reinterpret_cast<int2*>(lds + cand_lds_words(n));
+  for (int x = (int)threadIdx.x; x < n; x += BLOCK) s_xy[x] = xy[x];
+  return IntCoordDist{s_xy};
+}
+
 // Lanes per table row of cand_two_opt_tour for k candidates: the smallest instantiated width >= k.
 inline int cand_lane_group(int k) { return k <= 8 ? 8 : k <= 10 ? 10 : k <= 12 ? 12 : k <= 16 ? 16 : k <= 20 ? 20 : 32; }
 

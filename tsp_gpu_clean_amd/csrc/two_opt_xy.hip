@@ -114,10 +114,6 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
   }
 }
 
-// int32 words of LDS that cand_two_opt_tour uses (tour, edge costs, positions), rounded up to an
-// even count so that int2 coordinates can follow.
-__host__ __device__ constexpr int cand_lds_words(int n) { return (3 * n + 1 + 1) & ~1; }
-
 // XYD: CoordDist (float64 coordinates) or IntCoordDist (int32 coordinates, the integer form).
 // XY_LDS (IntCoordDist only): the workgroup first copies the n int32 coordinate pairs into LDS
 // after the scan's arrays and reads them there: the two coordinate reads of every candidate pair
@@ -133,12 +129,7 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restr
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
   XYD d = dist;
-  if constexpr (XY_LDS) {
-    int2* s_xy = reinterpret_cast<int2*>(lds + cand_lds_words(n));
-    // visible to the scan after the barrier that follows cand_two_opt_tour's tour load
-    for (int x = (int)threadIdx.x; x < n; x += BLOCK) s_xy[x] = dist.xy[x];
-    d = XYD{s_xy};
-  }
+  if constexpr (XY_LDS) d = stage_int_coords<BLOCK>(dist.xy, n, lds);
   cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, d, ct, k, max_iter, stats,
                                            lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr,
                                            pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1)
