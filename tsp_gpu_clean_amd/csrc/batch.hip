@@ -241,7 +241,7 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_kernel(const int64_t* __r
 
 // As batch_two_opt_kernel, but instances with n >= coord_min_n compute their distances from the
 // EUC_2D coordinates (same tours) instead of gathering them from a matrix that no longer fits the
-// caches; with xy_lds, int32 coordinates are first copied into LDS (stage_int_coords).
+// caches; with xy_lds, int32-coordinate instances take rec_two_opt_tour (records in LDS).
 template <int BLOCK, int KL>
 __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     const int64_t* __restrict__ desc, const int32_t* __restrict__ item_inst, const int32_t* __restrict__ item_idx,
@@ -258,8 +258,11 @@ __global__ __launch_bounds__(BLOCK) void batch_two_opt_mixed_kernel(
     if (I.n >= 4 && ke >= 1) {
       if (I.n >= coord_min_n && ixy_all != nullptr) {
         const int2* ixy = reinterpret_cast<const int2*>(ixy_all + I.xy);
-        const IntCoordDist dist = xy_lds ? stage_int_coords<BLOCK>(ixy, I.n, lds) : IntCoordDist{ixy};
-        cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, dist, ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+        if (xy_lds)
+          rec_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, ixy, ct_all + I.ct, ke, max_iter, stats, lds, red, red64);
+        else
+          cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, IntCoordDist{ixy}, ct_all + I.ct, ke, max_iter, stats, lds,
+                                              red, red64);
       }
       else if (I.n >= coord_min_n)
         cand_two_opt_tour<BLOCK, 32, 4, KL>(tour, I.n, CoordDist{reinterpret_cast<const double2*>(xy_all + I.xy)},
@@ -513,7 +516,9 @@ __global__ __launch_bounds__(BLOCK) void batch_fused_generation_kernel(
       double* kout = keysB + 2 * I.row + idx;
       const int ke = kc < n - 1 ? kc : n - 1;
       if (n >= 4 && ke >= 1 && max_iter != 0) {
-        if (n >= coord_min_n && ixy_all != nullptr)  // the child task's 6n words leave room for the coordinates
+        if (n >= coord_min_n && ixy_all != nullptr)
+          // coordinates staged in LDS after the scan's arrays (5n + 2 words, within the child task's 6n);
+          // rec_two_opt_tour measured slower inside this kernel (300 against 670 generations/s)
           cand_two_opt_tour<BLOCK, 32, 4, KL>(
               dst, n, stage_int_coords<BLOCK>(reinterpret_cast<const int2*>(ixy_all + I.xy), n, lds), ct_all + I.ct,
               ke, max_iter, stats, lds, red, red64, kout, nullptr, 0, src);
@@ -717,7 +722,7 @@ extern "C" int tspmi_batch_two_opt_packed_mixed(const int64_t* desc, const int32
   size_t lds = (size_t)(3 * max_n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_batch_two_opt_packed_mixed: n=%d exceeds the LDS limit",
                   max_n);
-  const size_t lds_xy = ((size_t)cand_lds_words(max_n) + 2 * (size_t)max_n) * sizeof(int32_t);
+  const size_t lds_xy = (size_t)rec_lds_words(max_n) * sizeof(int32_t);
   const int xy_lds = ixy_all != nullptr && lds_xy + 1024 <= (size_t)kMaxLds;
   if (xy_lds) lds = lds_xy;
   const int g = grid > 0 && grid < nitems ? grid : nitems;
@@ -804,10 +809,11 @@ extern "C" int tspmi_batch_fused_generation(
                   "tspmi_batch_fused_generation: coordinates needed for n >= coord_min_n");
   TSPMI_CHECK_ARG(D_all != nullptr || coord_min_n <= 4, "tspmi_batch_fused_generation: a matrix is needed below "
                   "coord_min_n");
-  // child: p1, p2, inv1, inv2, child, jump (6n); elite 2-opt: tour, edges, positions (3n + 1);
-  // survive: 2P keys as two 32-bit halves and the P survivor sources (5P)
+  // child: p1, p2, inv1, inv2, child, jump (6n); elite 2-opt: tour, edges, positions (3n + 1) and
+  // with int32 coordinates their copy (cand_lds_words + 2n); survive: 2P keys as two 32-bit halves
+  // and the P survivor sources (5P)
   size_t words = (size_t)6 * max_n;
-  if ((size_t)3 * max_n + 1 > words) words = (size_t)3 * max_n + 1;
+  if ((size_t)cand_lds_words(max_n) + 2 * (size_t)max_n > words) words = (size_t)cand_lds_words(max_n) + 2 * max_n;
   if ((size_t)5 * max_p > words) words = (size_t)5 * max_p;
   const size_t lds = words * sizeof(int32_t);
   TSPMI_CHECK_ARG(max_n >= 2 && lds + 2048 <= (size_t)kMaxLds, "tspmi_batch_fused_generation: max_n=%d exceeds LDS",

@@ -79,10 +79,9 @@ __device__ __forceinline__ void load_tour_lds(const int32_t* __restrict__ tour, 
 __host__ __device__ constexpr int cand_lds_words(int n) { return (3 * n + 2) & ~1; }
 
 // Copies the n int32 coordinate pairs of an instance into LDS after cand_two_opt_tour's arrays and
-// returns the distance policy that reads them there: the scan's two coordinate reads per candidate
-// pair then cost no texture-addresser cycles, which random reads from global memory make the
-// scan's binding unit.  The copies are visible to the scan after the barrier that follows its
-// tour load; the caller must have finished with that LDS (a barrier after the previous scan).
+// returns the distance policy that reads them there (no texture-addresser cycles for the scan's
+// coordinate reads).  The copies are visible to the scan after the barrier that follows its tour
+// load; the caller must have finished with that LDS (a barrier after the previous use).
 template <int BLOCK>
 __device__ __forceinline__ IntCoordDist stage_int_coords(const int2* __restrict__ xy, int n, int32_t* lds) {
   int2* s_xy = reinterpret_cast<int2*>(lds + cand_lds_words(n));
@@ -251,6 +250,153 @@ __device__ __forceinline__ void cand_two_opt_tour(int32_t* __restrict__ tour, in
   if (cost != nullptr) {
     int64_t part = 0;
     for (int x = tid; x < n; x += BLOCK) part += s_edge[x];
+    const int64_t total = block_sum_i64(part, red64);
+    if (tid == 0) *cost = (double)total;
+  }
+}
+
+// int32 words of LDS that rec_two_opt_tour uses for an n-city tour, with or without the copy of
+// the coordinates.
+__host__ __device__ constexpr int rec_lds_words(int n, bool coords_lds = true) {
+  return coords_lds ? 7 * n + 1 : 5 * n + 1;
+}
+
+// Candidate-list steepest 2-opt for EUC_2D instances with int32 coordinates whose per-axis span is
+// at most kIntCoordSpan, with node-indexed records in LDS.
+//
+// Same pairs, deltas, tie-break, moves, counters and outputs as cand_two_opt_tour with
+// IntCoordDist.  LDS holds, per node u, the record {position of u, cost of the edge from u to its
+// successor v, coordinates of v} (16 bytes), the coordinates by node and the tour.  A lane takes
+// node a and rank r (lane = node offset * KL + r, as there), loads the table entry {nb, D[a][nb]}
+// and the record of a together, then the record of nb: position i of a, position j of nb, the
+// two removed edges and the coordinates of the two new edge's endpoints t_{i+1} and t_{j+1} are
+// all in hand, so a pair costs one global load and two LDS reads, two levels deep, with no
+// position-to-node lookups.  The evaluation is branch-free and a lane keeps the smallest whole key
+// (delta, i * k + r): the nodes are not visited in scan order, and the key restores the
+// reference's order.  A move reverses the tour between i+1 and j and rewrites the records of the
+// nodes at positions i .. j, from the coordinates: a copy in LDS with COORDS_LDS, else global
+// memory (less LDS, one round of global reads per move).
+// LDS: rec_lds_words(n, COORDS_LDS) int32 words, 16-byte aligned.
+template <int BLOCK, int KP, int UNROLL, int KL = KP, bool COORDS_LDS = true>
+__device__ __forceinline__ void rec_two_opt_tour(int32_t* __restrict__ tour, int n, const int2* __restrict__ xy,
+                                                 const int2* __restrict__ ct, int k, int max_iter,
+                                                 int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
+                                                 int64_t* red64, double* __restrict__ cost = nullptr,
+                                                 int64_t* __restrict__ pass_log = nullptr, int log_moves = 0,
+                                                 const int32_t* __restrict__ src = nullptr) {
+  static_assert(KL <= KP && KL <= kWave, "lanes per row exceed the table row");
+  constexpr int NW = BLOCK / kWave;
+  constexpr int RPW = kWave / KL;  // nodes per wave step
+  int4* s_rec = reinterpret_cast<int4*>(lds);  // n
+  int2* s_xy = reinterpret_cast<int2*>(lds + 4 * n);             // n, with COORDS_LDS
+  int32_t* s_tour = lds + (COORDS_LDS ? 6 * n : 4 * n);          // n + 1 (s_tour[n] mirrors s_tour[0])
+  const int2* __restrict__ cxy = COORDS_LDS ? s_xy : xy;          // where the records are built from
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = wave_id();
+  const int sub = lane / KL;      // node offset inside the wave step (RPW for a spare lane)
+  const int r = lane - sub * KL;  // candidate rank handled by this lane
+  load_tour_lds<BLOCK>(src != nullptr ? src : tour, n, s_tour);
+  if constexpr (COORDS_LDS)
+    for (int x = tid; x < n; x += BLOCK) s_xy[x] = xy[x];
+  __syncthreads();
+  for (int x = tid; x < n; x += BLOCK) {
+    const int u = s_tour[x], v = s_tour[x + 1];
+    const int2 pu = cxy[u], pv = cxy[v];
+    s_rec[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
+  }
+  __syncthreads();
+
+  const int rows = n - 2;
+  int deltas = 0;  // admissible pairs evaluated by this lane
+  int moves = 0, passes = 0;
+  int64_t run_cost = 0;
+  if (pass_log != nullptr) {
+    int64_t part = 0;
+    for (int x = tid; x < n; x += BLOCK) part += s_rec[x].y;
+    run_cost = block_sum_i64(part, red64);
+    if (tid == 0) {
+      pass_log[0] = run_cost;
+      pass_log[1] = (int64_t)wall_clock64();
+    }
+  }
+  for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
+    ++passes;
+    uint64_t best = kNoMove;
+    for (int base = wid * RPW * UNROLL; base < n; base += NW * RPW * UNROLL) {
+      int2 c[UNROLL];
+      int4 ra[UNROLL];
+      bool va[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int a = base + u * RPW + sub;
+        va[u] = sub < RPW && a < n;
+        const int ac = va[u] ? a : 0;
+        // 32-bit byte offset from the table base (n * KP * 8 < 2^32 for every n served here)
+        c[u] = *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(ct) +
+                                              ((((uint32_t)ac * KP) + (uint32_t)r) << 3));
+        ra[u] = s_rec[ac];
+      }
+      int4 rb[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int nb = c[u].x;
+        va[u] = va[u] && r < k && (unsigned)nb < (unsigned)n;
+        rb[u] = s_rec[va[u] ? nb : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int i = ra[u].x, j = rb[u].x;
+        const int lim = (i == 0) ? n - 1 : n;
+        const bool ok = va[u] && i < rows && j > i + 1 && j < lim;
+        const int g = euc2d_nint_i32(ra[u].z, ra[u].w, rb[u].z, rb[u].w);
+        const int delta = c[u].y + g - ra[u].y - rb[u].y;
+        deltas += ok ? 1 : 0;
+        const uint64_t key = (ok && delta < 0) ? pack_key(delta, (uint32_t)i * (uint32_t)k + (uint32_t)r) : kNoMove;
+        best = key < best ? key : best;
+      }
+    }
+    best = block_min_u64(best, red);
+    if (best == kNoMove) break;
+    const uint32_t idx = key_index(best);
+    const int i = (int)(idx / (uint32_t)k);
+    const int j = s_rec[ct[(size_t)s_tour[i] * KP + (int)(idx - (uint32_t)i * (uint32_t)k)].x].x;
+    __syncthreads();  // every thread has read i and j before the tour changes
+    const int half = (j - i) >> 1;
+    for (int t = tid; t < half; t += BLOCK) {
+      const int lo = i + 1 + t, hi = j - t;
+      const int32_t tmp = s_tour[lo];
+      s_tour[lo] = s_tour[hi];
+      s_tour[hi] = tmp;
+    }
+    __syncthreads();
+    for (int x = i + tid; x <= j; x += BLOCK) {
+      const int u = s_tour[x], v = s_tour[x + 1];
+      const int2 pu = cxy[u], pv = cxy[v];
+      s_rec[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
+    }
+    __syncthreads();
+    ++moves;
+    if (pass_log != nullptr) {
+      run_cost += (int32_t)((uint32_t)(best >> 32) ^ 0x80000000u);  // the applied delta
+      if (tid == 0 && moves <= log_moves) {
+        pass_log[2 * moves] = run_cost;
+        pass_log[2 * moves + 1] = (int64_t)wall_clock64();
+      }
+    }
+  }
+  for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
+  if (stats != nullptr) {
+    const int64_t total = block_sum_i64(deltas, red64);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&stats[0], (unsigned long long)total);
+      atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
+      atomicAdd((unsigned long long*)&stats[2], (unsigned long long)passes);
+    }
+  }
+  if (cost != nullptr) {
+    int64_t part = 0;
+    for (int x = tid; x < n; x += BLOCK) part += s_rec[x].y;
     const int64_t total = block_sum_i64(part, red64);
     if (tid == 0) *cost = (double)total;
   }

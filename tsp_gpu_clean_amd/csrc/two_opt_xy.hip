@@ -115,10 +115,8 @@ __global__ __launch_bounds__(BLOCK) void two_opt_xy_kernel(int32_t* __restrict__
 }
 
 // XYD: CoordDist (float64 coordinates) or IntCoordDist (int32 coordinates, the integer form).
-// XY_LDS (IntCoordDist only): the workgroup first copies the n int32 coordinate pairs into LDS
-// after the scan's arrays and reads them there: the two coordinate reads of every candidate pair
-// then cost no texture-addresser cycles (with them in global memory the addresser is about 90 %
-// busy at P = 1024, n = 1000, k = 20, against 76 % for VALU issue).
+// XY_LDS (IntCoordDist only): rec_two_opt_tour, with the coordinates and node records in LDS (its
+// comment); otherwise cand_two_opt_tour reading the coordinates from global memory.
 template <int BLOCK, int KP, int UNROLL, int KL, class XYD, bool XY_LDS = false>
 __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restrict__ tours, int n, XYD dist,
                                                                 const int2* __restrict__ ct, int k, int max_iter,
@@ -128,13 +126,14 @@ __global__ __launch_bounds__(BLOCK) void two_opt_cand_xy_kernel(int32_t* __restr
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[BLOCK / kWave];
   __shared__ int64_t red64[BLOCK / kWave];
-  XYD d = dist;
-  if constexpr (XY_LDS) d = stage_int_coords<BLOCK>(dist.xy, n, lds);
-  cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, d, ct, k, max_iter, stats,
-                                           lds, red, red64, cost != nullptr ? cost + blockIdx.x : nullptr,
-                                           pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1)
-                                                               : nullptr,
-                                           log_moves);
+  double* const c = cost != nullptr ? cost + blockIdx.x : nullptr;
+  int64_t* const pl = pass_log != nullptr ? pass_log + (size_t)blockIdx.x * 2 * (log_moves + 1) : nullptr;
+  if constexpr (XY_LDS)
+    rec_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, dist.xy, ct, k, max_iter, stats, lds,
+                                            red, red64, c, pl, log_moves);
+  else
+    cand_two_opt_tour<BLOCK, KP, UNROLL, KL>(tours + (size_t)blockIdx.x * n, n, dist, ct, k, max_iter, stats, lds,
+                                             red, red64, c, pl, log_moves);
 }
 
 }  // namespace tspmi
@@ -185,7 +184,7 @@ int packed_euc2d(int32_t* tours, int P, int n, const void* xy, bool int_xy, cons
   TSPMI_CHECK_ARG((long)(n - 2) * k < (1L << 31), "tspmi_two_opt_packed_euc2d: (n-2)*k too large");
   const size_t lds = (size_t)(3 * n + 1) * sizeof(int32_t);
   TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_packed_euc2d: n=%d exceeds the LDS limit", n);
-  const size_t lds_xy = ((size_t)cand_lds_words(n) + 2 * (size_t)n) * sizeof(int32_t);
+  const size_t lds_xy = (size_t)rec_lds_words(n) * sizeof(int32_t);
   const bool xy_lds = int_xy && lds_xy + 1024 <= (size_t)kMaxLds;
   hipStream_t s = (hipStream_t)stream;
   const int2* ct = reinterpret_cast<const int2*>(table);
