@@ -22,16 +22,23 @@ MAX_DISTANCE = (1 << 29) - 1
 DEFAULT_MAX_MATRIX_DIMENSION = 10_000
 
 
-def coordinate_candidates_pay(n: int, tours: int | None = None) -> bool:
+def coordinate_candidates_pay(n: int, tours: int | None = None, integer: bool = False) -> bool:
   """Whether the candidate-list 2-opt should compute EUC_2D distances from coordinates instead of
-  gathering them from the matrix.  Measured on an MI355X with identical tours:
-  * many tours (more than CUs, the throughput regime; P = 1024, k = 20): the matrix form wins at
-    n = 500 (2.4e11 vs 1.9e11 deltas/s) and loses from n = 1000 on (1.06 vs 0.90 ms per 20-pass
-    launch at n = 1000, 1.3x at P = 4096; 2.5x at n = 2000, where the n x n int32 matrix has
-    outgrown an XCD's 4 MB L2) -- hence n > 768;
-  * a few tours (one per CU or fewer, latency-bound): the forms run within 10 % of each other
-    until the matrix leaves the caches altogether (n = 10000, k = 8, 256 tours, 200 passes: 9.4
-    vs 35.2 ms) -- hence n > 4096."""
+  gathering them from the matrix.  Measured on an MI355X with identical tours (k = 20):
+  * integer coordinates (`integer`, the int32 form with the coordinates and node records in LDS):
+    20 passes at P = 1024 take 0.16 / 0.19 / 0.28 / 0.47 ms against the matrix's 0.18 / 0.27 /
+    0.44 / 1.02 at n = 200 / 300 / 500 / 1000; at P = 256, 0.094 / 0.125 / 0.187 against 0.081 /
+    0.168 / 0.349 at n = 200 / 500 / 1000; at P = 16 (10 passes) 0.094 / 0.151 against 0.144 /
+    0.358 at n = 1000 / 2000 -- hence n > 256;
+  * FP64 coordinates, many tours (more than CUs, the throughput regime; P = 1024): the matrix form
+    wins at n = 500 (2.4e11 vs 1.9e11 deltas/s) and loses from n = 1000 on (1.06 vs 0.90 ms per
+    20-pass launch at n = 1000, 1.3x at P = 4096; 2.5x at n = 2000, where the n x n int32 matrix
+    has outgrown an XCD's 4 MB L2) -- hence n > 768;
+  * FP64 coordinates, a few tours (one per CU or fewer, latency-bound): the forms run within 10 %
+    of each other until the matrix leaves the caches altogether (n = 10000, k = 8, 256 tours, 200
+    passes: 9.4 vs 35.2 ms) -- hence n > 4096."""
+  if integer:
+    return n > 256
   if tours is not None and tours > _cu_count():
     return n > 768
   return n > 4096

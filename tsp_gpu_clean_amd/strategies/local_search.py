@@ -105,11 +105,12 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     if cand is not None and _MODES[self.mode] == TSPMI_MODE_BEST and 1 <= cand.shape[1] <= 32:
       if table is None:
         table = context.get_gpu_candidate_table(int(self.candidate_k))
-      xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(n, int(tours.shape[0])) else None
+      ixy = context.get_gpu_int_coordinates()  # integer instances: the integer distance form
+      pay = coordinate_candidates_pay(n, int(tours.shape[0]), integer=ixy is not None)
+      xy = context.get_gpu_euc2d_coordinates() if pay else None
       if large:  # the tour arrays in a global workspace (two_opt.py:75-94)
         _native.two_opt_large(tours, dist, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
-      elif xy is not None:  # same tours; distances from coordinates once the matrix leaves the caches
-        ixy = context.get_gpu_int_coordinates()  # integer instances: the integer distance form
+      elif xy is not None:  # same tours; distances from the coordinates
         _native.two_opt_packed_euc2d(tours, ixy if ixy is not None else xy, table, int(cand.shape[1]),
                                      self.max_iterations, self._stats)
       else:
@@ -120,7 +121,7 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       xy = None
       if cand is None and _MODES[self.mode] == TSPMI_MODE_BEST and n <= _native.FULL_EUC2D_MAX_N:
         xy = context.get_gpu_euc2d_coordinates()
-      if xy is not None:  # same tours; 9 rows in 16 take their distances from the coordinates
+      if xy is not None:  # same tours; 13 rows in 16 take their distances from the coordinates
         _native.two_opt_full_euc2d(tours, dist, xy, self.max_iterations, self._stats)
       else:
         _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
