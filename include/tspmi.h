@@ -182,6 +182,18 @@ int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, 
                         int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync, void* stream);
 
 /*
+ * tspmi_two_opt_elite for EUC_2D instances with int32 coordinates ixy ((n, 2), 8-byte aligned,
+ * each axis spanning at most 2^20): the distances of the scan and of the refreshed edges are
+ * computed from the coordinates held in LDS (the integer form, equal to the matrix entries), so
+ * no matrix is read and a pass touches no global memory but the cluster's barrier words.  Where
+ * no cluster applies, this is tspmi_two_opt_packed_euc2d_i32 with costs.  Same tours, moves,
+ * counters, costs and best index as tspmi_two_opt_elite on the matrix of the same coordinates.
+ * Replaces: as tspmi_two_opt_elite.
+ */
+int tspmi_two_opt_elite_xy(int32_t* tours, int P, int n, const int32_t* ixy, const int32_t* table, int k,
+                           int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync, void* stream);
+
+/*
  * tspmi_two_opt_elite without a caller workspace, for callers that cannot allocate (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): the cluster barrier
  * words live in a static device array that the library holds on every GPU it runs on until the

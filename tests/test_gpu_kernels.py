@@ -638,11 +638,13 @@ def test_packed_two_opt_fused_cost(torch_mod, nat, n, P, k, iters):
 
 @pytest.mark.parametrize("n,P,k,iters", [(1000, 16, 20, 10), (1000, 16, 20, -1), (300, 4, 8, -1), (2000, 8, 16, 20),
                                          (6000, 2, 5, 8), (1000, 200, 20, 5), (7, 3, 6, 3)])
-def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters):
-  """Workgroup clusters per tour (few tours) or the one-workgroup fallback: the tours, counters and
-  costs of tspmi_two_opt_packed, the C oracle's tours, and a workspace left cleared for the next
-  call (run twice in a row)."""
-  say(f"two_opt_elite n={n} P={P} k={k} iters={iters}")
+@pytest.mark.parametrize("form", ["matrix", "xy"])
+def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters, form):
+  """Workgroup clusters per tour (few tours) or the one-workgroup fallback, with the matrix or with
+  int32 coordinates (tspmi_two_opt_elite_xy: node records with the successor's coordinates): the
+  tours, counters and costs of tspmi_two_opt_packed, the C oracle's tours, and a workspace left
+  cleared for the next call (run twice in a row)."""
+  say(f"two_opt_elite n={n} P={P} k={k} iters={iters} {form}")
   xy = uniform_coords(n, seed=n * 3 + P)
   D, _ = device_dist(nat, torch_mod, xy)
   kk = min(k, n - 1)
@@ -659,7 +661,10 @@ def test_two_opt_elite_matches_packed_and_oracle(torch_mod, nat, n, P, k, iters)
     cost = torch_mod.full((P,), -1.0, dtype=torch_mod.float64, device="cuda")
     best = torch_mod.full((1,), -1, dtype=torch_mod.int32, device="cuda")
     nat.two_opt_packed(want, D, table, kk, iters, sw)
-    nat.two_opt_elite(got, D, table, kk, iters, sg, cost, sync, best)
+    if form == "xy":
+      nat.two_opt_elite(got, None, table, kk, iters, sg, cost, sync, best, ixy=cu(xy.astype(np.int32)))
+    else:
+      nat.two_opt_elite(got, D, table, kk, iters, sg, cost, sync, best)
     torch_mod.cuda.synchronize()
     assert int(best.item()) == int(np.argmin(cost.cpu().numpy()))
     assert not nat.elite_failed(sync)

@@ -483,7 +483,7 @@ def test_evolve_device_loop_equals_step_pipeline(t, case, monkeypatch):
   if case == "cluster_timeout":
     real = _native.two_opt_elite
 
-    def raising(tours, D, table, k, iters, stats, cost, sync, best=None):
+    def raising(tours, D, table, k, iters, stats, cost, sync, best=None, ixy=None):
       # every 4th elite search, eager or captured (a captured one raises the word in every replay)
       forced["calls"] += 1
       if forced["calls"] % 4 == 3:
@@ -491,7 +491,7 @@ def test_evolve_device_loop_equals_step_pipeline(t, case, monkeypatch):
         sync.view(-1, _native.ELITE_SYNC_WORDS)[0, e : e + 1].fill_(1)  # a fill kernel: capturable
         forced["raised"] += 1
         forced["captured"] = forced.get("captured", 0) + int(torch.cuda.is_current_stream_capturing())
-      return real(tours, D, table, k, iters, stats, cost, sync, best)
+      return real(tours, D, table, k, iters, stats, cost, sync, best, ixy=ixy)
 
   n, P, gens, elite, kw, opt, patience = 120, 64, 40, 8, {}, None, 3
   if case == "remainder_no_stop":

@@ -44,6 +44,7 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
   "tspmi_two_opt_elite": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P],
   "tspmi_two_opt_elite_noalloc": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P],
+  "tspmi_two_opt_elite_xy": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _P],
   "tspmi_tournament": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_pmx": [_P, _I, _I, _P, _P, _P, _P],
   "tspmi_inversion": [_P, _I, _I, _P, _P, _P, _D, _P],
@@ -328,15 +329,20 @@ ELITE_SYNC_WORDS = 528  # int64 words of the elite 2-opt workspace per tour
 ELITE_ERROR_WORD = 512  # its barrier-timeout word
 
 
-def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync, best=None) -> None:
+def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync, best=None, ixy=None) -> None:
   """Elite-set candidate 2-opt with final costs: :func:`two_opt_packed` semantics, one cluster of
   workgroups per tour when the tours leave CUs idle.  ``sync`` is an int64 device workspace of
   ``ELITE_SYNC_WORDS * P`` words, zeroed before first use; :func:`elite_failed` reports a cluster
   whose barrier timed out (its tour and cost are then left unchanged).  ``best`` (int32 (1,),
-  optional) receives the index of the first minimum cost."""
+  optional) receives the index of the first minimum cost.  With ``ixy`` (int32 (n, 2) EUC_2D
+  coordinates within the integer form's span) the distances come from the coordinates
+  (tspmi_two_opt_elite_xy; same results) and ``D`` is not read."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
-  _need(D, torch.int32, "D", 2)
+  if ixy is None:
+    _need(D, torch.int32, "D", 2)
+  else:
+    _need(ixy, torch.int32, "ixy", 2)
   _need(table, torch.int32, "table", 3)
   _need(sync, torch.int64, "sync", 1)
   if stats is not None:
@@ -354,6 +360,13 @@ def two_opt_elite(tours, D, table, k: int, max_iter: int, stats, cost, sync, bes
     _need(best, torch.int32, "best", 1)
     if cost is None:
       raise ValueError("two_opt_elite: best needs cost")
+  if ixy is not None:
+    if ixy.shape != (n, 2):
+      raise ValueError("two_opt_elite: ixy must be (n, 2)")
+    _check(_lib.tspmi_two_opt_elite_xy(_ptr(tours), P, n, _ptr(ixy), _ptr(table), int(k), int(max_iter),
+                                       _ptr(stats), _ptr(cost), _ptr(best), _ptr(sync), _stream(torch)),
+           "tspmi_two_opt_elite_xy")
+    return
   _check(_lib.tspmi_two_opt_elite(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(table), int(k), int(max_iter),
                                   _ptr(stats), _ptr(cost), _ptr(best), _ptr(sync), _stream(torch)),
          "tspmi_two_opt_elite")

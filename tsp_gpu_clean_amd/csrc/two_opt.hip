@@ -866,9 +866,11 @@ __global__ __launch_bounds__(1024) void two_opt_reg_kernel(int32_t* __restrict__
 // t*G + G - 1.
 constexpr int kClusterBlock = 256;
 
-template <int PP>
+// XY: distances from int32 coordinates ixy (cluster_two_opt_tour with IntCoordDist), else from D.
+template <int PP, bool XY = false>
 __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t* __restrict__ tours, int n,
                                                                         const int32_t* __restrict__ D, int ld,
+                                                                        const int2* __restrict__ ixy,
                                                                         const int2* __restrict__ ct, int kp, int k,
                                                                         int max_iter, int G, uint64_t* __restrict__ sync,
                                                                         int64_t* __restrict__ stats,
@@ -890,10 +892,16 @@ __global__ __launch_bounds__(kClusterBlock) void two_opt_cluster_kernel(int32_t*
     t = blockIdx.x / G;
     g = blockIdx.x - t * G;
   }
-  cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, G, g,
-                                          sync + (size_t)t * kClusterWords, stats,
-                                          cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best, P, sync, cost,
-                                          best);
+  if constexpr (XY)
+    cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, IntCoordDist{ixy}, ct, kp, k, max_iter, G, g,
+                                            sync + (size_t)t * kClusterWords, stats,
+                                            cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best, P, sync,
+                                            cost, best);
+  else
+    cluster_two_opt_tour<kClusterBlock, PP>(tours + (size_t)t * n, n, MatrixDist{D, ld}, ct, kp, k, max_iter, G, g,
+                                            sync + (size_t)t * kClusterWords, stats,
+                                            cost != nullptr ? cost + t : nullptr, lds, red, red64, s_best, P, sync,
+                                            cost, best);
 }
 
 // Packed candidate table: out[u, r] = {N[u, r], D[u, N[u, r]]} for r < k, {-1, 0} for k <= r < KP.
@@ -1279,9 +1287,11 @@ __global__ void count_elite_errors_kernel(const uint64_t* __restrict__ sync, int
 // The elite search of tspmi_two_opt_elite on workspace `sync`.  With `fresh`, the workspace of the
 // P tours is zeroed first (so nothing of an earlier call, finished or timed out, carries over),
 // and `failed` (nullable) receives the count of tours whose barrier timed out.
-int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k, int max_iter,
-              int64_t* stats, double* cost, int32_t* best, uint64_t* sync, bool fresh, int32_t* failed,
-              hipStream_t s) {
+// With ixy (int32 coordinates, each axis spanning at most kIntCoordSpan) the distances come from
+// the coordinates and D is not read.
+int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* ixy, const int32_t* table, int k,
+              int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync, bool fresh,
+              int32_t* failed, hipStream_t s) {
   TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_elite: negative shape");
   TSPMI_CHECK_ARG(k >= 1 && k <= 32, "tspmi_two_opt_elite: k=%d outside [1, 32]", k);
   const int cus = device_cu_count();
@@ -1291,12 +1301,14 @@ int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int3
   G = G > 16 ? 16 : G;
   const long pairs = (long)n * k;
   const long per = G > 0 ? (pairs + (long)G * kClusterBlock - 1) / ((long)G * kClusterBlock) : 1L << 30;
-  const size_t lds = (size_t)(5 * n + 1) * sizeof(int32_t);
+  const size_t lds = (size_t)(ixy != nullptr ? 7 * n + 2 : 5 * n + 1) * sizeof(int32_t);
   const bool cluster = P > 0 && n >= 4 && max_iter != 0 && G >= 2 && per <= 8 && n <= (1 << kRegJBits) &&
                        pairs < (1L << (32 - kRegJBits)) && lds + 1024 <= (size_t)kMaxLds;
   TSPMI_CHECK_ARG(best == nullptr || cost != nullptr, "tspmi_two_opt_elite: best needs cost");
   if (!cluster) {
-    int rc = tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, s);
+    int rc = ixy != nullptr
+                 ? tspmi_two_opt_packed_euc2d_i32(tours, P, n, ixy, table, k, max_iter, stats, cost, nullptr, 0, s)
+                 : tspmi_two_opt_packed_cost(tours, P, n, D, ld, table, k, max_iter, stats, cost, s);
     if (rc == TSPMI_OK && best != nullptr && P > 0) rc = tspmi_rank_select(cost, P, 1, best, s);
     if (rc == TSPMI_OK && fresh && failed != nullptr) {  // no barrier ran: writes failed[0] = 0
       hipLaunchKernelGGL(count_elite_errors_kernel, dim3(1), dim3(kWave), 0, s, sync, 0, failed);
@@ -1304,10 +1316,11 @@ int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int3
     }
     return rc;
   }
-  TSPMI_CHECK_ARG(tours && D && table && sync, "tspmi_two_opt_elite: null pointer");
-  TSPMI_CHECK_ARG(ld >= n && (long)n * ld < kMaxMatrixElems, "tspmi_two_opt_elite: bad ld=%d for n=%d", ld, n);
-  TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0,
-                  "tspmi_two_opt_elite: table and sync must be 8-byte aligned");
+  TSPMI_CHECK_ARG(tours && (D || ixy) && table && sync, "tspmi_two_opt_elite: null pointer");
+  TSPMI_CHECK_ARG(ixy != nullptr || (ld >= n && (long)n * ld < kMaxMatrixElems), "tspmi_two_opt_elite: bad ld=%d for n=%d",
+                  ld, n);
+  TSPMI_CHECK_ARG(((uintptr_t)table & 7) == 0 && ((uintptr_t)sync & 7) == 0 && ((uintptr_t)ixy & 7) == 0,
+                  "tspmi_two_opt_elite: table, sync and coordinates must be 8-byte aligned");
   const int2* ct = reinterpret_cast<const int2*>(table);
   const int kp = cand_lanes(k);
   const int xcds = (P % kXcds == 0 && cus % kXcds == 0) ? kXcds : 1;
@@ -1316,9 +1329,14 @@ int elite_run(int32_t* tours, int P, int n, const int32_t* D, int ld, const int3
     hipLaunchKernelGGL(zero_words_kernel, dim3((words + 255) / 256), dim3(256), 0, s, sync, words);
     TSPMI_CHECK_LAUNCH("tspmi_two_opt_elite workspace clear launch");
   }
-#define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                 \
-  hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, ct, kp, \
-                     k, max_iter, G, sync, stats, cost, P, best, xcds)
+  const int2* cxy = reinterpret_cast<const int2*>(ixy);
+#define TSPMI_LAUNCH_CLUSTER(PPV)                                                                                   \
+  if (ixy != nullptr)                                                                                              \
+    hipLaunchKernelGGL((two_opt_cluster_kernel<PPV, true>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, \
+                       cxy, ct, kp, k, max_iter, G, sync, stats, cost, P, best, xcds);                             \
+  else                                                                                                             \
+    hipLaunchKernelGGL((two_opt_cluster_kernel<PPV>), dim3(P * G), dim3(kClusterBlock), lds, s, tours, n, D, ld, cxy, \
+                       ct, kp, k, max_iter, G, sync, stats, cost, P, best, xcds)
   if (per <= 2) TSPMI_LAUNCH_CLUSTER(2);
   else if (per <= 4) TSPMI_LAUNCH_CLUSTER(4);
   else TSPMI_LAUNCH_CLUSTER(8);
@@ -1360,7 +1378,15 @@ int elite_scratch(uint64_t** out) {
 extern "C" int tspmi_two_opt_elite(int32_t* tours, int P, int n, const int32_t* D, int ld, const int32_t* table, int k,
                                    int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync,
                                    void* stream) {
-  return tspmi::elite_run(tours, P, n, D, ld, table, k, max_iter, stats, cost, best, sync, false, nullptr,
+  return tspmi::elite_run(tours, P, n, D, ld, nullptr, table, k, max_iter, stats, cost, best, sync, false, nullptr,
+                          (hipStream_t)stream);
+}
+
+extern "C" int tspmi_two_opt_elite_xy(int32_t* tours, int P, int n, const int32_t* ixy, const int32_t* table, int k,
+                                      int max_iter, int64_t* stats, double* cost, int32_t* best, uint64_t* sync,
+                                      void* stream) {
+  TSPMI_CHECK_ARG(ixy != nullptr || P == 0, "tspmi_two_opt_elite_xy: null coordinates");
+  return tspmi::elite_run(tours, P, n, nullptr, n, ixy, table, k, max_iter, stats, cost, best, sync, false, nullptr,
                           (hipStream_t)stream);
 }
 
@@ -1374,5 +1400,6 @@ extern "C" int tspmi_two_opt_elite_noalloc(int32_t* tours, int P, int n, const i
   uint64_t* ws = nullptr;
   const int rc = elite_scratch(&ws);
   if (rc != TSPMI_OK) return rc;
-  return elite_run(tours, P, n, D, ld, table, k, max_iter, stats, cost, best, ws, true, failed, (hipStream_t)stream);
+  return elite_run(tours, P, n, D, ld, nullptr, table, k, max_iter, stats, cost, best, ws, true, failed,
+                   (hipStream_t)stream);
 }

@@ -619,14 +619,24 @@ __device__ __forceinline__ void store_relaxed(uint64_t* p, uint64_t v) {
 // `best` (nullable) receives the index of the first minimum of the P tours' costs (cost_all,
 // written with agent-scope stores): the last tour to finish, counted in word 2 of tour 0's
 // control line (ws0), takes the minimum and clears the count.
+//
+// With int32 coordinates (Dist = IntCoordDist, each axis spanning at most kIntCoordSpan), the
+// workgroup copies them into LDS and a node record is {position, cost of the edge to the
+// successor, the successor's coordinates}: a pair's two record reads then give positions, both
+// removed edges and the new edge's endpoints, the new distance is computed, and a pass touches no
+// global memory but the cluster's key words (with the matrix, {position, successor, edge cost}
+// and a gather D[b][d] per pair, the stale edge costs refreshed in flight with those gathers).
+// LDS: 7n + 1 int32 words with coordinates, 5n + 1 with the matrix.
 template <int BLOCK, int PP, class Dist>
 __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour, int n, const Dist dist, const int2* __restrict__ ct,
                                      int kp, int k, int max_iter, int G, int g, uint64_t* __restrict__ ws,
                                      int64_t* __restrict__ stats, double* __restrict__ cost, int32_t* lds,
                                      uint64_t* red, int64_t* red64, uint64_t* s_best, int P, uint64_t* ws0,
                                      double* __restrict__ cost_all, int32_t* __restrict__ best) {
+  constexpr bool XY = __is_same(Dist, IntCoordDist);
   int4* s_nd = reinterpret_cast<int4*>(lds);  // n
   int32_t* s_tour = lds + 4 * n;              // n + 1 (s_tour[n] mirrors s_tour[0])
+  int2* s_xy = reinterpret_cast<int2*>(lds + 4 * n + ((n + 2) & ~1));  // n, with coordinates
   const int tid = threadIdx.x;
   const int total = n * k;
   const int base = g * BLOCK + tid;
@@ -656,6 +666,8 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
       }
     }
     load_tour_lds<BLOCK>(tour, n, s_tour);
+    if constexpr (XY)
+      for (int x = tid; x < n; x += BLOCK) s_xy[x] = dist.xy[x];
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
       const int p = base + m * stride;
@@ -665,7 +677,15 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
     }
   }
   __syncthreads();
-  build_node_records<BLOCK>(s_nd, s_tour, n, dist);
+  if constexpr (XY) {
+    for (int x = tid; x < n; x += BLOCK) {
+      const int u = s_tour[x], v = s_tour[x + 1];
+      const int2 pu = s_xy[u], pv = s_xy[v];
+      s_nd[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
+    }
+  } else {
+    build_node_records<BLOCK>(s_nd, s_tour, n, dist);
+  }
   __syncthreads();
 
   const int rows = n - 2;
@@ -681,7 +701,7 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
     // together and a pass waits on global memory once.  The deltas read the costs after the
     // barrier.  Segments longer than ZU * BLOCK finish in refresh_edge_costs.
     constexpr int ZU = 4;
-    const bool stale = zlo <= zhi;
+    const bool stale = !XY && zlo <= zhi;
     int zu[ZU], zd[ZU];
     if (stale) {
 #pragma unroll
@@ -700,11 +720,20 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
       if (wave_base + m * stride >= total) continue;  // no lane of this wave holds pair m
       const uint32_t w = ac[m];
       const int a = (int)(w & 0xffffu), c = (int)(w >> 16);
-      const int2 na = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : a]);
-      const int2 nc = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : c]);
-      pi[m] = na.x;
-      pj[m] = nc.x;
-      gd[m] = dist(na.y, nc.y);
+      if constexpr (XY) {
+        // the whole records: the edge costs come with them (gd holds the delta's part from them)
+        const int4 na = s_nd[w == 0xffffffffu ? 0 : a];
+        const int4 nc = s_nd[w == 0xffffffffu ? 0 : c];
+        pi[m] = na.x;
+        pj[m] = nc.x;
+        gd[m] = euc2d_nint_i32(na.z, na.w, nc.z, nc.w) - na.y - nc.y;
+      } else {
+        const int2 na = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : a]);
+        const int2 nc = *reinterpret_cast<const int2*>(&s_nd[w == 0xffffffffu ? 0 : c]);
+        pi[m] = na.x;
+        pj[m] = nc.x;
+        gd[m] = dist(na.y, nc.y);
+      }
     }
     if (stale) {
 #pragma unroll
@@ -724,7 +753,8 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
       const int i = pi[m], j = pj[m];
       const int lim = (i == 0) ? n - 1 : n;
       const bool ok = (w != 0xffffffffu) & ((unsigned)i < (unsigned)rows) & (j > i + 1) & (j < lim);
-      const int delta = dac[m] + gd[m] - s_nd[w == 0xffffffffu ? 0 : a].z - s_nd[w == 0xffffffffu ? 0 : c].z;
+      const int delta = XY ? dac[m] + gd[m]
+                           : dac[m] + gd[m] - s_nd[w == 0xffffffffu ? 0 : a].z - s_nd[w == 0xffffffffu ? 0 : c].z;
       my_deltas += ok;
       if (ok & (delta < 0)) {
         const uint32_t r = (uint32_t)(base + m * stride - a * k);
@@ -783,22 +813,31 @@ __device__ __forceinline__ void cluster_two_opt_tour(int32_t* __restrict__ tour,
     __syncthreads();
     for (int x = i + tid; x <= j; x += BLOCK) {
       const int u = s_tour[x], v = s_tour[x + 1];
-      *reinterpret_cast<int2*>(&s_nd[u]) = make_int2(x, v);  // its cost is refreshed next pass
+      if constexpr (XY) {
+        const int2 pu = s_xy[u], pv = s_xy[v];
+        s_nd[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
+      } else {
+        *reinterpret_cast<int2*>(&s_nd[u]) = make_int2(x, v);  // its cost is refreshed next pass
+      }
     }
     __syncthreads();
-    zlo = i;
-    zhi = j;
+    if constexpr (!XY) {
+      zlo = i;
+      zhi = j;
+    }
     ++moves;
   }
-  if (!failed && g == 0 && cost != nullptr && zlo <= zhi) {
-    refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo, zhi, dist);
-    __syncthreads();
+  if constexpr (!XY) {
+    if (!failed && g == 0 && cost != nullptr && zlo <= zhi) {
+      refresh_edge_costs<BLOCK>(s_nd, s_tour, zlo, zhi, dist);
+      __syncthreads();
+    }
   }
   if (!failed && g == 0) {
     for (int x = tid; x < n; x += BLOCK) tour[x] = s_tour[x];
     if (cost != nullptr) {
       int64_t part = 0;
-      for (int x = tid; x < n; x += BLOCK) part += s_nd[x].z;
+      for (int x = tid; x < n; x += BLOCK) part += XY ? s_nd[x].y : s_nd[x].z;
       const int64_t tot = block_sum_i64(part, red64);
       if (tid == 0) store_relaxed(reinterpret_cast<uint64_t*>(cost), (uint64_t)__double_as_longlong((double)tot));
     }

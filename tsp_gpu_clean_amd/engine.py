@@ -149,6 +149,9 @@ class DeviceMemeticLoop:
     self.xy = context.get_gpu_euc2d_coordinates() if coordinate_candidates_pay(self.n) else None
     if self.xy is not None and context.get_gpu_int_coordinates() is not None:
       self.xy = context.get_gpu_int_coordinates()  # integer instances: the integer distance form
+    # The elite clusters compute their distances from int32 coordinates where the instance has them
+    # (no matrix gathers in a pass).
+    self.ixy = context.get_gpu_int_coordinates()
     P, n = self.P, self.n
     dev = "cuda"
     self.pop = [torch.empty((P, n), dtype=torch.int32, device=dev) for _ in range(2)]
@@ -204,7 +207,7 @@ class DeviceMemeticLoop:
         _native.two_opt_packed_euc2d(elites, self.xy, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
       elif fused and self.clusters:
         _native.two_opt_elite(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E], self.sync,
-                              best_out)
+                              best_out, ixy=self.ixy)
         best_done = True
       elif fused:
         _native.two_opt_packed(elites, self.D, self.table, self.cand_k, self.ls_iters, self.stats, kb[:E])
