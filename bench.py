@@ -130,7 +130,8 @@ def c3_coords():
 def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
   """Returns (seconds per step, deltas per step, moves per step, kernel ms per launch, refined tours,
   passes per step).  With device coordinates `xy`, the candidate scan computes every distance from
-  them and the full scan 13 rows in 16 (tspmi_two_opt_full_euc2d)."""
+  them and the full scan 13 rows in 16, on the instance renumbered in Hilbert order inside the
+  timed call (tspmi_two_opt_full_euc2d_ordered)."""
   work = torch.empty_like(start)
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
   if cand is not None:
@@ -141,7 +142,8 @@ def time_two_opt(torch, native, D, start, cand, steps, warmup, xy=None):
     else:
       run = lambda: native.two_opt_packed(work, D, table, cand.shape[1], -1, stats)  # noqa: E731
   elif xy is not None:
-    run = lambda: native.two_opt_full_euc2d(work, D, xy, -1, stats)  # noqa: E731
+    ws = torch.empty(native.full_euc2d_ws_bytes(start.shape[1]), dtype=torch.uint8, device="cuda")
+    run = lambda: native.two_opt_full_euc2d(work, D, xy, -1, stats, ws=ws)  # noqa: E731
   else:
     run = lambda: native.two_opt(work, D, None, 0, -1, stats)  # noqa: E731
   for _ in range(warmup):

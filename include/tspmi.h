@@ -137,6 +137,22 @@ int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int
                         int64_t* stats, void* stream);
 
 /*
+ * tspmi_two_opt_full_euc2d on the instance renumbered in Hilbert-curve order of its coordinates:
+ * the nodes' new numbers follow a space-filling curve, so the scan's distance gathers of
+ * neighbouring tour positions fall into fewer cache lines once the tours are partly optimised.
+ * In one call, on `stream`: the order and its inverse, the renumbered matrix and coordinates
+ * (into ws), the tours renumbered in place, the scan, and the tours numbered back.  Renumbering
+ * changes no delta and no position, so the moves, tours and counters are tspmi_two_opt_full_euc2d's.
+ *   ws: device workspace of at least tspmi_two_opt_full_euc2d_ws_bytes(n) bytes, 256-byte aligned
+ *       (overwritten; contents need not be preserved between calls).
+ * Replaces: as tspmi_two_opt_full_euc2d.
+ */
+int tspmi_two_opt_full_euc2d_ordered(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy,
+                                     int max_iter, int64_t* stats, void* ws, size_t ws_bytes, void* stream);
+/* Bytes of workspace tspmi_two_opt_full_euc2d_ordered needs for n cities (n * n * 4 and a little). */
+size_t tspmi_two_opt_full_euc2d_ws_bytes(int n);
+
+/*
  * Packed candidate table for tspmi_two_opt_packed: out is (n, KP) pairs of int32
  * {N[u, r], D[u, N[u, r]]} (pairs {-1, 0} for k <= r < KP) with KP = 8, 16 or 32 — the smallest of
  * those >= k.  Build once per (instance, k); 1 <= k <= 32.

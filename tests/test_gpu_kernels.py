@@ -876,6 +876,35 @@ def test_two_opt_full_euc2d_matches_c_oracle(torch_mod, nat, n, P, iters, kind):
   assert stats.cpu().tolist() == list(wstats)
 
 
+@pytest.mark.parametrize("n,P,iters,kind", [(4, 3, -1, "uniform"), (10, 6, -1, "uniform"), (257, 3, -1, "lattice"),
+                                             (700, 3, 40, "float"), (1026, 3, 30, "stacked"), (2050, 2, 12, "span_limit"),
+                                             (1000, 64, -1, "uniform")])
+def test_two_opt_full_euc2d_ordered_equals_plain(torch_mod, nat, n, P, iters, kind):
+  """The Hilbert-renumbered scan (tspmi_two_opt_full_euc2d_ordered): the same tours and counters as
+  the scan on the instance as given, on integer, fractional, tie-heavy and span-limit instances;
+  a workspace one byte short is refused before the tours are touched."""
+  say(f"two_opt_full_euc2d_ordered n={n} P={P} iters={iters} {kind}")
+  xy = {"uniform": lambda: uniform_coords(n, seed=n + 21),
+        "float": lambda: np.random.default_rng(n).random((n, 2)) * 1000.0,
+        "lattice": lambda: lattice_coords(n),
+        "span_limit": lambda: span_limit_int_coords(n, seed=n + 1),
+        "stacked": lambda: uniform_coords(12, seed=n)[np.random.default_rng(n + 3).integers(0, 12, n)]}[kind]()
+  D, _ = device_dist(nat, torch_mod, xy)
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + 6), P))
+  a, b = cu(tours), cu(tours)
+  sa = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  sb = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+  nat.two_opt_full_euc2d(a, D, cu(xy), iters, sa)
+  ws = torch_mod.empty(nat.full_euc2d_ws_bytes(n), dtype=torch_mod.uint8, device="cuda")
+  nat.two_opt_full_euc2d(b, D, cu(xy), iters, sb, ws=ws)
+  np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy())
+  assert sb.cpu().tolist() == sa.cpu().tolist()
+  c = cu(tours)
+  with pytest.raises(ValueError, match="workspace"):
+    nat.two_opt_full_euc2d(c, D, cu(xy), iters, None, ws=ws[: ws.numel() - 1])
+  np.testing.assert_array_equal(c.cpu().numpy(), tours)
+
+
 def test_two_opt_full_euc2d_population_equals_matrix_kernel(torch_mod, nat):
   """P=1024, n=1000 to the local optimum: the same tours and counters as the matrix-only scan."""
   say("two_opt_full_euc2d P=1024 n=1000 to optimum")

@@ -39,6 +39,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_tour_cost": [_P, _I, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt": [_P, _I, _I, _P, _I, _P, _I, _I, _I, _P, _P],
   "tspmi_two_opt_full_euc2d": [_P, _I, _I, _P, _I, _P, _I, _P, _P],
+  "tspmi_two_opt_full_euc2d_ordered": [_P, _I, _I, _P, _I, _P, _I, _P, _P, ctypes.c_size_t, _P],
+  "tspmi_two_opt_full_euc2d_ws_bytes": [_I],
   "tspmi_candidate_table": [_P, _I, _I, _P, _I, _P, _P],
   "tspmi_two_opt_packed": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P],
   "tspmi_two_opt_packed_cost": [_P, _I, _I, _P, _I, _P, _I, _I, _P, _P, _P],
@@ -119,6 +121,7 @@ def load_library(path: str | None = None) -> Any:
       fn.restype = ctypes.c_int
     lib.tspmi_last_error.argtypes = []
     lib.tspmi_last_error.restype = ctypes.c_char_p
+    lib.tspmi_two_opt_full_euc2d_ws_bytes.restype = ctypes.c_size_t
     if path is None:
       _lib = lib
     return lib
@@ -595,9 +598,17 @@ def nn_init(seeds, customers, cand, work, out) -> None:
 FULL_EUC2D_MAX_N = 2050
 
 
-def two_opt_full_euc2d(tours, D, xy, max_iter: int = 10, stats=None) -> None:
-  """Full-neighbourhood best-improvement 2-opt on an EUC_2D instance: distances gathered from the
-  matrix ``D`` built from ``xy`` on most rows and computed from ``xy`` on the others (n <= 2050)."""
+def full_euc2d_ws_bytes(n: int) -> int:
+  """Workspace bytes of :func:`two_opt_full_euc2d` with ``ws`` (the renumbered instance)."""
+  return int(load_library().tspmi_two_opt_full_euc2d_ws_bytes(int(n)))
+
+
+def two_opt_full_euc2d(tours, D, xy, max_iter: int = 10, stats=None, ws=None) -> None:
+  """Full-neighbourhood best-improvement 2-opt on an EUC_2D instance: distances computed from
+  ``xy`` on most rows and gathered from the matrix ``D`` built from it on the others (n <= 2050).
+  With ``ws`` (a device byte tensor of at least :func:`full_euc2d_ws_bytes` bytes), the instance
+  is renumbered in Hilbert-curve order for the scan (tspmi_two_opt_full_euc2d_ordered: the same
+  moves, tours and counters, fewer texture-addresser cycles per gather)."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   _need(D, torch.int32, "D", 2)
@@ -607,6 +618,12 @@ def two_opt_full_euc2d(tours, D, xy, max_iter: int = 10, stats=None) -> None:
   P, n = tours.shape
   if xy.shape != (n, 2) or D.shape[0] < n or D.shape[1] < n:
     raise ValueError(f"two_opt_full_euc2d: xy {tuple(xy.shape)} / D {tuple(D.shape)} do not match n={n}")
+  if ws is not None:
+    _need(ws, torch.uint8, "ws", 1)
+    _check(_lib.tspmi_two_opt_full_euc2d_ordered(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(xy), int(max_iter),
+                                                 _ptr(stats), _ptr(ws), ws.numel(), _stream(torch)),
+           "tspmi_two_opt_full_euc2d_ordered")
+    return
   _check(_lib.tspmi_two_opt_full_euc2d(_ptr(tours), P, n, _ptr(D), D.shape[1], _ptr(xy), int(max_iter), _ptr(stats),
                                   _stream(torch)), "tspmi_two_opt_full_euc2d")
 

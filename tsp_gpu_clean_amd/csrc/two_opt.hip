@@ -1157,6 +1157,125 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   return TSPMI_OK;
 }
 
+namespace tspmi {
+
+// ---- Node labels in Hilbert-curve order ------------------------------------------------------
+//
+// The gathered rows of the full scan read D[t_{i+2}][t_{j+2}] for 64 consecutive positions j at
+// once; with the nodes numbered along a space-filling curve of their coordinates, the tour
+// neighbours of a partly optimised tour have nearby numbers, so a wave's 64 columns fall into fewer
+// cache lines and each gather costs the texture addresser fewer cycles.  Renumbering changes no
+// delta and no position, so the scan makes the same moves.
+
+// Index of (x, y) in [0, 2^16)^2 along the Hilbert curve of order 16.
+__device__ __forceinline__ uint32_t hilbert_index16(uint32_t x, uint32_t y) {
+  constexpr uint32_t kSide = 1u << 16;
+  uint32_t d = 0;
+  for (uint32_t s = kSide >> 1; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = kSide - 1 - x;
+        y = kSide - 1 - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
+constexpr int kOrderBlock = 1024;
+constexpr int kOrderMaxN = 8192;  // keys held in LDS by spatial_order_kernel
+
+// One workgroup: the coordinates' bounding box, each node's Hilbert index on a 2^16 grid over the
+// box (one scale for both axes), then order[r] = the node of rank r and rank[u] = the rank of u by
+// (index, node), a stable order.
+__global__ __launch_bounds__(kOrderBlock) void spatial_order_kernel(const double2* __restrict__ xy, int n,
+                                                                   int32_t* __restrict__ order,
+                                                                   int32_t* __restrict__ rank) {
+  __shared__ uint32_t s_key[kOrderMaxN];
+  __shared__ double s_box[4][kOrderBlock / kWave];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  double lo_x = __builtin_inf(), lo_y = __builtin_inf(), hi_x = -__builtin_inf(), hi_y = -__builtin_inf();
+  for (int u = tid; u < n; u += kOrderBlock) {
+    const double2 p = xy[u];
+    lo_x = fmin(lo_x, p.x);
+    lo_y = fmin(lo_y, p.y);
+    hi_x = fmax(hi_x, p.x);
+    hi_y = fmax(hi_y, p.y);
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    lo_x = fmin(lo_x, __shfl_xor(lo_x, m, kWave));
+    lo_y = fmin(lo_y, __shfl_xor(lo_y, m, kWave));
+    hi_x = fmax(hi_x, __shfl_xor(hi_x, m, kWave));
+    hi_y = fmax(hi_y, __shfl_xor(hi_y, m, kWave));
+  }
+  if (lane == 0) {
+    s_box[0][wid] = lo_x;
+    s_box[1][wid] = lo_y;
+    s_box[2][wid] = hi_x;
+    s_box[3][wid] = hi_y;
+  }
+  __syncthreads();
+  for (int w = 0; w < kOrderBlock / kWave; ++w) {
+    lo_x = fmin(lo_x, s_box[0][w]);
+    lo_y = fmin(lo_y, s_box[1][w]);
+    hi_x = fmax(hi_x, s_box[2][w]);
+    hi_y = fmax(hi_y, s_box[3][w]);
+  }
+  const double span = fmax(fmax(hi_x - lo_x, hi_y - lo_y), 1e-300);
+  const double scale = 65535.0 / span;
+  for (int u = tid; u < n; u += kOrderBlock) {
+    const double2 p = xy[u];
+    // NaN or infinite coordinates (an invalid input) land on cell 0 rather than anywhere undefined
+    const double fx = (p.x - lo_x) * scale, fy = (p.y - lo_y) * scale;
+    const uint32_t qx = fx >= 0.0 && fx <= 65535.0 ? (uint32_t)fx : 0u;
+    const uint32_t qy = fy >= 0.0 && fy <= 65535.0 ? (uint32_t)fy : 0u;
+    s_key[u] = hilbert_index16(qx, qy);
+  }
+  __syncthreads();
+  for (int u = tid; u < n; u += kOrderBlock) {
+    const uint32_t ku = s_key[u];
+    int r = 0;
+    for (int v = 0; v < n; ++v) {
+      const uint32_t kv = s_key[v];
+      r += (kv < ku || (kv == ku && v < u)) ? 1 : 0;
+    }
+    rank[u] = r;
+    order[r] = u;
+  }
+}
+
+// D2[r][c] = D[order[r]][order[c]] (row stride n) and xy2[r] = xy[order[r]].
+__global__ void permute_instance_kernel(const int32_t* __restrict__ D, int ld, const double2* __restrict__ xy, int n,
+                                        const int32_t* __restrict__ order, int32_t* __restrict__ D2,
+                                        double2* __restrict__ xy2) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx - (int64_t)r * n);
+  const int a = order[r], b = order[c];
+  D2[idx] = D[(size_t)a * ld + b];
+  if (r == 0) xy2[c] = xy[b];
+}
+
+// tours[x] = map[tours[x]] (ids outside [0, n) are read as node 0, as the scan itself does).
+__global__ void relabel_kernel(int32_t* __restrict__ tours, int64_t count, int n, const int32_t* __restrict__ map) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < count) tours[idx] = map[clamp_id(tours[idx], n)];
+}
+
+// Workspace layout of tspmi_two_opt_full_euc2d_ordered: D2 (n * n int32), xy2 (n double2), order
+// and rank (n int32 each), each part 256-byte aligned.
+__host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+inline size_t ordered_ws_bytes(int n) {
+  return align256((size_t)n * n * 4) + align256((size_t)n * 16) + 2 * align256((size_t)n * 4);
+}
+
+}  // namespace tspmi
+
 extern "C" int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int32_t* D, int ld, const double* xy,
                                    int max_iter, int64_t* stats, void* stream) {
   using namespace tspmi;
@@ -1179,6 +1298,55 @@ extern "C" int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int3
     hipLaunchKernelGGL(two_opt_mix_kernel<8>, dim3(P), dim3(kTwoOptBlock), lds, s, tours, n, D, ld, p, max_iter, pad,
                        stats);
   TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d launch");
+  return TSPMI_OK;
+}
+
+extern "C" size_t tspmi_two_opt_full_euc2d_ws_bytes(int n) { return n > 0 ? tspmi::ordered_ws_bytes(n) : 0; }
+
+extern "C" int tspmi_two_opt_full_euc2d_ordered(int32_t* tours, int P, int n, const int32_t* D, int ld,
+                                                const double* xy, int max_iter, int64_t* stats, void* ws,
+                                                size_t ws_bytes, void* stream) {
+  using namespace tspmi;
+  TSPMI_CHECK_ARG(P >= 0 && n >= 0, "tspmi_two_opt_full_euc2d_ordered: negative shape P=%d n=%d", P, n);
+  if (P == 0 || n < 4 || max_iter == 0) return TSPMI_OK;
+  TSPMI_CHECK_ARG(tours != nullptr && D != nullptr && xy != nullptr && ws != nullptr,
+                  "tspmi_two_opt_full_euc2d_ordered: null pointer");
+  TSPMI_CHECK_ARG(((uintptr_t)xy & 15) == 0 && ((uintptr_t)ws & 255) == 0,
+                  "tspmi_two_opt_full_euc2d_ordered: xy must be 16-byte and ws 256-byte aligned");
+  TSPMI_CHECK_ARG(ld >= n && (long)n * ld < kMaxMatrixElems, "tspmi_two_opt_full_euc2d_ordered: bad ld=%d for n=%d",
+                  ld, n);
+  TSPMI_CHECK_ARG(n <= kOrderMaxN && n - 2 <= kTwoOptWaves * 8 * kWave,
+                  "tspmi_two_opt_full_euc2d_ordered: n=%d above 2050", n);
+  TSPMI_CHECK_ARG(ws_bytes >= ordered_ws_bytes(n), "tspmi_two_opt_full_euc2d_ordered: workspace of %zu B, %zu needed",
+                  ws_bytes, ordered_ws_bytes(n));
+  {  // the scan's own LDS check, made before the tours are renumbered in place
+    const int q = n - 2 <= kTwoOptWaves * 4 * kWave ? 4 : 8;
+    const int pad = q * kTwoOptBlock + kWave;
+    const size_t lds = ((size_t)mix_coord_words(n, pad) + 2 * n + 1 + 2 * (size_t)pad) * sizeof(int32_t);
+    TSPMI_CHECK_ARG(lds + 1024 <= (size_t)kMaxLds, "tspmi_two_opt_full_euc2d_ordered: n=%d needs %zu B of LDS", n, lds);
+  }
+  char* w = static_cast<char*>(ws);
+  int32_t* D2 = reinterpret_cast<int32_t*>(w);
+  double2* xy2 = reinterpret_cast<double2*>(w + align256((size_t)n * n * 4));
+  int32_t* order = reinterpret_cast<int32_t*>(w + align256((size_t)n * n * 4) + align256((size_t)n * 16));
+  int32_t* rank = order + align256((size_t)n * 4) / 4;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(spatial_order_kernel, dim3(1), dim3(kOrderBlock), 0, s, reinterpret_cast<const double2*>(xy), n,
+                     order, rank);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d_ordered order launch");
+  const int64_t cells = (int64_t)n * n;
+  hipLaunchKernelGGL(permute_instance_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, D, ld,
+                     reinterpret_cast<const double2*>(xy), n, order, D2, xy2);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d_ordered permute launch");
+  const int64_t count = (int64_t)P * n;
+  const dim3 grid((unsigned)((count + 255) / 256));
+  hipLaunchKernelGGL(relabel_kernel, grid, dim3(256), 0, s, tours, count, n, rank);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d_ordered relabel launch");
+  const int rc = tspmi_two_opt_full_euc2d(tours, P, n, D2, n, reinterpret_cast<const double*>(xy2), max_iter, stats,
+                                          stream);
+  if (rc != TSPMI_OK) return rc;
+  hipLaunchKernelGGL(relabel_kernel, grid, dim3(256), 0, s, tours, count, n, order);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_full_euc2d_ordered relabel launch");
   return TSPMI_OK;
 }
 

@@ -58,6 +58,7 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       raise ValueError("a candidate scorer needs candidate_k in [1, 32]")
     self.last_stats: tuple[int, int, int] | None = None
     self._stats = None
+    self._full_ws = None  # workspace of the renumbered full scan
 
   def _warmup_impl(self, context) -> None:
     self.improve_batch(self._warmup_batch(context, rows=1), context)
@@ -122,7 +123,11 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       if cand is None and _MODES[self.mode] == TSPMI_MODE_BEST and n <= _native.FULL_EUC2D_MAX_N:
         xy = context.get_gpu_euc2d_coordinates()
       if xy is not None:  # same tours; 13 rows in 16 take their distances from the coordinates
-        _native.two_opt_full_euc2d(tours, dist, xy, self.max_iterations, self._stats)
+        # the scan on the instance renumbered in Hilbert order (same moves), in a workspace kept per n
+        need = _native.full_euc2d_ws_bytes(n)
+        if self._full_ws is None or self._full_ws.numel() < need:
+          self._full_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+        _native.two_opt_full_euc2d(tours, dist, xy, self.max_iterations, self._stats, ws=self._full_ws)
       else:
         _native.two_opt(tours, dist, cand, _MODES[self.mode], self.max_iterations, self._stats)
     self.kernel_launches += 1
