@@ -110,7 +110,9 @@ __device__ __forceinline__ int diag_slot_offset(int w, int q) {
   return kWave * (kTwoOptWaves * q + ((q & 1) ? (kTwoOptWaves - 1 - w) : w));
 }
 
-template <int QMAX, bool MIX = false, class XY = XyF64>
+// BYTES: the tour array holds byte offsets 4 * node (two_opt_mix_kernel), so a gather's column
+// offset is read ready-made and a row's base is D + offset * ld bytes.
+template <int QMAX, bool MIX = false, class XY = XyF64, bool BYTES = false>
 struct DiagScan {
   using T = typename XY::T;
   const int32_t* s_tour;
@@ -124,6 +126,15 @@ struct DiagScan {
   int carry[QMAX], gcur[QMAX], bd[QMAX];
 
   static constexpr int kSlotStride = 2 * kTwoOptBlock;
+  // node id and distance row of a tour entry
+  static __device__ __forceinline__ int node(int e) { return BYTES ? (int)((uint32_t)e >> 2) : e; }
+  __device__ __forceinline__ const int32_t* row_of(int e) const {
+    return BYTES ? reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(D) + (size_t)(uint32_t)e * ld)
+                 : D + (size_t)e * ld;
+  }
+  static __device__ __forceinline__ int at(const int32_t* __restrict__ row, int e) {
+    return BYTES ? *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(row) + (uint32_t)e) : row_at(row, e);
+  }
   __device__ __forceinline__ int jo(int q) const { return ((q & 1) ? jo_odd : jo_even) + (q >> 1) * kSlotStride; }
 
   // The lane's LDS positions at row i, per slot: row i + R reads entry [R] from them, a constant
@@ -167,14 +178,13 @@ struct DiagScan {
 #pragma unroll
       for (int q = 0; q < QL; ++q) gnext[q] = XY::scan_dist(pa, b.xy[q][R]);
     } else if constexpr (COORD) {
-      const T pa = s_xy[__builtin_amdgcn_readfirstlane(s_tour[r + 2])];
+      const T pa = s_xy[node(__builtin_amdgcn_readfirstlane(s_tour[r + 2]))];
 #pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = XY::scan_dist(pa, s_xy[b.tour[q][R]]);
+      for (int q = 0; q < QL; ++q) gnext[q] = XY::scan_dist(pa, s_xy[node(b.tour[q][R])]);
     } else {
-      const int a = __builtin_amdgcn_readfirstlane(s_tour[r + 2]);
-      const int32_t* __restrict__ rn = D + (size_t)a * ld;
+      const int32_t* __restrict__ rn = row_of(__builtin_amdgcn_readfirstlane(s_tour[r + 2]));
 #pragma unroll
-      for (int q = 0; q < QL; ++q) gnext[q] = row_at(rn, b.tour[q][R]);
+      for (int q = 0; q < QL; ++q) gnext[q] = at(rn, b.tour[q][R]);
     }
     const int neg_eab = opaque_i32(-s_edge[r]);  // one negation per row, then a three-operand add
 #pragma unroll
@@ -223,6 +233,7 @@ struct DiagScan {
 // gave the slot (row 0, then rows 1 .. (n - 2) - off(w, q) - 1) and takes every delta from the
 // matrix.  Four blocks of 64 rows per trip, branch-free (indices clamped into the tour, the key
 // of an inadmissible lane discarded), so the loads of the four are in flight together.
+template <bool BYTES>
 __device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                   const int32_t* __restrict__ D, int ld, int wid, int lane, int q,
                                                   int L) {
@@ -239,7 +250,13 @@ __device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const i
       const bool ok = i < last && j <= n - 1 && !(i == 0 && j == n - 1);
       const int ic = ok ? i : 0, jc = ok ? j : 2;  // (0, 2) is a pair of every tour with n >= 4
       const int a = s_tour[ic], b = s_tour[ic + 1], c = s_tour[jc], d = s_tour[jc + 1];
-      const int delta = D[(size_t)a * ld + c] + D[(size_t)b * ld + d] - s_edge[ic] - s_edge[jc];
+      const int dac = DiagScan<1, false, XyF64, BYTES>::at(
+          reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(D) + (BYTES ? (size_t)(uint32_t)a * ld
+                                                                                    : (size_t)a * ld * 4)), c);
+      const int dbd = DiagScan<1, false, XyF64, BYTES>::at(
+          reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(D) + (BYTES ? (size_t)(uint32_t)b * ld
+                                                                                    : (size_t)b * ld * 4)), d);
+      const int delta = dac + dbd - s_edge[ic] - s_edge[jc];
       const uint64_t key = ok ? pack_key(delta, (uint32_t)i * (uint32_t)n + (uint32_t)j) : kNoMove;
       best = key < best ? key : best;
     }
@@ -257,11 +274,11 @@ __device__ __forceinline__ uint64_t diag_best_key(const int32_t* s_tour, const i
 // lane whose slot minimum is within m + kSlack has its diagonal walked again with exact deltas by
 // its wave (diag_best_key) -- normally one lane in the workgroup -- and the smallest key among
 // them is the reference's first minimum in scan order.  red holds 2 * waves entries.
-template <int QMAX, bool MIX = false, class XY = XyF64>
+template <int QMAX, bool MIX = false, class XY = XyF64, bool BYTES = false>
 __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, const int32_t* s_edge, int n,
                                                         const int32_t* __restrict__ D, int ld, int wid, int lane,
                                                         uint64_t* red, const typename XY::T* s_xy = nullptr) {
-  DiagScan<QMAX, MIX, XY> sc;
+  DiagScan<QMAX, MIX, XY, BYTES> sc;
   sc.s_tour = s_tour;
   sc.s_edge = s_edge;
   sc.s_xy = s_xy;
@@ -272,13 +289,13 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
   sc.jo_even = 2 + diag_slot_offset(wid, 0) + lane;
   sc.jo_odd = 2 + diag_slot_offset(wid, 1) + lane;
   {
-    const int32_t* __restrict__ r0 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[0]) * ld;
-    const int32_t* __restrict__ r1 = D + (size_t)__builtin_amdgcn_readfirstlane(s_tour[1]) * ld;
+    const int32_t* __restrict__ r0 = sc.row_of(__builtin_amdgcn_readfirstlane(s_tour[0]));
+    const int32_t* __restrict__ r1 = sc.row_of(__builtin_amdgcn_readfirstlane(s_tour[1]));
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int j = sc.jo(q);
-      sc.carry[q] = row_at(r0, s_tour[j]) - s_edge[j];
-      sc.gcur[q] = row_at(r1, s_tour[j + 1]);
+      sc.carry[q] = sc.at(r0, s_tour[j]) - s_edge[j];
+      sc.gcur[q] = sc.at(r1, s_tour[j + 1]);
       sc.bd[q] = kFarDelta;
     }
   }
@@ -305,7 +322,7 @@ __device__ __forceinline__ uint64_t scan_full_best_diag(const int32_t* s_tour, c
     while (mask != 0) {
       const int L = __ffsll((unsigned long long)mask) - 1;
       mask &= mask - 1;
-      const uint64_t key = diag_best_key(s_tour, s_edge, n, D, ld, wid, lane, q, L);
+      const uint64_t key = diag_best_key<BYTES>(s_tour, s_edge, n, D, ld, wid, lane, q, L);
       best = key < best ? key : best;
     }
   }
@@ -757,8 +774,18 @@ __device__ __forceinline__ void apply_move_tour_xy(int32_t* s_tour, typename XY:
   __syncthreads();
 }
 
-// The pass loop of two_opt_mix_kernel with the coordinates staged in LDS in the form XY (s_tour
-// and its padding already in place).
+// The matrix read with byte-offset tour entries (4 * node): D[u][v] at byte u * ld + v.
+struct ByteMatrixDist {
+  const int32_t* __restrict__ D;
+  int ld;
+  __device__ __forceinline__ int operator()(int u4, int v4) const {
+    return *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(D) + (size_t)(uint32_t)u4 * ld +
+                                             (uint32_t)v4);
+  }
+};
+
+// The pass loop of two_opt_mix_kernel with the coordinates staged in LDS in the form XY (s_tour,
+// as byte offsets 4 * node, and its padding already in place).
 template <int QMAX, class XY>
 __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typename XY::T* s_xy, int32_t* s_tour,
                                            int32_t* s_edge, int n, int pad, const int32_t* __restrict__ D, int ld,
@@ -770,7 +797,7 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
   if constexpr (XY::kTourOrder) {
     for (int x = tid; x < n + 1 + pad; x += kTwoOptBlock) {
       // the mirror entry and the padding (node 0) included
-      s_xy[x] = XY::stage(xy[s_tour[x]], lo_x, lo_y);
+      s_xy[x] = XY::stage(xy[(uint32_t)s_tour[x] >> 2], lo_x, lo_y);
     }
   } else {
     for (int x = tid; x < n; x += kTwoOptBlock) s_xy[x] = xy[x];
@@ -779,7 +806,7 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
   for (int it = 0; max_iter < 0 || it < max_iter; ++it) {
     ++passes;
     const uint64_t best =
-        scan_full_best_diag<QMAX, true, XY>(s_tour, s_edge, n, D, ld, wid, lane, red, s_xy);
+        scan_full_best_diag<QMAX, true, XY, true>(s_tour, s_edge, n, D, ld, wid, lane, red, s_xy);
     if (best == kNoMove) break;
     const uint32_t idx = key_index(best);
     const int i = (int)(idx / (uint32_t)n);
@@ -788,7 +815,7 @@ __device__ __forceinline__ void mix_passes(const double2* __restrict__ xy, typen
     if constexpr (XY::kTourOrder)
       apply_move_tour_xy<XY>(s_tour, s_xy, s_edge, i, j);
     else
-      apply_move(s_tour, s_edge, nullptr, i, j, n, D, ld);
+      apply_move(s_tour, s_edge, nullptr, i, j, n, ByteMatrixDist{D, ld});
     ++moves;
   }
 }
@@ -806,13 +833,14 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
   extern __shared__ __align__(16) int32_t lds[];
   __shared__ uint64_t red[2 * kTwoOptWaves];
   __shared__ int s_box[4];
-  int32_t* s_tour = lds + mix_coord_words(n, pad);  // n + 1 + pad, after the coordinates
-  int32_t* s_edge = s_tour + n + 1 + pad;           // n + pad
+  // n + 1 + pad byte offsets 4 * node (the gathers' column offsets as stored), after the coordinates
+  int32_t* s_tour = lds + mix_coord_words(n, pad);
+  int32_t* s_edge = s_tour + n + 1 + pad;  // n + pad
   const int tid = threadIdx.x;
   int32_t* tour = tours + (size_t)blockIdx.x * n;
   const bool int_xy = int_coords_fit(xy, n, s_box);
   for (int x = tid; x < n; x += kTwoOptBlock) {
-    const int v = clamp_id(tour[x], n);
+    const int v = 4 * clamp_id(tour[x], n);
     s_tour[x] = v;
     if (x == 0) s_tour[n] = v;
   }
@@ -821,7 +849,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
     s_edge[n + x] = -(1 << 30);
   }
   __syncthreads();
-  for (int x = tid; x < n; x += kTwoOptBlock) s_edge[x] = D[(size_t)s_tour[x] * ld + s_tour[x + 1]];
+  for (int x = tid; x < n; x += kTwoOptBlock) s_edge[x] = ByteMatrixDist{D, ld}(s_tour[x], s_tour[x + 1]);
   int moves = 0, passes = 0;
   if (int_xy)
     mix_passes<QMAX, XyInt>(xy, reinterpret_cast<float2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
@@ -829,7 +857,7 @@ __global__ __launch_bounds__(kTwoOptBlock) void two_opt_mix_kernel(int32_t* __re
   else
     mix_passes<QMAX, XyF64>(xy, reinterpret_cast<double2*>(lds), s_tour, s_edge, n, pad, D, ld, max_iter, red, moves,
                             passes);
-  for (int x = tid; x < n; x += kTwoOptBlock) tour[x] = s_tour[x];
+  for (int x = tid; x < n; x += kTwoOptBlock) tour[x] = (int)((uint32_t)s_tour[x] >> 2);
   if (stats != nullptr && tid == 0) {
     atomicAdd((unsigned long long*)&stats[0], (unsigned long long)((int64_t)passes * n * (n - 3) / 2));
     atomicAdd((unsigned long long*)&stats[1], (unsigned long long)moves);
