@@ -272,6 +272,23 @@ int tspmi_two_opt_packed_euc2d_i32(int32_t* tours, int P, int n, const int32_t* 
                                    void* stream);
 
 /*
+ * tspmi_two_opt_packed_euc2d_i32 on the instance renumbered in Hilbert-curve order of its
+ * coordinates (as tspmi_two_opt_full_euc2d_ordered): in one call, the order (tspmi_rank_select of
+ * the nodes' curve indices), the renumbered coordinates and candidate table (into ws), the tours
+ * renumbered in place, the scan, and the tours numbered back.  Same moves, tours, counters and
+ * costs; the pass log's clock starts with the scan.  Shapes the renumbering does not apply to,
+ * and tours small enough for the scan's node records in LDS (n up to about 5700, where it gains
+ * nothing), go to tspmi_two_opt_packed_euc2d_i32 unchanged.
+ *   ws: device workspace of tspmi_two_opt_packed_euc2d_ws_bytes(n, k) bytes, 256-byte aligned.
+ * Replaces: as tspmi_two_opt_packed_euc2d_i32.
+ */
+int tspmi_two_opt_packed_euc2d_i32_ordered(int32_t* tours, int P, int n, const int32_t* ixy, const int32_t* table,
+                                           int k, int max_iter, int64_t* stats, double* cost, int64_t* pass_log,
+                                           int log_moves, void* ws, size_t ws_bytes, void* stream);
+/* Bytes of workspace tspmi_two_opt_packed_euc2d_i32_ordered needs (0 for k outside [1, 32]). */
+size_t tspmi_two_opt_packed_euc2d_ws_bytes(int n, int k);
+
+/*
  * Best-improvement 2-opt for instances too large for the LDS-resident kernels (candidate scan
  * above n ~ 13,500, full neighbourhood above n = 16,386): each tour's positions, edge costs and
  * (candidate mode) inverse positions live in the caller's workspace ws, device int32[P * (3n + 1)],

@@ -905,6 +905,34 @@ def test_two_opt_full_euc2d_ordered_equals_plain(torch_mod, nat, n, P, iters, ki
   np.testing.assert_array_equal(c.cpu().numpy(), tours)
 
 
+@pytest.mark.parametrize("n,P,k,iters,kind", [(40, 6, 8, -1, "uniform"), (1000, 24, 20, -1, "lattice"),
+                                               (1000, 300, 20, 30, "uniform"), (10000, 4, 8, 40, "span_limit")])
+def test_two_opt_packed_euc2d_ordered_equals_plain(torch_mod, nat, n, P, k, iters, kind):
+  """The Hilbert-renumbered candidate scan (tspmi_two_opt_packed_euc2d_i32_ordered): the tours,
+  counters, costs and per-move cost log of the scan on the instance as given."""
+  say(f"two_opt_packed_euc2d ordered n={n} P={P} k={k} {kind}")
+  xy = {"uniform": lambda: uniform_coords(n, seed=n + 31), "lattice": lambda: lattice_coords(n),
+        "span_limit": lambda: span_limit_int_coords(n, seed=n + 2)}[kind]()
+  D, _ = device_dist(nat, torch_mod, xy)
+  cand = torch_mod.empty((n, k), dtype=torch_mod.int32, device="cuda")
+  nat.knn(D, k, cand)
+  table = torch_mod.empty((n, nat.candidate_lanes(k), 2), dtype=torch_mod.int32, device="cuda")
+  nat.candidate_table(cand, D, table)
+  ixy = cu(xy.astype(np.int32))
+  tours = oinit.random_init(np.arange(n), P, oinit.draw_row_seeds(np.random.default_rng(n + 9), P))
+  ws = torch_mod.empty(nat.packed_euc2d_ws_bytes(n, k), dtype=torch_mod.uint8, device="cuda")
+  out = {}
+  for form in ("plain", "ordered"):
+    t = cu(tours)
+    st = torch_mod.zeros(3, dtype=torch_mod.int64, device="cuda")
+    cost = torch_mod.zeros(P, dtype=torch_mod.float64, device="cuda")
+    log = torch_mod.zeros((P, 6, 2), dtype=torch_mod.int64, device="cuda")
+    nat.two_opt_packed_euc2d_log(t, ixy, table, k, iters, log, st, cost, ws=ws if form == "ordered" else None)
+    out[form] = (t.cpu().numpy(), st.cpu().tolist(), cost.cpu().numpy(), log[:, :, 0].cpu().numpy())
+  for a, b in zip(out["plain"], out["ordered"]):
+    np.testing.assert_array_equal(np.asarray(b), np.asarray(a))
+
+
 def test_two_opt_full_euc2d_population_equals_matrix_kernel(torch_mod, nat):
   """P=1024, n=1000 to the local optimum: the same tours and counters as the matrix-only scan."""
   say("two_opt_full_euc2d P=1024 n=1000 to optimum")

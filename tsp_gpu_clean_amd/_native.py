@@ -79,6 +79,8 @@ _SIGNATURES: dict[str, list[Any]] = {
   "tspmi_two_opt_packed_euc2d_cost": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P],
   "tspmi_two_opt_packed_euc2d_log": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
   "tspmi_two_opt_packed_euc2d_i32": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P],
+  "tspmi_two_opt_packed_euc2d_i32_ordered": [_P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P, ctypes.c_size_t, _P],
+  "tspmi_two_opt_packed_euc2d_ws_bytes": [_I, _I],
   "tspmi_wall_clock_khz": [_P],
   "tspmi_two_opt_large": [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P],
   "tspmi_scorer_param_count": [],
@@ -122,6 +124,7 @@ def load_library(path: str | None = None) -> Any:
     lib.tspmi_last_error.argtypes = []
     lib.tspmi_last_error.restype = ctypes.c_char_p
     lib.tspmi_two_opt_full_euc2d_ws_bytes.restype = ctypes.c_size_t
+    lib.tspmi_two_opt_packed_euc2d_ws_bytes.restype = ctypes.c_size_t
     if path is None:
       _lib = lib
     return lib
@@ -642,11 +645,26 @@ def two_opt_euc2d(tours, xy, max_iter: int = 10, stats=None) -> None:
          "tspmi_two_opt_euc2d")
 
 
-def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None, cost=None) -> None:
+def packed_euc2d_ws_bytes(n: int, k: int) -> int:
+  """Workspace bytes of the renumbered candidate scan (``ws`` of :func:`two_opt_packed_euc2d`)."""
+  return int(load_library().tspmi_two_opt_packed_euc2d_ws_bytes(int(n), int(k)))
+
+
+def _ordered_call(tours, P, n, xy, table, k, max_iter, stats, cost, pass_log, log_moves, ws, torch) -> None:
+  _need(ws, torch.uint8, "ws", 1)
+  _check(_lib.tspmi_two_opt_packed_euc2d_i32_ordered(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
+                                                     _ptr(stats), _ptr(cost), _ptr(pass_log), int(log_moves),
+                                                     _ptr(ws), ws.numel(), _stream(torch)),
+         "tspmi_two_opt_packed_euc2d_i32_ordered")
+
+
+def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=None, cost=None, ws=None) -> None:
   """Candidate-list best-improvement 2-opt on a packed table, distances from EUC_2D coordinates:
   float64 ``xy``, or int32 ``xy`` (integer coordinates spanning at most 2^20 per axis, as
   ``ProblemContext.get_gpu_int_coordinates`` gives them) for the integer distance form; same
-  results.  ``cost`` as for :func:`two_opt_packed` (needs n >= 4 and max_iter != 0)."""
+  results.  ``cost`` as for :func:`two_opt_packed` (needs n >= 4 and max_iter != 0).  With int32
+  ``xy`` and ``ws`` (a device byte tensor of :func:`packed_euc2d_ws_bytes` bytes) the scan runs on
+  the instance renumbered in Hilbert order (same moves and tours)."""
   torch = require_device()
   _need(tours, torch.int32, "tours", 2)
   int_xy = isinstance(xy, torch.Tensor) and xy.dtype == torch.int32
@@ -661,7 +679,9 @@ def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=Non
     _need(cost, torch.float64, "cost", 1)
     if cost.shape[0] < P:
       raise ValueError("two_opt_packed_euc2d: cost shorter than the number of tours")
-  if int_xy:
+  if int_xy and ws is not None:
+    _ordered_call(tours, P, n, xy, table, k, max_iter, stats, cost, None, 0, ws, torch)
+  elif int_xy:
     _check(_lib.tspmi_two_opt_packed_euc2d_i32(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter),
                                                _ptr(stats), _ptr(cost), None, 0, _stream(torch)),
            "tspmi_two_opt_packed_euc2d_i32")
@@ -671,9 +691,11 @@ def two_opt_packed_euc2d(tours, xy, table, k: int, max_iter: int = 10, stats=Non
            "tspmi_two_opt_packed_euc2d_cost")
 
 
-def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, stats=None, cost=None) -> None:
+def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, stats=None, cost=None,
+                             ws=None) -> None:
   """:func:`two_opt_packed_euc2d` with a per-move progress log: ``pass_log`` int64 (P, M + 1, 2)
-  receives each tour's {cost, wall clock} at the start and after each of its first M moves."""
+  receives each tour's {cost, wall clock} at the start and after each of its first M moves (with
+  ``ws``, of the scan on the renumbered instance: the clock starts after the renumbering)."""
   torch = require_device()
   _need(pass_log, torch.int64, "pass_log", 3)
   P, n = tours.shape
@@ -689,6 +711,9 @@ def two_opt_packed_euc2d_log(tours, xy, table, k: int, max_iter: int, pass_log, 
     raise ValueError("two_opt_packed_euc2d_log: xy / table do not match n / k")
   if cost is not None:
     _need(cost, torch.float64, "cost", 1)
+  if int_xy and ws is not None:
+    _ordered_call(tours, P, n, xy, table, k, max_iter, stats, cost, pass_log, pass_log.shape[1] - 1, ws, torch)
+    return
   fn = _lib.tspmi_two_opt_packed_euc2d_i32 if int_xy else _lib.tspmi_two_opt_packed_euc2d_log
   _check(fn(_ptr(tours), P, n, _ptr(xy), _ptr(table), int(k), int(max_iter), _ptr(stats), _ptr(cost), _ptr(pass_log),
             pass_log.shape[1] - 1, _stream(torch)),

@@ -151,6 +151,26 @@ __device__ __forceinline__ int opaque_i32(int v) {
   return v;
 }
 
+// Index of (x, y) in [0, 2^16)^2 along the Hilbert curve of order 16.
+__device__ __forceinline__ uint32_t hilbert_index16(uint32_t x, uint32_t y) {
+  constexpr uint32_t kSide = 1u << 16;
+  uint32_t d = 0;
+  for (uint32_t s = kSide >> 1; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = kSide - 1 - x;
+        y = kSide - 1 - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
 // Node ids outside [0, limit) are an invalid input; clamping keeps every access in bounds.
 __device__ __forceinline__ int clamp_id(int v, int limit) { return (unsigned)v < (unsigned)limit ? v : 0; }
 

@@ -1185,6 +1185,11 @@ extern "C" int tspmi_two_opt(int32_t* tours, int P, int n, const int32_t* D, int
   return TSPMI_OK;
 }
 
+namespace {
+// Row width of the packed candidate table for k candidates (8, 16 or 32).
+int cand_lanes(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : 32); }
+}  // namespace
+
 namespace tspmi {
 
 // ---- Node labels in Hilbert-curve order ------------------------------------------------------
@@ -1194,26 +1199,6 @@ namespace tspmi {
 // neighbours of a partly optimised tour have nearby numbers, so a wave's 64 columns fall into fewer
 // cache lines and each gather costs the texture addresser fewer cycles.  Renumbering changes no
 // delta and no position, so the scan makes the same moves.
-
-// Index of (x, y) in [0, 2^16)^2 along the Hilbert curve of order 16.
-__device__ __forceinline__ uint32_t hilbert_index16(uint32_t x, uint32_t y) {
-  constexpr uint32_t kSide = 1u << 16;
-  uint32_t d = 0;
-  for (uint32_t s = kSide >> 1; s > 0; s >>= 1) {
-    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
-    d += s * s * ((3u * rx) ^ ry);
-    if (ry == 0) {
-      if (rx == 1) {
-        x = kSide - 1 - x;
-        y = kSide - 1 - y;
-      }
-      const uint32_t t = x;
-      x = y;
-      y = t;
-    }
-  }
-  return d;
-}
 
 constexpr int kOrderBlock = 1024;
 constexpr int kOrderMaxN = 8192;  // keys held in LDS by spatial_order_kernel
@@ -1329,7 +1314,129 @@ extern "C" int tspmi_two_opt_full_euc2d(int32_t* tours, int P, int n, const int3
   return TSPMI_OK;
 }
 
+namespace tspmi {
+
+// One workgroup: keys[u] = the Hilbert index of int32 coordinates xy[u] on a 2^16 grid over their
+// bounding box (one scale for both axes), as a double (exact below 2^53).
+__global__ __launch_bounds__(kOrderBlock) void hilbert_keys_i32_kernel(const int2* __restrict__ xy, int n,
+                                                                      double* __restrict__ keys) {
+  __shared__ int s_box[4];
+  const int tid = threadIdx.x;
+  if (tid < 4) s_box[tid] = 0x7fffffff;
+  __syncthreads();
+  int lo_x = 0x7fffffff, lo_y = 0x7fffffff, nhi_x = 0x7fffffff, nhi_y = 0x7fffffff;  // hi as -max
+  for (int u = tid; u < n; u += kOrderBlock) {
+    const int2 p = xy[u];
+    lo_x = min(lo_x, p.x);
+    lo_y = min(lo_y, p.y);
+    nhi_x = min(nhi_x, -p.x);
+    nhi_y = min(nhi_y, -p.y);
+  }
+  atomicMin(&s_box[0], lo_x);
+  atomicMin(&s_box[1], lo_y);
+  atomicMin(&s_box[2], nhi_x);
+  atomicMin(&s_box[3], nhi_y);
+  __syncthreads();
+  const double span = fmax(fmax((double)-s_box[2] - s_box[0], (double)-s_box[3] - s_box[1]), 1.0);
+  const double scale = 65535.0 / span;
+  for (int u = tid; u < n; u += kOrderBlock) {
+    const int2 p = xy[u];
+    const uint32_t qx = (uint32_t)(((double)p.x - s_box[0]) * scale);
+    const uint32_t qy = (uint32_t)(((double)p.y - s_box[1]) * scale);
+    keys[u] = (double)hilbert_index16(qx, qy);
+  }
+}
+
+// rank[order[r]] = r.
+__global__ void invert_order_kernel(const int32_t* __restrict__ order, int n, int32_t* __restrict__ rank) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) rank[order[r]] = r;
+}
+
+// The renumbered candidate table and coordinates: row r is node order[r], its neighbours renamed by
+// rank (the padding entries, neighbour -1, kept); xy2[r] = xy[order[r]].
+__global__ void permute_table_kernel(const int2* __restrict__ table, int kp, int n, const int32_t* __restrict__ order,
+                                     const int32_t* __restrict__ rank, const int2* __restrict__ xy,
+                                     int2* __restrict__ table2, int2* __restrict__ xy2) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * kp) return;
+  const int r = idx / kp, q = idx - r * kp;
+  const int u = order[r];
+  int2 e = table[(size_t)u * kp + q];
+  e.x = (unsigned)e.x < (unsigned)n ? rank[e.x] : e.x;
+  table2[idx] = e;
+  if (q == 0) xy2[r] = xy[u];
+}
+
+// Workspace of tspmi_two_opt_packed_euc2d_i32_ordered: keys (n double), order and rank (n int32
+// each), coordinates (n int2) and table (n * kp int2), each part 256-byte aligned.
+inline size_t packed_ordered_ws_bytes(int n, int kp) {
+  return align256((size_t)n * 8) + 2 * align256((size_t)n * 4) + align256((size_t)n * 8) +
+         align256((size_t)n * kp * 8);
+}
+
+}  // namespace tspmi
+
 extern "C" size_t tspmi_two_opt_full_euc2d_ws_bytes(int n) { return n > 0 ? tspmi::ordered_ws_bytes(n) : 0; }
+
+extern "C" size_t tspmi_two_opt_packed_euc2d_ws_bytes(int n, int k) {
+  return n > 0 && k >= 1 && k <= 32 ? tspmi::packed_ordered_ws_bytes(n, cand_lanes(k)) : 0;
+}
+
+extern "C" int tspmi_two_opt_packed_euc2d_i32_ordered(int32_t* tours, int P, int n, const int32_t* ixy,
+                                                      const int32_t* table, int k, int max_iter, int64_t* stats,
+                                                      double* cost, int64_t* pass_log, int log_moves, void* ws,
+                                                      size_t ws_bytes, void* stream) {
+  using namespace tspmi;
+  // Shapes the renumbering does not apply to (and every argument error) go to the plain call as
+  // given, and so do tours whose node records fit LDS: that scan reads no coordinates from global
+  // memory, and renumbering measured no gain there (n = 1000 to 5000); above, at n = 10000, P = 256,
+  // k = 8 / 20, it saves 9 / 10 %.
+  if (P <= 0 || n < 4 || max_iter == 0 || k < 1 || k > 32 || ws == nullptr || tours == nullptr || ixy == nullptr ||
+      table == nullptr || (size_t)rec_lds_words(n) * sizeof(int32_t) + 1024 <= (size_t)kMaxLds)
+    return tspmi_two_opt_packed_euc2d_i32(tours, P, n, ixy, table, k, max_iter, stats, cost, pass_log, log_moves,
+                                          stream);
+  const int kp = cand_lanes(k);
+  TSPMI_CHECK_ARG(((uintptr_t)ws & 255) == 0, "tspmi_two_opt_packed_euc2d_i32_ordered: ws must be 256-byte aligned");
+  TSPMI_CHECK_ARG(ws_bytes >= packed_ordered_ws_bytes(n, kp),
+                  "tspmi_two_opt_packed_euc2d_i32_ordered: workspace of %zu B, %zu needed", ws_bytes,
+                  packed_ordered_ws_bytes(n, kp));
+  TSPMI_CHECK_ARG(((uintptr_t)ixy & 7) == 0 && ((uintptr_t)table & 7) == 0,
+                  "tspmi_two_opt_packed_euc2d_i32_ordered: xy and table must be 8-byte aligned");
+  TSPMI_CHECK_ARG((size_t)(3 * n + 1) * sizeof(int32_t) + 1024 <= (size_t)kMaxLds,
+                  "tspmi_two_opt_packed_euc2d_i32_ordered: n=%d exceeds the LDS limit", n);
+  TSPMI_CHECK_ARG(pass_log == nullptr || log_moves >= 0, "tspmi_two_opt_packed_euc2d_i32_ordered: bad pass log");
+  char* w = static_cast<char*>(ws);
+  double* keys = reinterpret_cast<double*>(w);
+  int32_t* order = reinterpret_cast<int32_t*>(w + align256((size_t)n * 8));
+  int32_t* rank = reinterpret_cast<int32_t*>(w + align256((size_t)n * 8) + align256((size_t)n * 4));
+  int2* xy2 = reinterpret_cast<int2*>(w + align256((size_t)n * 8) + 2 * align256((size_t)n * 4));
+  int2* table2 = reinterpret_cast<int2*>(w + align256((size_t)n * 8) + 2 * align256((size_t)n * 4) +
+                                         align256((size_t)n * 8));
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(hilbert_keys_i32_kernel, dim3(1), dim3(kOrderBlock), 0, s, reinterpret_cast<const int2*>(ixy), n,
+                     keys);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d_i32_ordered keys launch");
+  int rc = tspmi_rank_select(keys, n, n, order, stream);
+  if (rc != TSPMI_OK) return rc;
+  hipLaunchKernelGGL(invert_order_kernel, dim3((n + 255) / 256), dim3(256), 0, s, order, n, rank);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d_i32_ordered rank launch");
+  hipLaunchKernelGGL(permute_table_kernel, dim3((n * kp + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<const int2*>(table), kp, n, order, rank, reinterpret_cast<const int2*>(ixy),
+                     table2, xy2);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d_i32_ordered permute launch");
+  const int64_t count = (int64_t)P * n;
+  const dim3 grid((unsigned)((count + 255) / 256));
+  hipLaunchKernelGGL(relabel_kernel, grid, dim3(256), 0, s, tours, count, n, rank);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d_i32_ordered relabel launch");
+  rc = tspmi_two_opt_packed_euc2d_i32(tours, P, n, reinterpret_cast<const int32_t*>(xy2),
+                                      reinterpret_cast<const int32_t*>(table2), k, max_iter, stats, cost, pass_log,
+                                      log_moves, stream);
+  if (rc != TSPMI_OK) return rc;
+  hipLaunchKernelGGL(relabel_kernel, grid, dim3(256), 0, s, tours, count, n, order);
+  TSPMI_CHECK_LAUNCH("tspmi_two_opt_packed_euc2d_i32_ordered relabel launch");
+  return TSPMI_OK;
+}
 
 extern "C" int tspmi_two_opt_full_euc2d_ordered(int32_t* tours, int P, int n, const int32_t* D, int ld,
                                                 const double* xy, int max_iter, int64_t* stats, void* ws,
@@ -1378,9 +1485,6 @@ extern "C" int tspmi_two_opt_full_euc2d_ordered(int32_t* tours, int P, int n, co
   return TSPMI_OK;
 }
 
-namespace {
-int cand_lanes(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : 32); }
-}  // namespace
 
 extern "C" int tspmi_candidate_table(const int32_t* cand, int n, int k, const int32_t* D, int ld, int32_t* out,
                                      void* stream) {

@@ -282,11 +282,33 @@ def ls_trace(tours0, D, table=None, k: int = 0, chunk: int = 50, limit_s: float 
   return trace, tours
 
 
-def ls_logged(tours0, xy, table, k: int, log_moves: int = 16384):
+def renumber_seconds(tours0, xy, table, k: int, ws, reps: int = 3) -> float:
+  """Device seconds a renumbered candidate scan spends before its first pass (the Hilbert order,
+  the permuted table and coordinates, the tours renumbered in and out): a one-pass call with the
+  workspace less the same call without it, best of ``reps`` each, at least 0."""
+  torch = _native.require_device()
+
+  def one(w) -> float:
+    best = float("inf")
+    for _ in range(reps):
+      t = tours0.clone()
+      e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+      e0.record()
+      _native.two_opt_packed_euc2d(t, xy, table, k, 1, None, ws=w)
+      e1.record()
+      torch.cuda.synchronize()
+      best = min(best, e0.elapsed_time(e1) * 1e-3)
+    return best
+
+  return max(0.0, one(ws) - one(None))
+
+
+def ls_logged(tours0, xy, table, k: int, log_moves: int = 16384, ws=None):
   """Candidate 2-opt (best improvement, distances from coordinates) to the local optimum in one
   launch with a per-move progress log.  Returns (log, tours, launch seconds, clock Hz): log[p, m]
   = {cost, device clock} of tour p at its start (m = 0) and after its m-th move (-1 past its
-  last move)."""
+  last move).  With ``ws`` the scan runs on the renumbered instance (the log's clock starts with
+  the scan; :func:`renumber_seconds` gives what comes before it)."""
   torch = _native.require_device()
   tours = tours0.clone()
   P = int(tours.shape[0])
@@ -294,7 +316,7 @@ def ls_logged(tours0, xy, table, k: int, log_moves: int = 16384):
   stats = torch.zeros(3, dtype=torch.int64, device="cuda")
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  _native.two_opt_packed_euc2d_log(tours, xy, table, k, -1, log, stats)
+  _native.two_opt_packed_euc2d_log(tours, xy, table, k, -1, log, stats, ws=ws)
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   host = log.cpu().numpy()
@@ -392,8 +414,13 @@ def evaluate_gap(n: int = 10000, P: int = 256, scorer=None, k_learned: int = 8, 
     progress("full", res["full"])
   for name, (tab, k, prep) in tables.items():
     if coords:
-      # one launch to the local optimum, the population's best known after every move
-      log, tours, wall, hz = ls_logged(start, scan_xy, tab, k)
+      # one launch to the local optimum, the population's best known after every move; integer
+      # instances on the renumbered instance, its renumbering counted with the candidate set-up
+      ws = None
+      if scan_xy.dtype == torch.int32:
+        ws = torch.empty(_native.packed_euc2d_ws_bytes(n, k), dtype=torch.uint8, device="cuda")
+        prep += renumber_seconds(start, scan_xy, tab, k, ws)
+      log, tours, wall, hz = ls_logged(start, scan_xy, tab, k, ws=ws)
       cost = torch.empty(P, dtype=torch.float64, device="cuda")
       _native.tour_cost(tours, D, cost)
       final = cost.cpu().numpy()

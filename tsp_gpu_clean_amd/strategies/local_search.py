@@ -59,6 +59,7 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
     self.last_stats: tuple[int, int, int] | None = None
     self._stats = None
     self._full_ws = None  # workspace of the renumbered full scan
+    self._cand_ws = None  # workspace of the renumbered candidate scan
 
   def _warmup_impl(self, context) -> None:
     self.improve_batch(self._warmup_batch(context, rows=1), context)
@@ -112,8 +113,14 @@ class LocalSearchGPUTourTwoOpt(StrategyBase):
       if large:  # the tour arrays in a global workspace (two_opt.py:75-94)
         _native.two_opt_large(tours, dist, xy, table, int(cand.shape[1]), self.max_iterations, self._stats)
       elif xy is not None:  # same tours; distances from the coordinates
+        ws = None
+        if ixy is not None:  # large tours: the scan on the instance renumbered in Hilbert order
+          need = _native.packed_euc2d_ws_bytes(n, int(cand.shape[1]))
+          if self._cand_ws is None or self._cand_ws.numel() < need:
+            self._cand_ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+          ws = self._cand_ws
         _native.two_opt_packed_euc2d(tours, ixy if ixy is not None else xy, table, int(cand.shape[1]),
-                                     self.max_iterations, self._stats)
+                                     self.max_iterations, self._stats, ws=ws)
       else:
         _native.two_opt_packed(tours, dist, table, int(cand.shape[1]), self.max_iterations, self._stats)
     elif large and cand is None:
