@@ -255,11 +255,8 @@ __device__ __forceinline__ void cand_two_opt_tour(int32_t* __restrict__ tour, in
   }
 }
 
-// int32 words of LDS that rec_two_opt_tour uses for an n-city tour, with or without the copy of
-// the coordinates.
-__host__ __device__ constexpr int rec_lds_words(int n, bool coords_lds = true) {
-  return coords_lds ? 7 * n + 1 : 5 * n + 1;
-}
+// int32 words of LDS that rec_two_opt_tour uses for an n-city tour.
+__host__ __device__ constexpr int rec_lds_words(int n) { return 7 * n + 1; }
 
 // Candidate-list steepest 2-opt for EUC_2D instances with int32 coordinates whose per-axis span is
 // at most kIntCoordSpan, with node-indexed records in LDS.
@@ -274,10 +271,9 @@ __host__ __device__ constexpr int rec_lds_words(int n, bool coords_lds = true) {
 // position-to-node lookups.  The evaluation is branch-free and a lane keeps the smallest whole key
 // (delta, i * k + r): the nodes are not visited in scan order, and the key restores the
 // reference's order.  A move reverses the tour between i+1 and j and rewrites the records of the
-// nodes at positions i .. j, from the coordinates: a copy in LDS with COORDS_LDS, else global
-// memory (less LDS, one round of global reads per move).
-// LDS: rec_lds_words(n, COORDS_LDS) int32 words, 16-byte aligned.
-template <int BLOCK, int KP, int UNROLL, int KL = KP, bool COORDS_LDS = true>
+// nodes at positions i .. j from the coordinates' copy in LDS.
+// LDS: rec_lds_words(n) int32 words, 16-byte aligned.
+template <int BLOCK, int KP, int UNROLL, int KL = KP>
 __device__ __forceinline__ void rec_two_opt_tour(int32_t* __restrict__ tour, int n, const int2* __restrict__ xy,
                                                  const int2* __restrict__ ct, int k, int max_iter,
                                                  int64_t* __restrict__ stats, int32_t* lds, uint64_t* red,
@@ -288,21 +284,19 @@ __device__ __forceinline__ void rec_two_opt_tour(int32_t* __restrict__ tour, int
   constexpr int NW = BLOCK / kWave;
   constexpr int RPW = kWave / KL;  // nodes per wave step
   int4* s_rec = reinterpret_cast<int4*>(lds);  // n
-  int2* s_xy = reinterpret_cast<int2*>(lds + 4 * n);             // n, with COORDS_LDS
-  int32_t* s_tour = lds + (COORDS_LDS ? 6 * n : 4 * n);          // n + 1 (s_tour[n] mirrors s_tour[0])
-  const int2* __restrict__ cxy = COORDS_LDS ? s_xy : xy;          // where the records are built from
+  const int2* s_xy = reinterpret_cast<const int2*>(lds + 4 * n);  // n
+  int32_t* s_tour = lds + 6 * n;                                    // n + 1 (s_tour[n] mirrors s_tour[0])
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = wave_id();
   const int sub = lane / KL;      // node offset inside the wave step (RPW for a spare lane)
   const int r = lane - sub * KL;  // candidate rank handled by this lane
   load_tour_lds<BLOCK>(src != nullptr ? src : tour, n, s_tour);
-  if constexpr (COORDS_LDS)
-    for (int x = tid; x < n; x += BLOCK) s_xy[x] = xy[x];
+  for (int x = tid; x < n; x += BLOCK) reinterpret_cast<int2*>(lds + 4 * n)[x] = xy[x];
   __syncthreads();
   for (int x = tid; x < n; x += BLOCK) {
     const int u = s_tour[x], v = s_tour[x + 1];
-    const int2 pu = cxy[u], pv = cxy[v];
+    const int2 pu = s_xy[u], pv = s_xy[v];
     s_rec[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
   }
   __syncthreads();
@@ -372,7 +366,7 @@ __device__ __forceinline__ void rec_two_opt_tour(int32_t* __restrict__ tour, int
     __syncthreads();
     for (int x = i + tid; x <= j; x += BLOCK) {
       const int u = s_tour[x], v = s_tour[x + 1];
-      const int2 pu = cxy[u], pv = cxy[v];
+      const int2 pu = s_xy[u], pv = s_xy[v];
       s_rec[u] = make_int4(x, euc2d_nint_i32(pu.x, pu.y, pv.x, pv.y), pv.x, pv.y);
     }
     __syncthreads();
