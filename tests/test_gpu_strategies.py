@@ -264,17 +264,20 @@ def test_device_loop_graph_replays_match_eager(t):
   loop.verify()
 
 
-def test_local_search_strategy_large_n_computes_distances_from_coordinates(t):
-  """n = 5000 > 4096: the strategy's candidate scan reads coordinates, with the matrix kernel's tours."""
+@pytest.mark.parametrize("n", [5000, 8000])
+def test_local_search_strategy_large_n_computes_distances_from_coordinates(t, n):
+  """Integer instances: the strategy's candidate scan reads coordinates (node records in LDS at
+  n = 5000; above, the instance renumbered in Hilbert order inside the call), with the matrix
+  kernel's tours."""
   import torch
 
   from tsp_gpu_clean_amd import _native
 
-  say("LocalSearchGPUTourTwoOpt n=5000 k=8 (coordinate candidate scan)")
+  say(f"LocalSearchGPUTourTwoOpt n={n} k=8 (coordinate candidate scan)")
   rng = np.random.default_rng(31)
-  xy = rng.integers(0, 1_000_001, (5000, 2)).astype(np.float64)
-  ctx = t.ProblemContext(t.Problem.from_coordinates("u5000", xy), seed=2)
-  routes = np.stack([rng.permutation(5000) for _ in range(4)]).astype(np.int32)
+  xy = rng.integers(0, 1_000_001, (n, 2)).astype(np.float64)
+  ctx = t.ProblemContext(t.Problem.from_coordinates(f"u{n}", xy), seed=2)
+  routes = np.stack([rng.permutation(n) for _ in range(4)]).astype(np.int32)
   ls = t.LocalSearchGPUTourTwoOpt(max_iterations=40, mode="best", candidate_k=8)
   got = ls.improve_batch(routes, ctx)
   want = torch.from_numpy(routes).cuda()
